@@ -1,0 +1,249 @@
+/*
+ * nngp.h — C ABI of the MI355X (gfx950) NNGP library, libnngp_hip.so.
+ *
+ * Drop-in boundary for spNNGP 1.0.1's NNGP hot path.  Every entry point takes
+ * plain pointers and sizes (no R, no torch types).  Matrices keep the
+ * reference's column-major FP64 layouts and int32 index layouts, so an R
+ * `.Call` shim is pure pointer pass-through.
+ *
+ * Conventions shared by all entry points
+ *   - n sites, m = n.neighbors, p covariates, m_i = min(i, m).
+ *   - coords: n x 2 column-major (x in [0,n), y in [n,2n)), already in NNGP
+ *     order for the fitting routines (R/spNNGP.R:144).
+ *   - nnIndx / nnIndxLU: flat neighbor layout of src/util.cpp:55-70; site i's
+ *     neighbors are nnIndx[nnIndxLU[i] .. nnIndxLU[i]+nnIndxLU[n+i]).
+ *     nIndx = m(m+1)/2 + (n-m-1)m  (src/rNNGP.cpp:218).
+ *   - covModel: 0 exponential, 1 spherical, 2 matern, 3 gaussian
+ *     (src/util.cpp:192-206).
+ *   - theta for the response model: {sigma.sq, tau.sq, phi[, nu]}
+ *     (src/rNNGP.cpp:182-190).
+ *   - Return value: NNGP_OK (0) or an NNGP_ERR_* code; nngp_last_error()
+ *     then holds a message.  Where the reference raises Rf_error the message
+ *     starts with the reference's text, e.g. "c++ Rf_error: dpotrf failed".
+ *   - Caller owns every host buffer.  The library owns device memory; every
+ *     call allocates what it needs and frees it before returning, except the
+ *     nngp_model_* handle, which keeps the fitted data resident in HBM.
+ *   - Calls on one nngp_model are not thread-safe; distinct handles may be
+ *     used from distinct host threads.
+ */
+#ifndef NNGP_H
+#define NNGP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum nngp_status {
+  NNGP_OK = 0,
+  NNGP_ERR_ARG = 1,         /* bad size, null pointer, malformed nnIndxLU   */
+  NNGP_ERR_NOT_PD = 2,      /* a neighbor covariance failed Cholesky        */
+  NNGP_ERR_COV_MODEL = 3,   /* covModel outside 0..3                        */
+  NNGP_ERR_HIP = 4,         /* HIP runtime error (no device, launch, copy)  */
+  NNGP_ERR_NOMEM = 5,       /* device or host allocation failed             */
+  NNGP_ERR_INTERRUPT = 6,   /* the caller's interrupt callback said stop    */
+  NNGP_ERR_UNSUPPORTED = 7  /* configuration the library does not handle    */
+};
+
+enum nngp_precision {
+  NNGP_FP64 = 0,            /* plain FP64: the reference's arithmetic       */
+  NNGP_MIXED = 1            /* FP32 factor/solve + one FP64 refinement step */
+};
+
+/* Host random number source.  The library never draws random numbers on
+ * the device for the MCMC; it calls these between kernels, in the order of
+ * src/rNNGP.cpp (beta normals, proposal normals, one uniform).  An R shim
+ * passes wrappers of norm_rand()/unif_rand() between GetRNGstate() and
+ * PutRNGstate().  NULL selects the library's own generator, seeded by
+ * nngp_opts.seed. */
+typedef struct nngp_rng {
+  void *state;
+  double (*norm_rand)(void *state);
+  double (*unif_rand)(void *state);
+} nngp_rng;
+
+/* Per-call options.  NULL means all defaults (FP64, device 0, seed 1). */
+typedef struct nngp_opts {
+  int precision;            /* enum nngp_precision                          */
+  int device;               /* HIP device ordinal                           */
+  unsigned long long seed;  /* seed of the built-in generator (rng == NULL) */
+  int (*interrupt)(void *); /* polled once per MCMC iteration / grid point;
+                               nonzero stops with NNGP_ERR_INTERRUPT        */
+  void *interrupt_arg;
+  int reserved[8];
+} nngp_opts;
+
+/* Message for the last nonzero status returned on this host thread. */
+const char *nngp_last_error(void);
+
+/* Library version, e.g. 10000 for 1.0.0. */
+int nngp_version(void);
+
+/* Number of HIP devices visible; 0 when none (never an error). */
+int nngp_device_count(void);
+
+/* ------------------------------------------------------------------------
+ * Neighbor search
+ * --------------------------------------------------------------------- */
+
+/* mkNNIndxCB — replaces src/nn.cpp:214-257 (.Call by name from R/nn.R:71).
+ * Exact m nearest neighbors of every site among its NNGP predecessors
+ * (sites with a smaller index), ascending by squared distance
+ * fl(fl(dx*dx)+fl(dy*dy)); nnDist holds the Euclidean distances.
+ * Fills caller-allocated nnIndx[nIndx], nnDist[nIndx], nnIndxLU[2n] in
+ * place.  nThreads is accepted for signature parity and ignored. */
+int nngp_mkNNIndxCB(int n, int m, const double *coords, int *nnIndx,
+                    double *nnDist, int *nnIndxLU, int nThreads);
+
+/* mkNNIndx — replaces the brute-force twin src/nn.cpp:63-111.  Same output
+ * as nngp_mkNNIndxCB (the two differ in the reference only in how exact
+ * distance ties are broken). */
+int nngp_mkNNIndx(int n, int m, const double *coords, int *nnIndx,
+                  double *nnDist, int *nnIndxLU, int nThreads);
+
+/* nn2 replacement (RANN, called at R/generics.R:642, R/spConjNNGP.R:158,311):
+ * exact k nearest fitted sites of each of q query sites, no ordering
+ * constraint.  nnIndx0 is q x k column-major, 0-based (the R code's
+ * `nn2(...)$nn.idx - 1`); nnDist0 (may be NULL) likewise. */
+int nngp_nn2(int n, const double *coords, int q, const double *coords0,
+             int k, int *nnIndx0, double *nnDist0);
+
+/* ------------------------------------------------------------------------
+ * Numerical kernels (src/rNNGP.cpp:46-101, src/cNNGP.cpp:24-69,
+ * src/util.cpp:261-280), host buffers in and out.
+ * --------------------------------------------------------------------- */
+
+/* updateBF — replaces src/rNNGP.cpp:46-101.  theta = {sigma.sq, tau.sq,
+ * phi[, nu]}.  Writes B[nIndx], F[n]; *logDet = sum_i log F_i.  B is not
+ * written for site 0 (it has no neighbors). */
+int nngp_updateBF(double *B, double *F, const double *coords,
+                  const int *nnIndx, const int *nnIndxLU, int n, int m,
+                  const double *theta, int covModel, double *logDet,
+                  const nngp_opts *opts);
+
+/* updateConjBF — replaces src/cNNGP.cpp:24-69 (correlation scale:
+ * C = R + alpha I, F = 1 + alpha - B.c). */
+int nngp_updateConjBF(double *B, double *F, const double *coords,
+                      const int *nnIndx, const int *nnIndxLU, int n, int m,
+                      double phi, double alpha, double nu, int covModel,
+                      const nngp_opts *opts);
+
+/* Q — replaces src/util.cpp:261-280:
+ * sum_i (u_i - B_i.u_N(i)) (v_i - B_i.v_N(i)) / F_i. */
+int nngp_Q(const double *B, const double *F, const double *u,
+           const double *v, int n, int m, const int *nnIndx,
+           const int *nnIndxLU, double *q);
+
+/* ------------------------------------------------------------------------
+ * .Call drivers (the argument lists follow the SEXP order of
+ * src/spNNGP.h; outputs are caller-allocated)
+ * --------------------------------------------------------------------- */
+
+/* rNNGP — replaces src/rNNGP.cpp:105-420 (.Call at R/spNNGP.R:401-405).
+ * Response-model Metropolis-within-Gibbs.  The 28 reference arguments in
+ * order, then:
+ *   rng             host random source (NULL: built-in, opts->seed)
+ *   pBetaSamples    p x nSamples        (column s = iteration s)
+ *   pThetaSamples   nTheta x nSamples   (nTheta = 3, or 4 for matern)
+ *   yRepSamples     n x nRep, or NULL when nRep == 0
+ *   nAccept         optional: number of accepted proposals
+ * verbose prints the reference's progress lines on stdout. */
+int nngp_rNNGP(const double *y, const double *X, int p, int n, int m,
+               const double *coords, int covModel, const int *nnIndx,
+               const int *nnIndxLU, const double *sigmaSqIG,
+               const double *tauSqIG, const double *phiUnif,
+               const double *nuUnif, const double *betaStarting,
+               double sigmaSqStarting, double tauSqStarting,
+               double phiStarting, double nuStarting, double sigmaSqTuning,
+               double tauSqTuning, double phiTuning, double nuTuning,
+               int nSamples, int nThreads, int verbose, int nReport,
+               int nRep, const int *repIndx, const nngp_rng *rng,
+               const nngp_opts *opts, double *pBetaSamples,
+               double *pThetaSamples, double *yRepSamples, int *nAccept);
+
+/* cNNGP — replaces src/cNNGP.cpp:73-358 (.Call at R/spConjNNGP.R:336,378).
+ * thetaAlpha is nTheta x g ({phi, alpha[, nu]} per column).  Outputs:
+ *   betaHat  p x g,  ab  2 x g (a, b per grid point),  bBInv  p*p x g,
+ *   bb  p x g,  y0Hat  n0 x g,  y0HatVar  n0 x g (both unused if n0 == 0).
+ * The reference stores ab at [k*2], [k*2+1] inside an nTheta x g matrix;
+ * this entry point writes the same values into a 2 x g array. */
+int nngp_cNNGP(const double *y, const double *X, int p, int n,
+               const double *coords, const double *thetaAlpha,
+               const int *nnIndx, const int *nnIndxLU, const double *X0,
+               const double *coords0, int n0, const int *nnIndx0, int g,
+               int m, const double *sigmaSqIG, int covModel, int nThreads,
+               int verbose, const nngp_opts *opts, double *betaHat,
+               double *ab, double *bBInv, double *bb, double *y0Hat,
+               double *y0HatVar);
+
+/* rNNGPPredict — replaces src/rNNGPPredict.cpp:25-222 (.Call at
+ * R/generics.R:674).  betaSamples p x nSamples, thetaSamples
+ * nTheta x nSamples, nnIndx0 q x m column-major.  z (q*nSamples standard
+ * normals, site-major: z[i*nSamples+s], the reference's single-thread
+ * order) may be NULL, in which case z is drawn from rng (or the built-in
+ * generator) in that order.  Output pY0 is q x nSamples. */
+int nngp_rNNGPPredict(const double *X, const double *y, const double *coords,
+                      int n, int p, int m, const double *X0,
+                      const double *coords0, int q, const int *nnIndx0,
+                      const double *betaSamples, const double *thetaSamples,
+                      int nSamples, int covModel, int nThreads, int verbose,
+                      int nReport, const double *z, const nngp_rng *rng,
+                      const nngp_opts *opts, double *pY0);
+
+/* ------------------------------------------------------------------------
+ * Device-resident model: coordinates, neighbor index, X and y stay in HBM
+ * across MCMC iterations.  nngp_rNNGP is built on this; it is exported for
+ * callers that run their own sampler.
+ * --------------------------------------------------------------------- */
+
+typedef struct nngp_model nngp_model;
+
+/* Quadratic forms produced by one pass (all over the same B, F). */
+typedef struct nngp_pass_out {
+  double logDet;            /* sum_i log F_i                               */
+  double qResid;            /* Q(r, r), r = X beta - y                     */
+  double XtViX[64];         /* p x p column-major, lower triangle filled   */
+  double XtViy[8];          /* p                                            */
+  double ytViy;             /* Q(y, y)                                      */
+} nngp_pass_out;
+
+int nngp_model_create(nngp_model **out, int n, int m, int p,
+                      const double *coords, const int *nnIndx,
+                      const int *nnIndxLU, const double *X, const double *y,
+                      int covModel, const nngp_opts *opts);
+void nngp_model_destroy(nngp_model *model);
+
+/* B, F of theta into slot (0 or 1) with logDet; when beta != NULL also
+ * Q(r, r) for r = X beta - y; when want_xprod != 0 also XtViX, XtViy,
+ * ytViy.  p <= 8. */
+int nngp_model_update(nngp_model *model, const double *theta, int slot,
+                      const double *beta, int want_xprod,
+                      nngp_pass_out *out);
+
+/* Q(r, r) for r = X beta - y over the B, F already in slot. */
+int nngp_model_qresid(nngp_model *model, int slot, const double *beta,
+                      double *q);
+
+/* Copy B[nIndx], F[n] of slot back to the host (B only after a pass that
+ * stored it: nngp_model_update always does). */
+int nngp_model_get_bf(nngp_model *model, int slot, double *B, double *F);
+
+/* The rNNGP sampler (src/rNNGP.cpp:192-388) on a model created with X and
+ * y, so that the data is already resident when the chain starts.  Same
+ * arguments and outputs as nngp_rNNGP without the data and without fit.rep
+ * (nRep = 0). */
+int nngp_model_rNNGP(nngp_model *model, const double *sigmaSqIG,
+                     const double *tauSqIG, const double *phiUnif,
+                     const double *nuUnif, const double *betaStarting,
+                     double sigmaSqStarting, double tauSqStarting,
+                     double phiStarting, double nuStarting,
+                     double sigmaSqTuning, double tauSqTuning,
+                     double phiTuning, double nuTuning, int nSamples,
+                     int verbose, int nReport, const nngp_rng *rng,
+                     const nngp_opts *opts, double *pBetaSamples,
+                     double *pThetaSamples, int *nAccept);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NNGP_H */

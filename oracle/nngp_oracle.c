@@ -1,0 +1,996 @@
+/*
+ * nngp_oracle.c — CPU oracle for spNNGP 1.0.1's NNGP hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see nngp_oracle.h).  Written from the
+ * reference's algorithms; compiled with -ffp-contract=off so that every
+ * product/sum rounds separately, as the reference's x86-64 build does.
+ */
+#include "nngp_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static _Thread_local char g_err[256];
+
+const char *or_last_error(void) { return g_err; }
+
+static int fail(int code, const char *msg) {
+  snprintf(g_err, sizeof g_err, "%s", msg);
+  return code;
+}
+
+/* ------------------------------------------------------------------ */
+/* layout and geometry                                                 */
+/* ------------------------------------------------------------------ */
+
+/* src/util.cpp:55-70: offset and count of site i in the flat arrays.  The
+ * reference evaluates the offsets in double and truncates; the products
+ * are exact integers in double for every n that fits in int32. */
+void or_getNNIndx(int i, int m, int *iNNIndx, int *iNN) {
+  if (i == 0) {
+    *iNNIndx = 0;
+    *iNN = 0;
+  } else if (i < m) {
+    *iNNIndx = (int)((double)i / 2 * (i - 1));
+    *iNN = i;
+  } else {
+    *iNNIndx = (int)((double)m / 2 * (m - 1) + (double)(i - m) * m);
+    *iNN = m;
+  }
+}
+
+static long long n_indx(int n, int m) {
+  /* src/rNNGP.cpp:218 */
+  return (long long)((double)(1 + m) / 2 * m + (double)(n - m - 1) * m);
+}
+
+/* src/util.cpp:51-53: sqrt(pow(a1-b1,2)+pow(a2-b2,2)); pow(x,2) == x*x. */
+double or_dist2(double a1, double a2, double b1, double b2) {
+  double dx = a1 - b1, dy = a2 - b2;
+  return sqrt(dx * dx + dy * dy);
+}
+
+/* ------------------------------------------------------------------ */
+/* Bessel K_nu (Temme 1975: series for x <= 2, Steed continued fraction  */
+/* CF2 for x > 2, then upward recurrence in the order)                 */
+/* ------------------------------------------------------------------ */
+
+/* Coefficients of 1/Gamma(z) = sum_k c_k z^k (Abramowitz & Stegun 6.1.34),
+ * even k: used for Gamma_1(mu) = -(c2 + c4 mu^2 + c6 mu^4 + ...) where the
+ * difference formula cancels. */
+static const double RG_C2 = 0.5772156649015329;
+static const double RG_C4 = -0.0420026350340952;
+static const double RG_C6 = -0.0421977345555443;
+static const double RG_C8 = 0.0072189432466630;
+static const double RG_C10 = -0.0002152416741149;
+static const double RG_C12 = -0.0000201348547807;
+static const double RG_C14 = 0.0000011330272320;
+
+static void temme_gammas(double mu, double *g1, double *g2, double *rgp,
+                         double *rgm) {
+  double rp = 1.0 / tgamma(1.0 + mu); /* 1/Gamma(1+mu) */
+  double rm = 1.0 / tgamma(1.0 - mu); /* 1/Gamma(1-mu) */
+  *rgp = rp;
+  *rgm = rm;
+  *g2 = 0.5 * (rm + rp);
+  if (fabs(mu) < 0.1) {
+    double u = mu * mu;
+    *g1 = -(RG_C2 +
+            u * (RG_C4 +
+                 u * (RG_C6 + u * (RG_C8 + u * (RG_C10 + u * (RG_C12 +
+                                                            u * RG_C14))))));
+  } else {
+    *g1 = (rm - rp) / (2.0 * mu);
+  }
+}
+
+double or_bessel_k(double x, double nu) {
+  const double eps = 1e-17;
+  const double pi = 3.14159265358979323846;
+  if (!(x > 0.0)) return x == 0.0 ? INFINITY : NAN;
+  nu = fabs(nu);
+  int nl = (int)(nu + 0.5);
+  double mu = nu - nl; /* in [-1/2, 1/2) */
+  double kmu, kmu1;    /* K_mu, K_{mu+1} */
+  if (x <= 2.0) {
+    double g1, g2, rgp, rgm;
+    temme_gammas(mu, &g1, &g2, &rgp, &rgm);
+    double half = 0.5 * x;
+    double lg = -log(half); /* ln(2/x) */
+    double sg = mu * lg;    /* sigma */
+    double pm = pi * mu;
+    double fa = fabs(pm) < 1e-300 ? 1.0 : pm / sin(pm);
+    double fs = fabs(sg) < 1e-300 ? 1.0 : sinh(sg) / sg;
+    double f = fa * (g1 * cosh(sg) + g2 * fs * lg);
+    double es = exp(sg);
+    double pk = 0.5 * es / rgp; /* p_0 = (x/2)^-mu Gamma(1+mu) / 2 */
+    double qk = 0.5 / (es * rgm); /* q_0 = (x/2)^mu Gamma(1-mu) / 2 */
+    double ck = 1.0;
+    double d = half * half;
+    double s0 = f, s1 = pk;
+    for (int k = 1; k < 500; k++) {
+      f = (k * f + pk + qk) / ((double)k * k - mu * mu);
+      ck *= d / k;
+      pk /= (k - mu);
+      qk /= (k + mu);
+      double t0 = ck * f;
+      s0 += t0;
+      s1 += ck * (pk - k * f);
+      if (fabs(t0) < fabs(s0) * eps) break;
+    }
+    kmu = s0;
+    kmu1 = s1 * (2.0 / x);
+  } else {
+    /* Steed's algorithm for CF2 */
+    double b = 2.0 * (1.0 + x);
+    double d = 1.0 / b;
+    double h = d, dh = d;
+    double q1 = 0.0, q2 = 1.0;
+    double a1 = 0.25 - mu * mu;
+    double q = a1, c = a1;
+    double a = -a1;
+    double s = 1.0 + q * dh;
+    for (int i = 2; i < 100000; i++) {
+      a -= 2.0 * (i - 1);
+      c = -a * c / i;
+      double qn = (q1 - b * q2) / a;
+      q1 = q2;
+      q2 = qn;
+      q += c * qn;
+      b += 2.0;
+      d = 1.0 / (b + a * d);
+      dh = (b * d - 1.0) * dh;
+      h += dh;
+      double ds = q * dh;
+      s += ds;
+      if (fabs(ds / s) < eps) break;
+    }
+    h = a1 * h;
+    kmu = sqrt(pi / (2.0 * x)) * exp(-x) / s;
+    kmu1 = kmu * (mu + x + 0.5 - h) / x;
+  }
+  for (int i = 1; i <= nl; i++) {
+    double kn = (mu + i) * (2.0 / x) * kmu1 + kmu;
+    kmu = kmu1;
+    kmu1 = kn;
+  }
+  return kmu;
+}
+
+/* src/util.cpp:208-245 */
+double or_spCor(double D, double phi, double nu, int covModel) {
+  if (covModel == 0) {
+    return exp(-phi * D);
+  } else if (covModel == 1) {
+    if (D > 0 && D <= 1.0 / phi) {
+      return 1.0 - 1.5 * phi * D + 0.5 * pow(phi * D, 3);
+    } else if (D >= 1.0 / phi) {
+      return 0.0;
+    } else {
+      return 1.0;
+    }
+  } else if (covModel == 2) {
+    if (D * phi > 0.0) {
+      return pow(D * phi, nu) / (pow(2, nu - 1) * tgamma(nu)) *
+             or_bessel_k(D * phi, nu);
+    }
+    return 1.0;
+  } else if (covModel == 3) {
+    double t = phi * D;
+    return exp(-1.0 * (t * t));
+  }
+  return NAN;
+}
+
+/* ------------------------------------------------------------------ */
+/* sorting and neighbor search                                         */
+/* ------------------------------------------------------------------ */
+
+/* comparison with NaN last (R's rcmp with nalast = TRUE) */
+static int rcmp_last(double x, double y) {
+  int nx = isnan(x), ny = isnan(y);
+  if (nx && ny) return 0;
+  if (nx) return 1;
+  if (ny) return -1;
+  if (x < y) return -1;
+  if (x > y) return 1;
+  return 0;
+}
+
+/* R's rsort_with_index: shell sort over the gap sequence 1, 4, 13, 40, ...
+ * starting from the first gap above n/9. */
+void or_rsort_with_index(double *x, int *indx, int n) {
+  int h = 1;
+  while (h <= n / 9) h = 3 * h + 1;
+  for (; h > 0; h /= 3) {
+    for (int i = h; i < n; i++) {
+      double v = x[i];
+      int iv = indx[i];
+      int j = i;
+      while (j >= h && rcmp_last(x[j - h], v) > 0) {
+        x[j] = x[j - h];
+        indx[j] = indx[j - h];
+        j -= h;
+      }
+      x[j] = v;
+      indx[j] = iv;
+    }
+  }
+}
+
+static int check_nn_args(int n, int m) {
+  if (n < 1 || m < 1) return fail(OR_ERR_ARG, "n and m must be positive");
+  if (m >= n) return fail(OR_ERR_ARG, "m must be smaller than n");
+  return OR_OK;
+}
+
+/* src/nn.cpp:63-111 */
+int or_mkNNIndx(int n, int m, const double *coords, int *nnIndx,
+                double *nnDist, int *nnIndxLU, int nThreads) {
+  int st = check_nn_args(n, m);
+  if (st) return st;
+  long long nIndx = n_indx(n, m);
+  for (long long i = 0; i < nIndx; i++) {
+    nnDist[i] = INFINITY;
+    nnIndx[i] = 0;
+  }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nThreads > 0 ? nThreads : 1)
+#endif
+  for (int i = 0; i < n; i++) {
+    int off, cnt;
+    or_getNNIndx(i, m, &off, &cnt);
+    nnIndxLU[i] = off;
+    nnIndxLU[n + i] = cnt;
+    for (int j = 0; j < i; j++) {
+      double d = or_dist2(coords[i], coords[n + i], coords[j], coords[n + j]);
+      if (d < nnDist[off + cnt - 1]) {
+        nnDist[off + cnt - 1] = d;
+        nnIndx[off + cnt - 1] = j;
+        or_rsort_with_index(&nnDist[off], &nnIndx[off], cnt);
+      }
+    }
+  }
+  return OR_OK;
+}
+
+/* src/nn.cpp:128-134 */
+static double cb_dmi(const double *x, const double *c, int inc) {
+  double t = x[0] + x[inc] - c[0] - c[inc];
+  return t * t;
+}
+static double cb_dei(const double *x, const double *c, int inc) {
+  double a = x[0] - c[0], b = x[inc] - c[inc];
+  return a * a + b * b;
+}
+
+/* src/nn.cpp:136-211.  A site with no predecessors (m == 0) has nothing to
+ * fill; the reference reads rSNNDist[-1] there and writes nothing. */
+static void cb_fastNN(int m, int n, const double *coords, int ui,
+                      const int *sIndx, int *rSIndx, double *rSNNDist) {
+  if (m == 0) return;
+  for (int k = 0; k < m; k++) rSNNDist[k] = INFINITY;
+  int i = ui, j = ui;
+  int up = 1, down = 1;
+  const double *q = &coords[sIndx[ui]];
+  while (up || down) {
+    if (i == 0) down = 0;
+    if (j == n - 1) up = 0;
+    if (down) {
+      i--;
+      double dm = cb_dmi(q, &coords[sIndx[i]], n);
+      if (dm > 2 * rSNNDist[m - 1]) {
+        down = 0;
+      } else {
+        double de = cb_dei(q, &coords[sIndx[i]], n);
+        if (de < rSNNDist[m - 1] && sIndx[i] < sIndx[ui]) {
+          rSNNDist[m - 1] = de;
+          rSIndx[m - 1] = sIndx[i];
+          or_rsort_with_index(rSNNDist, rSIndx, m);
+        }
+      }
+    }
+    if (up) {
+      j++;
+      double dm = cb_dmi(q, &coords[sIndx[j]], n);
+      if (dm > 2 * rSNNDist[m - 1]) {
+        up = 0;
+      } else {
+        double de = cb_dei(q, &coords[sIndx[j]], n);
+        if (de < rSNNDist[m - 1] && sIndx[j] < sIndx[ui]) {
+          rSNNDist[m - 1] = de;
+          rSIndx[m - 1] = sIndx[j];
+          or_rsort_with_index(rSNNDist, rSIndx, m);
+        }
+      }
+    }
+  }
+  for (int k = 0; k < m; k++) rSNNDist[k] = sqrt(rSNNDist[k]);
+}
+
+/* src/nn.cpp:214-257 */
+int or_mkNNIndxCB(int n, int m, const double *coords, int *nnIndx,
+                  double *nnDist, int *nnIndxLU, int nThreads) {
+  int st = check_nn_args(n, m);
+  if (st) return st;
+  long long nIndx = n_indx(n, m);
+  for (long long i = 0; i < nIndx; i++) {
+    nnDist[i] = 0.0;
+    nnIndx[i] = 0;
+  }
+  int *sIndx = (int *)malloc(sizeof(int) * (size_t)n);
+  double *u = (double *)malloc(sizeof(double) * (size_t)n);
+  if (!sIndx || !u) {
+    free(sIndx);
+    free(u);
+    return fail(OR_ERR_ARG, "out of memory");
+  }
+  for (int i = 0; i < n; i++) {
+    sIndx[i] = i;
+    u[i] = coords[i] + coords[n + i];
+  }
+  or_rsort_with_index(u, sIndx, n);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nThreads > 0 ? nThreads : 1)
+#endif
+  for (int i = 0; i < n; i++) {
+    int off, cnt;
+    or_getNNIndx(sIndx[i], m, &off, &cnt);
+    nnIndxLU[sIndx[i]] = off;
+    nnIndxLU[n + sIndx[i]] = cnt;
+    cb_fastNN(cnt, n, coords, i, sIndx, &nnIndx[off], &nnDist[off]);
+  }
+  free(sIndx);
+  free(u);
+  return OR_OK;
+}
+
+/* RANN nn2 (eps = 0): plain scan, ties by ascending index. */
+int or_nn2(int n, const double *coords, int q, const double *coords0, int k,
+           int *nnIndx0, double *nnDist0) {
+  if (k < 1 || k > n) return fail(OR_ERR_ARG, "k must be in 1..n");
+  double *bd = (double *)malloc(sizeof(double) * (size_t)k);
+  int *bi = (int *)malloc(sizeof(int) * (size_t)k);
+  for (int i = 0; i < q; i++) {
+    for (int t = 0; t < k; t++) {
+      bd[t] = INFINITY;
+      bi[t] = -1;
+    }
+    for (int j = 0; j < n; j++) {
+      double dx = coords0[i] - coords[j], dy = coords0[q + i] - coords[n + j];
+      double d2 = dx * dx + dy * dy;
+      if (d2 < bd[k - 1]) {
+        int t = k - 1;
+        while (t > 0 && bd[t - 1] > d2) {
+          bd[t] = bd[t - 1];
+          bi[t] = bi[t - 1];
+          t--;
+        }
+        bd[t] = d2;
+        bi[t] = j;
+      }
+    }
+    for (int t = 0; t < k; t++) {
+      nnIndx0[i + (size_t)q * t] = bi[t];
+      if (nnDist0) nnDist0[i + (size_t)q * t] = sqrt(bd[t]);
+    }
+  }
+  free(bd);
+  free(bi);
+  return OR_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* dense kernels: LAPACK dpotf2 / dtrti2 / dlauu2, BLAS dsymv, dtrmv   */
+/* (reference implementations' loop orders, lower triangle, lda = n)   */
+/* ------------------------------------------------------------------ */
+
+#define A_(i, j) A[(size_t)(j) * n + (i)]
+
+/* dpotf2, UPLO = 'L': returns 0 or the 1-based failing column. */
+int or_potrf_L(int n, double *A) {
+  for (int j = 0; j < n; j++) {
+    double ajj = A_(j, j);
+    double dot = 0.0;
+    for (int k = 0; k < j; k++) dot += A_(j, k) * A_(j, k);
+    ajj = ajj - dot;
+    if (ajj <= 0.0 || isnan(ajj)) {
+      A_(j, j) = ajj;
+      return j + 1;
+    }
+    ajj = sqrt(ajj);
+    A_(j, j) = ajj;
+    if (j < n - 1) {
+      /* dgemv('N'): A(j+1:n, j) -= A(j+1:n, 0:j) * A(j, 0:j)^T */
+      for (int k = 0; k < j; k++) {
+        double t = -A_(j, k);
+        for (int i = j + 1; i < n; i++) A_(i, j) += t * A_(i, k);
+      }
+      double r = 1.0 / ajj;
+      for (int i = j + 1; i < n; i++) A_(i, j) *= r;
+    }
+  }
+  return 0;
+}
+
+/* dtrti2('L','N') followed by dlauu2('L'): inv(A) from its lower Cholesky
+ * factor, lower triangle. */
+int or_potri_L(int n, double *A) {
+  for (int j = 0; j < n; j++)
+    if (A_(j, j) == 0.0) return j + 1;
+  /* dtrti2 lower, non-unit */
+  for (int j = n - 1; j >= 0; j--) {
+    A_(j, j) = 1.0 / A_(j, j);
+    double ajj = -A_(j, j);
+    if (j < n - 1) {
+      /* dtrmv('L','N','N') on x = A(j+1:n, j) with T = A(j+1:n, j+1:n) */
+      int nn = n - j - 1;
+      for (int jj = nn - 1; jj >= 0; jj--) {
+        double t = A_(j + 1 + jj, j);
+        if (t != 0.0) {
+          for (int ii = nn - 1; ii > jj; ii--)
+            A_(j + 1 + ii, j) += t * A_(j + 1 + ii, j + 1 + jj);
+          A_(j + 1 + jj, j) = t * A_(j + 1 + jj, j + 1 + jj);
+        }
+      }
+      for (int ii = 0; ii < nn; ii++) A_(j + 1 + ii, j) *= ajj;
+    }
+  }
+  /* dlauu2 lower: A := L^T L */
+  for (int i = 0; i < n; i++) {
+    double aii = A_(i, i);
+    if (i < n - 1) {
+      double dot = 0.0;
+      for (int k = i; k < n; k++) dot += A_(k, i) * A_(k, i);
+      A_(i, i) = dot;
+      /* dgemv('T'): A(i, 0:i) = aii*A(i,0:i) + A(i+1:n, 0:i)^T A(i+1:n, i) */
+      for (int jj = 0; jj < i; jj++) {
+        double t = 0.0;
+        for (int k = i + 1; k < n; k++) t += A_(k, jj) * A_(k, i);
+        A_(i, jj) = t + aii * A_(i, jj);
+      }
+    } else {
+      for (int jj = 0; jj <= i; jj++) A_(i, jj) *= aii;
+    }
+  }
+  return 0;
+}
+
+/* dsymv('L'), alpha = 1, beta = 0 */
+void or_symv_L(int n, const double *A, const double *x, double *y) {
+  for (int i = 0; i < n; i++) y[i] = 0.0;
+  for (int j = 0; j < n; j++) {
+    double t1 = x[j], t2 = 0.0;
+    y[j] += t1 * A_(j, j);
+    for (int i = j + 1; i < n; i++) {
+      y[i] += t1 * A_(i, j);
+      t2 += A_(i, j) * x[i];
+    }
+    y[j] += t2;
+  }
+}
+
+/* dtrmv('L','N','N') */
+static void trmv_L(int n, const double *A, double *x) {
+  for (int j = n - 1; j >= 0; j--) {
+    double t = x[j];
+    if (t != 0.0) {
+      for (int i = n - 1; i > j; i--) x[i] += t * A_(i, j);
+      x[j] = t * A_(j, j);
+    }
+  }
+}
+
+#undef A_
+
+static double ddot(int n, const double *x, int incx, const double *y,
+                   int incy) {
+  double s = 0.0;
+  for (int i = 0; i < n; i++) s += x[(size_t)i * incx] * y[(size_t)i * incy];
+  return s;
+}
+
+/* ------------------------------------------------------------------ */
+/* updateBF / updateConjBF / Q                                         */
+/* ------------------------------------------------------------------ */
+
+/* One site's neighbor system: C = s2 R(N,N) + t2 I (lower), c = s2 R(N,i),
+ * potrf -> potri -> symv, F = s2 - B.c + t2  (src/rNNGP.cpp:74-89). */
+static int site_bf(int i, int n, int mi, const int *nb, const double *coords,
+                   double s2, double t2, double phi, double nu, int covModel,
+                   double *C, double *c, double *Bout, double *Fout) {
+  for (int k = 0; k < mi; k++) {
+    int a = nb[k];
+    double e = or_dist2(coords[i], coords[n + i], coords[a], coords[n + a]);
+    c[k] = s2 * or_spCor(e, phi, nu, covModel);
+    for (int l = 0; l <= k; l++) {
+      int b = nb[l];
+      e = or_dist2(coords[a], coords[n + a], coords[b], coords[n + b]);
+      C[l * mi + k] = s2 * or_spCor(e, phi, nu, covModel);
+      if (l == k) C[l * mi + k] += t2;
+    }
+  }
+  if (or_potrf_L(mi, C) != 0) return OR_ERR_NOT_PD;
+  if (or_potri_L(mi, C) != 0) return OR_ERR_NOT_PD;
+  or_symv_L(mi, C, c, Bout);
+  *Fout = s2 - ddot(mi, Bout, 1, c, 1) + t2;
+  return OR_OK;
+}
+
+static int bf_common(double *B, double *F, const double *coords,
+                     const int *nnIndx, const int *nnIndxLU, int n, int m,
+                     double s2, double t2, double phi, double nu,
+                     int covModel, int nThreads) {
+  if (covModel < 0 || covModel > 3)
+    return fail(OR_ERR_COV_MODEL,
+                "c++ Rf_error: cov.model is not correctly specified");
+  int bad = -1;
+  int nt = nThreads > 0 ? nThreads : 1;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt)
+#endif
+  {
+    double *C = (double *)malloc(sizeof(double) * (size_t)m * m);
+    double *c = (double *)malloc(sizeof(double) * (size_t)m);
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (int i = 0; i < n; i++) {
+      if (i > 0) {
+        int mi = nnIndxLU[n + i];
+        if (site_bf(i, n, mi, &nnIndx[nnIndxLU[i]], coords, s2, t2, phi, nu,
+                    covModel, C, c, &B[nnIndxLU[i]], &F[i]) != OR_OK) {
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+          if (bad < 0 || i < bad) bad = i;
+        }
+      } else {
+        F[i] = s2 + t2;
+      }
+    }
+    free(C);
+    free(c);
+  }
+  if (bad >= 0) {
+    snprintf(g_err, sizeof g_err, "c++ Rf_error: dpotrf failed (site %d)",
+             bad);
+    return OR_ERR_NOT_PD;
+  }
+  return OR_OK;
+}
+
+/* src/rNNGP.cpp:46-101 */
+int or_updateBF(double *B, double *F, const double *coords,
+                const int *nnIndx, const int *nnIndxLU, int n, int m,
+                const double *theta, int covModel, int nThreads,
+                double *logDet) {
+  double nu = covModel == 2 ? theta[3] : 0.0;
+  int st = bf_common(B, F, coords, nnIndx, nnIndxLU, n, m, theta[0],
+                     theta[1], theta[2], nu, covModel, nThreads);
+  if (st) return st;
+  double ld = 0.0;
+  for (int i = 0; i < n; i++) ld += log(F[i]);
+  *logDet = ld;
+  return OR_OK;
+}
+
+/* src/cNNGP.cpp:24-69: correlation scale, F = 1 + alpha - B.c */
+int or_updateConjBF(double *B, double *F, const double *coords,
+                    const int *nnIndx, const int *nnIndxLU, int n, int m,
+                    double phi, double alpha, double nu, int covModel,
+                    int nThreads) {
+  return bf_common(B, F, coords, nnIndx, nnIndxLU, n, m, 1.0, alpha, phi,
+                   nu, covModel, nThreads);
+}
+
+/* src/util.cpp:261-280 */
+double or_Q(const double *B, const double *F, const double *u,
+            const double *v, int n, const int *nnIndx, const int *nnIndxLU) {
+  double q = 0.0;
+  for (int i = 0; i < n; i++) {
+    double a = 0.0, b = 0.0;
+    for (int j = 0; j < nnIndxLU[n + i]; j++) {
+      a += B[nnIndxLU[i] + j] * u[nnIndx[nnIndxLU[i] + j]];
+      b += B[nnIndxLU[i] + j] * v[nnIndx[nnIndxLU[i] + j]];
+    }
+    q += (u[i] - a) * (v[i] - b) / F[i];
+  }
+  return q;
+}
+
+/* ------------------------------------------------------------------ */
+/* RNG helpers with R's argument semantics                              */
+/* ------------------------------------------------------------------ */
+
+/* R's rnorm(mu, sigma): sigma == 0 or non-finite mu returns mu and draws
+ * nothing. */
+static double r_rnorm(const or_rng *r, double mu, double sd) {
+  if (isnan(mu) || !isfinite(sd) || sd < 0.0) return NAN;
+  if (sd == 0.0 || !isfinite(mu)) return mu;
+  return mu + sd * r->norm_rand(r->state);
+}
+
+/* R's runif(a, b) */
+static double r_runif(const or_rng *r, double a, double b) {
+  if (!isfinite(a) || !isfinite(b) || b < a) return NAN;
+  if (a == b) return a;
+  double u;
+  do {
+    u = r->unif_rand(r->state);
+  } while (u <= 0.0 || u >= 1.0);
+  return a + (b - a) * u;
+}
+
+/* src/util.cpp:43-49 */
+static double logit_ab(double t, double a, double b) {
+  return log((t - a) / (b - t));
+}
+static double logit_inv(double z, double a, double b) {
+  return b - (b - a) / (1 + exp(z));
+}
+
+/* src/util.cpp:28-41: des = mu + L z */
+static void mvrnorm(const or_rng *r, double *des, const double *mu,
+                    const double *L, int dim) {
+  for (int i = 0; i < dim; i++) des[i] = r_rnorm(r, 0, 1);
+  trmv_L(dim, L, des);
+  for (int i = 0; i < dim; i++) des[i] += mu[i];
+}
+
+void or_nngp_draw(const double *B, const double *F, const int *nnIndx,
+                  const int *nnIndxLU, int n, const double *z, double *w) {
+  for (int i = 0; i < n; i++) {
+    if (i == 0) {
+      w[i] = sqrt(F[i]) * z[i];
+    } else {
+      double s = 0.0;
+      for (int j = 0; j < nnIndxLU[n + i]; j++)
+        s += B[nnIndxLU[i] + j] * w[nnIndx[nnIndxLU[i] + j]];
+      w[i] = s + sqrt(F[i]) * z[i];
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* rNNGP: response-model MCMC                                          */
+/* ------------------------------------------------------------------ */
+
+static double log_post(const double *th, double logDet, double Q,
+                       const double *sIG, const double *tIG,
+                       const double *phiU, const double *nuU, int matern) {
+  double lp = -0.5 * logDet - 0.5 * Q;
+  lp += log(th[2] - phiU[0]) + log(phiU[1] - th[2]);
+  lp += -1.0 * (1.0 + sIG[0]) * log(th[0]) - sIG[1] / th[0] + log(th[0]);
+  lp += -1.0 * (1.0 + tIG[0]) * log(th[1]) - tIG[1] / th[1] + log(th[1]);
+  if (matern) lp += log(th[3] - nuU[0]) + log(nuU[1] - th[3]);
+  return lp;
+}
+
+/* src/rNNGP.cpp:105-420 */
+int or_rNNGP(const double *y, const double *X, int p, int n, int m,
+             const double *coords, int covModel, const int *nnIndx,
+             const int *nnIndxLU, const double *sigmaSqIG,
+             const double *tauSqIG, const double *phiUnif,
+             const double *nuUnif, const double *betaStarting,
+             double sigmaSqStarting, double tauSqStarting,
+             double phiStarting, double nuStarting, double sigmaSqTuning,
+             double tauSqTuning, double phiTuning, double nuTuning,
+             int nSamples, int nThreads, int nRep, const int *repIndx,
+             const or_rng *rng, double *pBetaSamples, double *pThetaSamples,
+             double *yRepSamples, int *nAccept) {
+  if (covModel < 0 || covModel > 3)
+    return fail(OR_ERR_COV_MODEL,
+                "c++ Rf_error: cov.model is not correctly specified");
+  int matern = covModel == 2;
+  int nTheta = matern ? 4 : 3;
+  double theta[4] = {sigmaSqStarting, tauSqStarting, phiStarting,
+                     matern ? nuStarting : 0.0};
+  double tuning[4] = {sigmaSqTuning, tauSqTuning, phiTuning,
+                      matern ? nuTuning : 0.0};
+  double thetaCand[4] = {0, 0, 0, 0};
+  double nuU[2] = {matern ? nuUnif[0] : 0.0, matern ? nuUnif[1] : 0.0};
+  long long nIndx = n_indx(n, m);
+  double *beta = (double *)calloc((size_t)p, sizeof(double));
+  double *B = (double *)calloc((size_t)nIndx, sizeof(double));
+  double *F = (double *)calloc((size_t)n, sizeof(double));
+  double *tmp_pp = (double *)calloc((size_t)p * p, sizeof(double));
+  double *tmp_p = (double *)calloc((size_t)p, sizeof(double));
+  double *tmp_p2 = (double *)calloc((size_t)p, sizeof(double));
+  double *tmp_n = (double *)calloc((size_t)n, sizeof(double));
+  double *tmp_n2 = (double *)calloc((size_t)n, sizeof(double));
+  double *z = (double *)calloc((size_t)n, sizeof(double));
+  int st = OR_OK, accept = 0, repCnt = 0;
+  for (int i = 0; i < p; i++) beta[i] = betaStarting[i];
+
+  double logDetCurrent, logDetCand, QCurrent, QCand;
+  st = or_updateBF(B, F, coords, nnIndx, nnIndxLU, n, m, theta, covModel,
+                   nThreads, &logDetCurrent);
+  if (st) goto done;
+  int thetaUpdate = 1;
+  for (int s = 0; s < nSamples; s++) {
+    if (thetaUpdate) {
+      thetaUpdate = 0;
+      for (int i = 0; i < p; i++) {
+        tmp_p[i] = or_Q(B, F, &X[(size_t)n * i], y, n, nnIndx, nnIndxLU);
+        for (int j = 0; j <= i; j++)
+          tmp_pp[j * p + i] = or_Q(B, F, &X[(size_t)n * j],
+                                   &X[(size_t)n * i], n, nnIndx, nnIndxLU);
+      }
+      if (or_potrf_L(p, tmp_pp) || or_potri_L(p, tmp_pp)) {
+        st = fail(OR_ERR_NOT_PD, "c++ Rf_error: dpotrf failed");
+        goto done;
+      }
+      or_symv_L(p, tmp_pp, tmp_p, tmp_p2);
+      if (or_potrf_L(p, tmp_pp)) {
+        st = fail(OR_ERR_NOT_PD, "c++ Rf_error: dpotrf failed");
+        goto done;
+      }
+    }
+    mvrnorm(rng, beta, tmp_p2, tmp_pp, p);
+    /* dgemv: tmp_n = X beta; then tmp_n -= y */
+    for (int i = 0; i < n; i++) tmp_n[i] = 0.0;
+    for (int j = 0; j < p; j++) {
+      double t = beta[j];
+      for (int i = 0; i < n; i++) tmp_n[i] += t * X[(size_t)n * j + i];
+    }
+    if (nRep && repIndx[s])
+      for (int i = 0; i < n; i++)
+        yRepSamples[(size_t)repCnt * n + i] = tmp_n[i];
+    for (int i = 0; i < n; i++) tmp_n[i] += -1.0 * y[i];
+
+    st = or_updateBF(B, F, coords, nnIndx, nnIndxLU, n, m, theta, covModel,
+                     nThreads, &logDetCurrent);
+    if (st) goto done;
+    if (nRep && repIndx[s]) {
+      for (int i = 0; i < n; i++) z[i] = r_rnorm(rng, 0.0, 1.0);
+      or_nngp_draw(B, F, nnIndx, nnIndxLU, n, z, tmp_n2);
+      for (int i = 0; i < n; i++)
+        yRepSamples[(size_t)repCnt * n + i] += tmp_n2[i];
+      repCnt++;
+    }
+    QCurrent = or_Q(B, F, tmp_n, tmp_n, n, nnIndx, nnIndxLU);
+    double lpCur = log_post(theta, logDetCurrent, QCurrent, sigmaSqIG,
+                            tauSqIG, phiUnif, nuU, matern);
+
+    thetaCand[2] = logit_inv(
+        r_rnorm(rng, logit_ab(theta[2], phiUnif[0], phiUnif[1]), tuning[2]),
+        phiUnif[0], phiUnif[1]);
+    thetaCand[0] = exp(r_rnorm(rng, log(theta[0]), tuning[0]));
+    thetaCand[1] = exp(r_rnorm(rng, log(theta[1]), tuning[1]));
+    if (matern)
+      thetaCand[3] = logit_inv(
+          r_rnorm(rng, logit_ab(theta[3], nuU[0], nuU[1]), tuning[3]),
+          nuU[0], nuU[1]);
+
+    st = or_updateBF(B, F, coords, nnIndx, nnIndxLU, n, m, thetaCand,
+                     covModel, nThreads, &logDetCand);
+    if (st) goto done;
+    QCand = or_Q(B, F, tmp_n, tmp_n, n, nnIndx, nnIndxLU);
+    double lpCand = log_post(thetaCand, logDetCand, QCand, sigmaSqIG,
+                             tauSqIG, phiUnif, nuU, matern);
+    if (r_runif(rng, 0.0, 1.0) <= exp(lpCand - lpCur)) {
+      thetaUpdate = 1;
+      for (int k = 0; k < nTheta; k++) theta[k] = thetaCand[k];
+      accept++;
+    }
+    for (int k = 0; k < p; k++) pBetaSamples[(size_t)s * p + k] = beta[k];
+    for (int k = 0; k < nTheta; k++)
+      pThetaSamples[(size_t)s * nTheta + k] = theta[k];
+  }
+done:
+  if (nAccept) *nAccept = accept;
+  free(beta);
+  free(B);
+  free(F);
+  free(tmp_pp);
+  free(tmp_p);
+  free(tmp_p2);
+  free(tmp_n);
+  free(tmp_n2);
+  free(z);
+  return st;
+}
+
+/* ------------------------------------------------------------------ */
+/* rNNGPPredict                                                         */
+/* ------------------------------------------------------------------ */
+
+/* src/rNNGPPredict.cpp:139-183 with one thread: y0[s*q+i] uses z[i*S+s] */
+int or_rNNGPPredict(const double *X, const double *y, const double *coords,
+                    int n, int p, int m, const double *X0,
+                    const double *coords0, int q, const int *nnIndx0,
+                    const double *betaSamples, const double *thetaSamples,
+                    int nSamples, int covModel, int nThreads, const double *z,
+                    double *pY0) {
+  if (covModel < 0 || covModel > 3)
+    return fail(OR_ERR_COV_MODEL,
+                "c++ Rf_error: cov.model is not correctly specified");
+  int nTheta = covModel == 2 ? 4 : 3;
+  int bad = 0;
+  int nt = nThreads > 0 ? nThreads : 1;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt)
+#endif
+  {
+    double *C = (double *)malloc(sizeof(double) * (size_t)m * m);
+    double *c = (double *)malloc(sizeof(double) * (size_t)m);
+    double *w = (double *)malloc(sizeof(double) * (size_t)m);
+    for (int i = 0; i < q; i++) {
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+      for (int s = 0; s < nSamples; s++) {
+        const double *th = &thetaSamples[(size_t)s * nTheta];
+        double s2 = th[0], t2 = th[1], phi = th[2];
+        double nu = covModel == 2 ? th[3] : 0.0;
+        for (int k = 0; k < m; k++) {
+          int a = nnIndx0[i + (size_t)q * k];
+          double d = or_dist2(coords[a], coords[n + a], coords0[i],
+                              coords0[q + i]);
+          c[k] = s2 * or_spCor(d, phi, nu, covModel);
+          for (int l = 0; l < m; l++) {
+            int b = nnIndx0[i + (size_t)q * l];
+            d = or_dist2(coords[a], coords[n + a], coords[b], coords[n + b]);
+            C[l * m + k] = s2 * or_spCor(d, phi, nu, covModel);
+            if (k == l) C[l * m + k] += t2;
+          }
+        }
+        if (or_potrf_L(m, C) || or_potri_L(m, C)) {
+          bad = 1;
+          continue;
+        }
+        or_symv_L(m, C, c, w);
+        const double *bs = &betaSamples[(size_t)s * p];
+        double d = 0.0;
+        for (int k = 0; k < m; k++) {
+          int a = nnIndx0[i + (size_t)q * k];
+          d += w[k] * (y[a] - ddot(p, &X[a], n, bs, 1));
+        }
+        pY0[(size_t)s * q + i] =
+            ddot(p, &X0[i], q, bs, 1) + d +
+            sqrt(s2 + t2 - ddot(m, w, 1, c, 1)) * z[(size_t)i * nSamples + s];
+      }
+    }
+    free(C);
+    free(c);
+    free(w);
+  }
+  if (bad) return fail(OR_ERR_NOT_PD, "c++ Rf_error: dpotrf failed");
+  return OR_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* cNNGP                                                                */
+/* ------------------------------------------------------------------ */
+
+/* src/cNNGP.cpp:73-358 */
+int or_cNNGP(const double *y, const double *X, int p, int n,
+             const double *coords, const double *thetaAlpha,
+             const int *nnIndx, const int *nnIndxLU, const double *X0,
+             const double *coords0, int n0, const int *nnIndx0, int g, int m,
+             const double *sigmaSqIG, int covModel, int nThreads,
+             double *betaHat, double *ab, double *bBInv, double *bb,
+             double *y0Hat, double *y0HatVar) {
+  if (covModel < 0 || covModel > 3)
+    return fail(OR_ERR_COV_MODEL,
+                "c++ Rf_error: cov.model is not correctly specified");
+  int nTheta = covModel == 2 ? 3 : 2;
+  long long nIndx = n_indx(n, m);
+  double *B = (double *)calloc((size_t)nIndx, sizeof(double));
+  double *F = (double *)calloc((size_t)n, sizeof(double));
+  double *tmp_pp = (double *)calloc((size_t)p * p, sizeof(double));
+  double *tmp_p = (double *)calloc((size_t)p, sizeof(double));
+  int st = OR_OK;
+  int nt = nThreads > 0 ? nThreads : 1;
+  for (int k = 0; k < g; k++) {
+    double phi = thetaAlpha[k * nTheta];
+    double alpha = thetaAlpha[k * nTheta + 1];
+    double nu = covModel == 2 ? thetaAlpha[k * nTheta + 2] : 0.0;
+    st = or_updateConjBF(B, F, coords, nnIndx, nnIndxLU, n, m, phi, alpha,
+                         nu, covModel, nThreads);
+    if (st) break;
+    for (int i = 0; i < p; i++) {
+      tmp_p[i] = or_Q(B, F, &X[(size_t)n * i], y, n, nnIndx, nnIndxLU);
+      for (int j = 0; j <= i; j++)
+        tmp_pp[j * p + i] = or_Q(B, F, &X[(size_t)n * j], &X[(size_t)n * i],
+                                 n, nnIndx, nnIndxLU);
+    }
+    if (or_potrf_L(p, tmp_pp) || or_potri_L(p, tmp_pp)) {
+      st = fail(OR_ERR_NOT_PD, "c++ Rf_error: dpotrf failed");
+      break;
+    }
+    double *bk = &betaHat[(size_t)k * p];
+    or_symv_L(p, tmp_pp, tmp_p, bk);
+    double a = sigmaSqIG[0] + 1.0 * n / 2.0;
+    double b = sigmaSqIG[1] +
+               0.5 * (or_Q(B, F, y, y, n, nnIndx, nnIndxLU) -
+                      ddot(p, bk, 1, tmp_p, 1));
+    ab[k * 2] = a;
+    ab[k * 2 + 1] = b;
+    for (int i = 0; i < p; i++)
+      for (int j = 0; j < p; j++)
+        bBInv[(size_t)k * p * p + j * p + i] =
+            i >= j ? tmp_pp[j * p + i] : tmp_pp[i * p + j];
+    for (int i = 0; i < p; i++) bb[(size_t)k * p + i] = tmp_p[i];
+
+    int bad = 0;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt)
+#endif
+    {
+      double *C0 = (double *)malloc(sizeof(double) * (size_t)m * m);
+      double *c0 = (double *)malloc(sizeof(double) * (size_t)m);
+      double *w = (double *)malloc(sizeof(double) * (size_t)m);
+      double *tm = (double *)malloc(sizeof(double) * (size_t)m);
+      double *tmp = (double *)malloc(sizeof(double) * (size_t)m * p);
+      double *u = (double *)malloc(sizeof(double) * (size_t)p);
+      double *u2 = (double *)malloc(sizeof(double) * (size_t)p);
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+      for (int i = 0; i < n0; i++) {
+        for (int j = 0; j < m; j++) {
+          int aj = nnIndx0[(size_t)j * n0 + i];
+          double e = or_dist2(coords0[i], coords0[n0 + i], coords[aj],
+                              coords[n + aj]);
+          c0[j] = or_spCor(e, phi, nu, covModel);
+          for (int l = 0; l <= j; l++) {
+            int al = nnIndx0[(size_t)l * n0 + i];
+            e = or_dist2(coords[aj], coords[n + aj], coords[al],
+                         coords[n + al]);
+            C0[l * m + j] = or_spCor(e, phi, nu, covModel);
+            if (l == j) C0[l * m + j] += alpha;
+          }
+        }
+        if (or_potrf_L(m, C0) || or_potri_L(m, C0)) {
+          bad = 1;
+          continue;
+        }
+        or_symv_L(m, C0, c0, w);
+        for (int j = 0; j < m; j++) {
+          int aj = nnIndx0[(size_t)j * n0 + i];
+          tm[j] = y[aj] - ddot(p, &X[aj], n, bk, 1);
+        }
+        y0Hat[(size_t)k * n0 + i] =
+            ddot(p, &X0[i], n0, bk, 1) + ddot(m, w, 1, tm, 1);
+        for (int j = 0; j < m; j++) {
+          int aj = nnIndx0[(size_t)j * n0 + i];
+          for (int t = 0; t < p; t++)
+            tmp[(size_t)t * m + j] = X[(size_t)t * n + aj];
+        }
+        for (int t = 0; t < p; t++) {
+          double acc = 0.0;
+          for (int j = 0; j < m; j++) acc += tmp[(size_t)t * m + j] * w[j];
+          u[t] = acc;
+        }
+        for (int t = 0; t < p; t++) u[t] = X0[(size_t)t * n0 + i] - u[t];
+        or_symv_L(p, tmp_pp, u, u2);
+        y0HatVar[(size_t)k * n0 + i] =
+            b * (ddot(p, u, 1, u2, 1) + 1.0 + alpha - ddot(m, w, 1, c0, 1)) /
+            (a - 1.0);
+      }
+      free(C0);
+      free(c0);
+      free(w);
+      free(tm);
+      free(tmp);
+      free(u);
+      free(u2);
+    }
+    if (bad) {
+      st = fail(OR_ERR_NOT_PD, "c++ Rf_error: dpotrf failed");
+      break;
+    }
+  }
+  free(B);
+  free(F);
+  free(tmp_pp);
+  free(tmp_p);
+  return st;
+}

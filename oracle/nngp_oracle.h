@@ -1,0 +1,142 @@
+/*
+ * nngp_oracle.h — CPU oracle for the NNGP hot path of spNNGP 1.0.1.
+ *
+ * TEST INFRASTRUCTURE ONLY.  A plain-C restatement of the reference's
+ * algorithms, used as the checker by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py.  The product (spnngp_amd, libnngp_hip.so)
+ * never links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates.  Layouts are the
+ * reference's: column-major FP64, int32 indices, flat nnIndx/nnIndxLU.
+ *
+ * Parity pinning (tests/test_oracle_*.py):
+ *   or_getNNIndx      exhaustive offsets/counts vs the closed form
+ *   or_spCor          closed forms (exp, gaussian, spherical branches,
+ *                     matern nu = 0.5/1.5/2.5), general nu vs scipy kv
+ *   or_bessel_k       scipy.special.kv, half-integer closed forms
+ *   or_mkNNIndx(CB)   brute force in numpy, scipy cKDTree on prefixes,
+ *                     CB == brute on tie-free data
+ *   or_updateBF       residual ||C B - c||, scipy potrf/potri/symv sequence,
+ *                     exact-GP identity (m >= n-1: NNGP likelihood equals the
+ *                     dense Gaussian log-density)
+ *   or_Q              dense (I-B)^T F^-1 (I-B) quadratic form
+ *   or_rNNGP          RNG draw order by hand-rolled single steps, posterior
+ *                     recovery; dense-GP likelihood at m >= n-1
+ *   or_rNNGPPredict   dense GP kriging when m = n (z = 0 isolates the mean)
+ *   or_cNNGP          dense GLS when m >= n-1
+ * Parity unpinned (no R in this image): R's RNG stream, the exact bits of
+ * Rmath's bessel_k_ex / gammafn, R's LAPACK rounding order, and the order
+ * rsort_with_index gives to exactly tied distances.
+ */
+#ifndef NNGP_ORACLE_H
+#define NNGP_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_rng {
+  void *state;
+  double (*norm_rand)(void *state);
+  double (*unif_rand)(void *state);
+} or_rng;
+
+/* status codes (same numbers as the product's) */
+#define OR_OK 0
+#define OR_ERR_ARG 1
+#define OR_ERR_NOT_PD 2
+#define OR_ERR_COV_MODEL 3
+
+const char *or_last_error(void);
+
+/* src/util.cpp:55-70 */
+void or_getNNIndx(int i, int m, int *iNNIndx, int *iNN);
+
+/* src/util.cpp:51-53 */
+double or_dist2(double a1, double a2, double b1, double b2);
+
+/* K_nu(x), x > 0 (Rmath bessel_k_ex with expo = 1; Temme series / Steed
+ * continued fraction + upward recurrence). */
+double or_bessel_k(double x, double nu);
+
+/* src/util.cpp:208-245.  Returns NaN for a covModel outside 0..3. */
+double or_spCor(double D, double phi, double nu, int covModel);
+
+/* R's rsort_with_index (shell sort, gaps 1, 4, 13, ...; used at
+ * src/nn.cpp:104,176,197,243). */
+void or_rsort_with_index(double *x, int *indx, int n);
+
+/* src/nn.cpp:63-111 (brute force). */
+int or_mkNNIndx(int n, int m, const double *coords, int *nnIndx,
+                double *nnDist, int *nnIndxLU, int nThreads);
+
+/* src/nn.cpp:128-257 (code-book / u = x + y walk). */
+int or_mkNNIndxCB(int n, int m, const double *coords, int *nnIndx,
+                  double *nnDist, int *nnIndxLU, int nThreads);
+
+/* Exact k nearest of q query sites among n sites (RANN nn2 semantics,
+ * R/generics.R:642); ties by ascending index.  nnIndx0 q x k col-major. */
+int or_nn2(int n, const double *coords, int q, const double *coords0, int k,
+           int *nnIndx0, double *nnDist0);
+
+/* LAPACK/BLAS-style unblocked routines on the lower triangle,
+ * column-major, leading dimension n (dpotf2, dtrti2 + dlauu2, dsymv). */
+int or_potrf_L(int n, double *A);
+int or_potri_L(int n, double *A);
+void or_symv_L(int n, const double *A, const double *x, double *y);
+
+/* src/rNNGP.cpp:46-101.  theta = {sigma.sq, tau.sq, phi, nu}. */
+int or_updateBF(double *B, double *F, const double *coords,
+                const int *nnIndx, const int *nnIndxLU, int n, int m,
+                const double *theta, int covModel, int nThreads,
+                double *logDet);
+
+/* src/cNNGP.cpp:24-69. */
+int or_updateConjBF(double *B, double *F, const double *coords,
+                    const int *nnIndx, const int *nnIndxLU, int n, int m,
+                    double phi, double alpha, double nu, int covModel,
+                    int nThreads);
+
+/* src/util.cpp:261-280, summed serially in site order. */
+double or_Q(const double *B, const double *F, const double *u,
+            const double *v, int n, const int *nnIndx, const int *nnIndxLU);
+
+/* src/rNNGP.cpp:105-420 (+ updateRep, src/rNNGP.cpp:24-42). */
+int or_rNNGP(const double *y, const double *X, int p, int n, int m,
+             const double *coords, int covModel, const int *nnIndx,
+             const int *nnIndxLU, const double *sigmaSqIG,
+             const double *tauSqIG, const double *phiUnif,
+             const double *nuUnif, const double *betaStarting,
+             double sigmaSqStarting, double tauSqStarting,
+             double phiStarting, double nuStarting, double sigmaSqTuning,
+             double tauSqTuning, double phiTuning, double nuTuning,
+             int nSamples, int nThreads, int nRep, const int *repIndx,
+             const or_rng *rng, double *pBetaSamples, double *pThetaSamples,
+             double *yRepSamples, int *nAccept);
+
+/* src/rNNGPPredict.cpp:25-222 with z supplied site-major (z[i*S+s]). */
+int or_rNNGPPredict(const double *X, const double *y, const double *coords,
+                    int n, int p, int m, const double *X0,
+                    const double *coords0, int q, const int *nnIndx0,
+                    const double *betaSamples, const double *thetaSamples,
+                    int nSamples, int covModel, int nThreads, const double *z,
+                    double *pY0);
+
+/* src/cNNGP.cpp:73-358; ab is written as 2 x g. */
+int or_cNNGP(const double *y, const double *X, int p, int n,
+             const double *coords, const double *thetaAlpha,
+             const int *nnIndx, const int *nnIndxLU, const double *X0,
+             const double *coords0, int n0, const int *nnIndx0, int g, int m,
+             const double *sigmaSqIG, int covModel, int nThreads,
+             double *betaHat, double *ab, double *bBInv, double *bb,
+             double *y0Hat, double *y0HatVar);
+
+/* Sequential NNGP draw w_i = B_i . w_N(i) + sqrt(F_i) z_i with z supplied
+ * (src/rNNGP.cpp:24-42, src/rNNGPReplicated.cpp:183-199). */
+void or_nngp_draw(const double *B, const double *F, const int *nnIndx,
+                  const int *nnIndxLU, int n, const double *z, double *w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

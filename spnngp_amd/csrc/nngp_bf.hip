@@ -1,0 +1,425 @@
+// nngp_bf.hip — neighbor-system kernels: updateBF / updateConjBF
+// (src/rNNGP.cpp:46-101, src/cNNGP.cpp:24-69), the quadratic form Q
+// (src/util.cpp:261-280) and the fused sums the MCMC driver needs.
+//
+// Two updateBF kernels:
+//  * k_bf_lane<M, COV>: one site per lane, the whole M x M system in
+//    registers, fully unrolled (M <= 16, closed-form covariances).  Sites
+//    with fewer than M neighbors (the first m sites of the NNGP order) are
+//    padded with far-away points: their rows/columns of C are exactly zero
+//    off the diagonal, so the real block factors bit-for-bit as if unpadded
+//    and the padded coefficients come out exactly 0.
+//  * k_bf_coop: one site per wavefront, the matrix in LDS, one row per lane
+//    (any m <= 64, any covariance including the Bessel-K matern).
+// Both compute C = s2 R(N,N) + t2 I, c = s2 R(N,i), the Cholesky factor of C
+// (pivots kept as reciprocal square roots), v = L^-1 c, B = L^-T v and
+// F = s2 + t2 - v.v, i.e. the reference's potrf -> potri -> symv result
+// without forming the inverse.
+#include <climits>
+
+#include "nngp_kernels.hpp"
+
+namespace nngp {
+
+constexpr int kLaneBlock = 256;
+constexpr int kCoopWaves = 4;
+constexpr int kCoopMaxM = 64;
+constexpr int kRedBlock = 256;
+constexpr int kRedMaxBlocks = 1024;
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// Site with no neighbors (site 0): F = s2 + t2, e_j(i) = Z(i, j).
+__device__ inline void site_without_neighbors(const BfArgs &A, int i) {
+  A.F[i] = A.c.diag;
+  if (A.E)
+    for (int j = 0; j < A.nz; j++)
+      A.E[(size_t)j * A.n + i] = A.Z[(size_t)i * A.zs + j];
+}
+
+// ---------------------------------------------------------------------------
+// one site per lane
+// ---------------------------------------------------------------------------
+template <int M, int COV>
+__global__ __launch_bounds__(kLaneBlock) void k_bf_lane(BfArgs A) {
+  const int b = xcd_block((A.n + kLaneBlock - 1) / kLaneBlock);
+  if (b < 0) return;
+  const int i = b * kLaneBlock + (int)threadIdx.x;
+  if (i >= A.n) return;
+  const int cnt = A.lu[A.n + i];
+  if (cnt == 0) {
+    site_without_neighbors(A, i);
+    return;
+  }
+  const int off = A.lu[i];
+  const CovArgs &c = A.c;
+
+  // The index array carries 64 padding entries, so off + k is in bounds;
+  // entries past this site's count are not its neighbors and are replaced
+  // by index 0 before any use.
+  int nb[M];
+#pragma unroll
+  for (int k = 0; k < M; k++) nb[k] = A.nnIndx[off + k];
+  if (cnt < M) {
+#pragma unroll
+    for (int k = 0; k < M; k++)
+      if (k >= cnt) nb[k] = 0;
+  }
+  double px[M], py[M];
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const double2 q = A.xy[nb[k]];
+    px[k] = q.x;
+    py[k] = q.y;
+  }
+  if (cnt < M) {
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+      if (k >= cnt) {  // far away and far from each other: zero correlation
+        px[k] = 1e100 * (k + 1);
+        py[k] = -1e100 * (k + 1);
+      }
+    }
+  }
+  const double2 s = A.xy[i];
+
+  constexpr int T = M * (M + 1) / 2;
+  double a[T];  // packed lower triangle, a[k(k+1)/2 + l], l <= k
+  double v[M];
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    v[k] = s2_corr<COV>(sqdist(s.x, s.y, px[k], py[k]), c);
+#pragma unroll
+    for (int l = 0; l < k; l++)
+      a[k * (k + 1) / 2 + l] =
+          s2_corr<COV>(sqdist(px[k], py[k], px[l], py[l]), c);
+    a[k * (k + 1) / 2 + k] = c.diag;
+  }
+
+  double vv;
+  const bool ok = chol_solve<M>(a, v, vv);  // v <- B_i = C^-1 c
+
+  if (A.B) {
+#pragma unroll
+    for (int k = 0; k < M; k++)
+      if (k < cnt) A.B[off + k] = v[k];
+  }
+  A.F[i] = c.diag - vv;
+  if (A.E) {
+    // padded coefficients are exactly 0, so their (valid) rows add nothing
+    for (int j = 0; j < A.nz; j++) {
+      double e = A.Z[(size_t)i * A.zs + j];
+#pragma unroll
+      for (int k = 0; k < M; k++)
+        e = fma(-v[k], A.Z[(size_t)nb[k] * A.zs + j], e);
+      A.E[(size_t)j * A.n + i] = e;
+    }
+  }
+  if (!ok) atomicMin(A.err, i);
+}
+
+// ---------------------------------------------------------------------------
+// one site per wavefront (generic m, generic covariance)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int tri_row(int p) {
+  // largest k with k(k+1)/2 <= p
+  int k = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
+  while ((k + 1) * (k + 2) / 2 <= p) k++;
+  while (k * (k + 1) / 2 > p) k--;
+  return k;
+}
+
+// Per-wave LDS of the wavefront kernels, in doubles.
+__host__ __device__ inline int coop_wave_doubles(int m) {
+  return m * (m + 1) + 3 * m;
+}
+// Waves per block so that the block's LDS stays within 64 KB.
+static int coop_waves(int m) {
+  const int per = coop_wave_doubles(m) * (int)sizeof(double);
+  int w = 65536 / per;
+  return w < 1 ? 1 : (w > kCoopWaves ? kCoopWaves : w);
+}
+
+__global__ __launch_bounds__(64 * kCoopWaves) void k_bf_coop(BfArgs A) {
+  extern __shared__ double smem[];
+  const int nw = (int)blockDim.x >> 6;
+  const int ld = A.m + 1;
+  const int per_wave = coop_wave_doubles(A.m);
+  const int w = (int)threadIdx.x >> 6;
+  const int t = (int)threadIdx.x & 63;
+  double *Am = smem + (size_t)w * per_wave;  // row-major, Am[k*ld + l]
+  double *cv = Am + A.m * ld;
+  double *px = cv + A.m;
+  double *py = px + A.m;
+
+  const int nlog = (A.n + nw - 1) / nw;
+  const int b = xcd_block(nlog);
+  if (b < 0) return;
+  const int i = b * nw + w;
+  if (i >= A.n) return;  // whole wave leaves; no block-level barrier below
+  const CovArgs &c = A.c;
+  const int cnt = A.lu[A.n + i];
+  if (cnt == 0) {
+    if (t == 0) site_without_neighbors(A, i);
+    return;
+  }
+  const int off = A.lu[i];
+  const double2 s = A.xy[i];
+  int myid = 0;
+  if (t < cnt) {
+    myid = A.nnIndx[off + t];
+    const double2 q = A.xy[myid];
+    px[t] = q.x;
+    py[t] = q.y;
+  }
+  wave_sync();
+  // augmented lower triangle: rows 0..cnt-1 are C, row cnt is c
+  const int np = (cnt + 1) * (cnt + 2) / 2 - 1;
+  for (int p = t; p < np; p += 64) {
+    const int k = tri_row(p);
+    const int l = p - k * (k + 1) / 2;
+    if (k < cnt && k == l) {
+      Am[k * ld + k] = c.diag;
+    } else {
+      const double ax = k < cnt ? px[k] : s.x;
+      const double ay = k < cnt ? py[k] : s.y;
+      const double val = s2_corr_rt(sqdist(ax, ay, px[l], py[l]), c);
+      if (k < cnt)
+        Am[k * ld + l] = val;
+      else
+        cv[l] = val;
+    }
+  }
+  wave_sync();
+  // right-looking Cholesky, lane t owns row t; diagonal keeps 1/L_jj
+  bool ok = true;
+  for (int j = 0; j < cnt; j++) {
+    const double piv = Am[j * ld + j];
+    ok = ok && (piv > 0.0);
+    if (!ok) break;
+    const double r = nn_rsqrt(piv);
+    double ltj = 0.0;
+    if (t > j && t < cnt) {
+      ltj = Am[t * ld + j] * r;
+      Am[t * ld + j] = ltj;
+    }
+    wave_sync();
+    if (t == j) Am[j * ld + j] = r;
+    if (t > j && t < cnt) {
+      for (int l = j + 1; l <= t; l++)
+        Am[t * ld + l] = fma(-ltj, Am[l * ld + j], Am[t * ld + l]);
+    }
+    wave_sync();
+  }
+  if (!ok) {
+    if (t == 0) atomicMin(A.err, i);
+    return;
+  }
+  // forward solve: lane t holds x_t
+  double x = t < cnt ? cv[t] : 0.0;
+  for (int k = 0; k < cnt; k++) {
+    const double vk = readlane_d(x, k) * Am[k * ld + k];
+    if (t == k) x = vk;
+    if (t > k && t < cnt) x = fma(-Am[t * ld + k], vk, x);
+  }
+  double vv = t < cnt ? x * x : 0.0;
+  for (int o = 32; o > 0; o >>= 1) vv += __shfl_xor(vv, o);
+  // back solve B = L^-T v
+  for (int k = cnt - 1; k >= 0; k--) {
+    const double bk = readlane_d(x, k) * Am[k * ld + k];
+    if (t == k) x = bk;
+    if (t < k) x = fma(-Am[k * ld + t], bk, x);
+  }
+  if (A.B && t < cnt) A.B[off + t] = x;
+  if (t == 0) A.F[i] = c.diag - vv;
+  if (A.E) {
+    for (int j = 0; j < A.nz; j++) {
+      double part = t < cnt ? x * A.Z[(size_t)myid * A.zs + j] : 0.0;
+      for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+      if (t == 0)
+        A.E[(size_t)j * A.n + i] = A.Z[(size_t)i * A.zs + j] - part;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// dispatch
+// ---------------------------------------------------------------------------
+template <int M>
+static void launch_lane_m(const BfArgs &a, int cov, hipStream_t st) {
+  const int nlog = (a.n + kLaneBlock - 1) / kLaneBlock;
+  const dim3 grid(8 * ((nlog + 7) / 8));
+  switch (cov) {
+    case 0: hipLaunchKernelGGL((k_bf_lane<M, 0>), grid, dim3(kLaneBlock), 0, st, a); break;
+    case 1: hipLaunchKernelGGL((k_bf_lane<M, 1>), grid, dim3(kLaneBlock), 0, st, a); break;
+    default: hipLaunchKernelGGL((k_bf_lane<M, 3>), grid, dim3(kLaneBlock), 0, st, a); break;
+  }
+}
+
+int bf_launch(const BfArgs &a, hipStream_t st) {
+  if (a.n <= 0) return 0;
+  const int cov = a.c.cov;
+  if (cov != 2 && a.m <= 16) {
+    if (a.m <= 4) launch_lane_m<4>(a, cov, st);
+    else if (a.m <= 8) launch_lane_m<8>(a, cov, st);
+    else if (a.m <= 10) launch_lane_m<10>(a, cov, st);
+    else if (a.m <= 12) launch_lane_m<12>(a, cov, st);
+    else if (a.m <= 15) launch_lane_m<15>(a, cov, st);
+    else launch_lane_m<16>(a, cov, st);
+  } else {
+    if (a.m > kCoopMaxM) return -1;
+    const int nw = coop_waves(a.m);
+    const int nlog = (a.n + nw - 1) / nw;
+    const dim3 grid(8 * ((nlog + 7) / 8));
+    const size_t shm = (size_t)nw * coop_wave_doubles(a.m) * sizeof(double);
+    hipLaunchKernelGGL(k_bf_coop, grid, dim3(64 * nw), shm, st, a);
+  }
+  return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// deterministic sums: fixed grid, fixed-order tree inside the block, one
+// partial per block, then a single block adds the partials in block order
+// ---------------------------------------------------------------------------
+template <int NV>
+__device__ inline void block_sum_store(double (&acc)[NV], double *dst) {
+  __shared__ double red[kRedBlock / 64][NV];
+  const int w = (int)threadIdx.x >> 6, t = (int)threadIdx.x & 63;
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    double x = acc[v];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    if (t == 0) red[w][v] = x;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < NV) {
+    double s = 0.0;
+    for (int k = 0; k < kRedBlock / 64; k++) s += red[k][threadIdx.x];
+    dst[threadIdx.x] = s;
+  }
+}
+
+__global__ void k_sum_partials(const double *partials, int nblocks, int nv,
+                               int stride, double *out) {
+  const int v = (int)threadIdx.x;
+  if (v >= nv) return;
+  double s = 0.0;
+  for (int b = 0; b < nblocks; b++) s += partials[(size_t)b * stride + v];
+  out[v] = s;
+}
+
+static int red_blocks(int n) {
+  int b = (n + kRedBlock - 1) / kRedBlock;
+  return b < 1 ? 1 : (b > kRedMaxBlocks ? kRedMaxBlocks : b);
+}
+int stats_blocks(int n) { return red_blocks(n); }
+int q_blocks(int n) { return red_blocks(n); }
+
+template <int NZ>
+__global__ __launch_bounds__(kRedBlock) void k_stats(StatsArgs A,
+                                                     double *partials) {
+  constexpr int NG = NZ * (NZ + 1) / 2;
+  constexpr int NV = 2 + NG;
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) acc[v] = 0.0;
+  for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < A.n;
+       i += gridDim.x * kRedBlock) {
+    const double f = A.F[i];
+    acc[0] += log(f);
+    if constexpr (NZ > 0) {
+      double e[NZ];
+#pragma unroll
+      for (int j = 0; j < NZ; j++) e[j] = A.E[(size_t)j * A.n + i];
+      if (A.want_q) {
+        double er = -e[NZ - 1];
+#pragma unroll
+        for (int j = 0; j < NZ - 1; j++) er = fma(A.beta[j], e[j], er);
+        acc[1] += er * er / f;
+      }
+      int g = 2;
+#pragma unroll
+      for (int p = 0; p < NZ; p++)
+#pragma unroll
+        for (int q = 0; q <= p; q++) acc[g++] += e[p] * e[q] / f;
+    }
+  }
+  block_sum_store<NV>(acc, partials + (size_t)blockIdx.x * kStatsMax);
+}
+
+int stats_launch(const StatsArgs &a, double *partials, double *out,
+                 hipStream_t st) {
+  const int nb = red_blocks(a.n);
+  const int nz = a.E ? a.nz : 0;
+  switch (nz) {
+#define NNGP_STATS_CASE(K) \
+  case K: hipLaunchKernelGGL(k_stats<K>, dim3(nb), dim3(kRedBlock), 0, st, a, partials); break;
+    NNGP_STATS_CASE(0)
+    NNGP_STATS_CASE(2)
+    NNGP_STATS_CASE(3)
+    NNGP_STATS_CASE(4)
+    NNGP_STATS_CASE(5)
+    NNGP_STATS_CASE(6)
+    NNGP_STATS_CASE(7)
+    NNGP_STATS_CASE(8)
+    NNGP_STATS_CASE(9)
+#undef NNGP_STATS_CASE
+    default: return -1;
+  }
+  const int nv = 2 + nz * (nz + 1) / 2;
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, partials, nb,
+                     nv, kStatsMax, out);
+  return (int)hipGetLastError();
+}
+
+__global__ __launch_bounds__(kRedBlock) void k_quadform(QArgs A,
+                                                        double *partials) {
+  double acc[1] = {0.0};
+  for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < A.n;
+       i += gridDim.x * kRedBlock) {
+    const int cnt = A.lu[A.n + i], off = A.lu[i];
+    double a = 0.0, b = 0.0;
+    for (int j = 0; j < cnt; j++) {
+      const double bj = A.B[off + j];
+      const int id = A.nnIndx[off + j];
+      a += bj * A.u[id];
+      b += bj * A.v[id];
+    }
+    acc[0] += (A.u[i] - a) * (A.v[i] - b) / A.F[i];
+  }
+  block_sum_store<1>(acc, partials + blockIdx.x);
+}
+
+int q_launch(const QArgs &a, double *partials, double *out, hipStream_t st) {
+  const int nb = red_blocks(a.n);
+  hipLaunchKernelGGL(k_quadform, dim3(nb), dim3(kRedBlock), 0, st, a,
+                     partials);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, partials, nb, 1,
+                     1, out);
+  return (int)hipGetLastError();
+}
+
+__global__ void k_interleave(const double *cm, int n, double2 *xy) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) xy[i] = make_double2(cm[i], cm[(size_t)n + i]);
+}
+
+int interleave_launch(const double *coords_cm, int n, double2 *xy,
+                      hipStream_t st) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_interleave, dim3((n + 255) / 256), dim3(256), 0, st,
+                     coords_cm, n, xy);
+  return (int)hipGetLastError();
+}
+
+}  // namespace nngp

@@ -1,0 +1,244 @@
+// nngp_device.hpp — device-side math shared by the NNGP kernels (gfx950).
+//
+// Covariance functions follow src/util.cpp:208-245 (spCor) of spNNGP 1.0.1:
+//   0 exponential  exp(-phi D)
+//   1 spherical    1 - 1.5 phi D + 0.5 (phi D)^3 for 0 < D <= 1/phi, 0 beyond
+//   2 matern       (D phi)^nu / (2^(nu-1) Gamma(nu)) K_nu(D phi)
+//   3 gaussian     exp(-(phi D)^2)
+// every one scaled by s2 (sigma.sq; 1 on the correlation scale of cNNGP).
+// The host fills CovArgs once per theta, so per-pair work is only the
+// distance, the special function and the scale.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nngp {
+
+constexpr int kExpDeg = 11;  // Taylor degree of exp on |r| <= ln2/2
+
+struct CovArgs {
+  double s2;       // sigma.sq (1 on the correlation scale)
+  double t2;       // tau.sq (alpha on the correlation scale)
+  double phi;
+  double nu;
+  double diag;     // s2 * rho(0) + t2  (rho(0) == 1 for every model)
+  double inv_phi;  // 1.0 / phi, the spherical support (src/util.cpp:221)
+  double phi2;     // phi * phi (gaussian on squared distances)
+  double log2e;    // 1 / ln 2
+  double ln2_hi;   // ln 2 split so that k * ln2_hi is exact for |k| < 2^20
+  double ln2_lo;
+  double ec[kExpDeg + 1];  // s2 / k!: exp polynomial with the scale folded in
+  // matern (Temme's method; see matern_s2 below)
+  double mnorm;    // s2 / (2^(nu-1) Gamma(nu))
+  double mu;       // nu - nl, in [-1/2, 1/2)
+  double g1, g2;   // Gamma_1(mu), Gamma_2(mu)
+  double rgp;      // 1 / Gamma(1 + mu)
+  double rgm;      // 1 / Gamma(1 - mu)
+  int nl;          // round(nu)
+  int cov;         // covModel
+};
+
+// ---------------------------------------------------------------------------
+// elementary functions
+// ---------------------------------------------------------------------------
+
+// sqrt(s) for s >= 0: hardware reciprocal square root and one Newton step
+// (relative error below 1e-14; exact 0 for s == 0).
+__device__ __forceinline__ double nn_sqrt(double s) {
+  double y = __builtin_amdgcn_rsq(fmax(s, 1e-300));
+  double d = s * y;
+  double r = fma(-d, d, s);
+  return fma(r * y, 0.5, d);
+}
+
+// 1/sqrt(a) for a > 0 to full double precision (two Newton steps).
+__device__ __forceinline__ double nn_rsqrt(double a) {
+  double y = __builtin_amdgcn_rsq(a);
+  double e = fma(-(a * y), y, 1.0);
+  y = fma(y * e, 0.5, y);
+  e = fma(-(a * y), y, 1.0);
+  return fma(y * e, 0.5, y);
+}
+
+// s2 * exp(x) for x <= 0 (large negative x gives 0).  Range reduction
+// x = k ln2 + r, |r| <= ln2/2, Taylor polynomial of degree 11 whose
+// coefficients s2/j! live in SGPRs (relative error ~1e-15).
+__device__ __forceinline__ double s2_exp(double x, const CovArgs &a) {
+  x = fmax(x, -1000.0);
+  double k = __builtin_rint(x * a.log2e);
+  double r = fma(k, -a.ln2_hi, x);
+  r = fma(k, -a.ln2_lo, r);
+  double p = a.ec[11];
+#pragma unroll
+  for (int j = 10; j >= 0; j--) p = fma(p, r, a.ec[j]);
+  return ldexp(p, (int)k);
+}
+
+// K_nu(x) for x > 0 by Temme's series (x <= 2) or Steed's continued
+// fraction (x > 2) for K_mu, K_{mu+1}, |mu| <= 1/2, then upward recurrence.
+__device__ inline double bessel_k(double x, const CovArgs &a) {
+  const double eps = 1e-17;
+  const double pi = 3.14159265358979323846;
+  const double mu = a.mu;
+  double kmu, kmu1;
+  if (x <= 2.0) {
+    double half = 0.5 * x;
+    double lg = -log(half);
+    double sg = mu * lg;
+    double pm = pi * mu;
+    double fa = fabs(pm) < 1e-300 ? 1.0 : pm / sin(pm);
+    double fs = fabs(sg) < 1e-300 ? 1.0 : sinh(sg) / sg;
+    double f = fa * (a.g1 * cosh(sg) + a.g2 * fs * lg);
+    double es = exp(sg);
+    double pk = 0.5 * es / a.rgp;
+    double qk = 0.5 / (es * a.rgm);
+    double ck = 1.0, d = half * half;
+    double s0 = f, s1 = pk;
+    for (int k = 1; k < 200; k++) {
+      f = (k * f + pk + qk) / ((double)k * k - mu * mu);
+      ck *= d / k;
+      pk /= (k - mu);
+      qk /= (k + mu);
+      double t0 = ck * f;
+      s0 += t0;
+      s1 += ck * (pk - k * f);
+      if (fabs(t0) < fabs(s0) * eps) break;
+    }
+    kmu = s0;
+    kmu1 = s1 * (2.0 / x);
+  } else {
+    double b = 2.0 * (1.0 + x);
+    double d = 1.0 / b;
+    double h = d, dh = d;
+    double q1 = 0.0, q2 = 1.0;
+    double a1 = 0.25 - mu * mu;
+    double q = a1, c = a1;
+    double aa = -a1;
+    double s = 1.0 + q * dh;
+    for (int i = 2; i < 2000; i++) {
+      aa -= 2.0 * (i - 1);
+      c = -aa * c / i;
+      double qn = (q1 - b * q2) / aa;
+      q1 = q2;
+      q2 = qn;
+      q += c * qn;
+      b += 2.0;
+      d = 1.0 / (b + aa * d);
+      dh = (b * d - 1.0) * dh;
+      h += dh;
+      double ds = q * dh;
+      s += ds;
+      if (fabs(ds / s) < eps) break;
+    }
+    h = a1 * h;
+    kmu = sqrt(pi / (2.0 * x)) * exp(-x) / s;
+    kmu1 = kmu * (mu + x + 0.5 - h) / x;
+  }
+  for (int i = 1; i <= a.nl; i++) {
+    double kn = (mu + i) * (2.0 / x) * kmu1 + kmu;
+    kmu = kmu1;
+    kmu1 = kn;
+  }
+  return kmu;
+}
+
+// s2 * matern correlation at distance d (src/util.cpp:228-237).
+__device__ inline double matern_s2(double d, const CovArgs &a) {
+  double x = d * a.phi;
+  if (!(x > 0.0)) return a.s2;
+  if (x > 700.0) return 0.0;  // K_nu(x) x^nu underflows long before this
+  return a.mnorm * pow(x, a.nu) * bessel_k(x, a);
+}
+
+// s2 * rho(D) from the squared distance d2 (COV is the covModel).
+template <int COV>
+__device__ __forceinline__ double s2_corr(double d2, const CovArgs &a) {
+  if constexpr (COV == 0) {
+    return s2_exp(-a.phi * nn_sqrt(d2), a);
+  } else if constexpr (COV == 1) {
+    double D = nn_sqrt(d2);
+    double t = a.phi * D;
+    double v = a.s2 * (1.0 - 1.5 * t + 0.5 * (t * t * t));
+    return (D > 0.0 && D <= a.inv_phi) ? v : (D >= a.inv_phi ? 0.0 : a.s2);
+  } else if constexpr (COV == 2) {
+    return matern_s2(nn_sqrt(d2), a);
+  } else {
+    return s2_exp(-a.phi2 * d2, a);
+  }
+}
+
+// Runtime-dispatched variant for the generic (non-templated) kernels.
+__device__ inline double s2_corr_rt(double d2, const CovArgs &a) {
+  switch (a.cov) {
+    case 0: return s2_corr<0>(d2, a);
+    case 1: return s2_corr<1>(d2, a);
+    case 2: return s2_corr<2>(d2, a);
+    default: return s2_corr<3>(d2, a);
+  }
+}
+
+// Squared distance of two points.
+__device__ __forceinline__ double sqdist(double ax, double ay, double bx,
+                                         double by) {
+  double dx = ax - bx, dy = ay - by;
+  return fma(dx, dx, dy * dy);
+}
+
+// Packed lower triangle index, a[k(k+1)/2 + l] for l <= k.
+__host__ __device__ constexpr int tri(int k, int l) { return k * (k + 1) / 2 + l; }
+
+// In-register solve of one M x M SPD system held as a packed lower triangle
+// `a` (overwritten by the Cholesky factor, diagonal as 1/L_jj) with
+// right-hand side `v` (overwritten by C^-1 v).  vv = v^T C^-1 v.  Returns
+// false when a pivot is not positive (the reference's dpotrf failure).
+template <int M>
+__device__ __forceinline__ bool chol_solve(double (&a)[M * (M + 1) / 2],
+                                           double (&v)[M], double &vv) {
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < M; j++) {
+    const double piv = a[tri(j, j)];
+    ok = ok && (piv > 0.0);
+    const double r = nn_rsqrt(piv);
+    a[tri(j, j)] = r;
+#pragma unroll
+    for (int k = j + 1; k < M; k++) a[tri(k, j)] *= r;
+#pragma unroll
+    for (int k = j + 1; k < M; k++) {
+      const double lkj = a[tri(k, j)];
+#pragma unroll
+      for (int l = j + 1; l <= k; l++)
+        a[tri(k, l)] = fma(-lkj, a[tri(l, j)], a[tri(k, l)]);
+    }
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    double t = v[k];
+#pragma unroll
+    for (int l = 0; l < k; l++) t = fma(-a[tri(k, l)], v[l], t);
+    t *= a[tri(k, k)];
+    v[k] = t;
+    s = fma(t, t, s);
+  }
+#pragma unroll
+  for (int k = M - 1; k >= 0; k--) {
+    const double t = v[k] * a[tri(k, k)];
+    v[k] = t;
+#pragma unroll
+    for (int l = 0; l < k; l++) v[l] = fma(-a[tri(k, l)], t, v[l]);
+  }
+  vv = s;
+  return ok;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs of an MI355X; give each
+// XCD a contiguous range of logical blocks so that neighbor gathers of one
+// XCD stay in its own L2.  grid = 8 * per; returns -1 for padding blocks.
+__device__ __forceinline__ int xcd_block(int nlogical) {
+  int per = (nlogical + 7) >> 3;
+  int b = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+  return b < nlogical ? b : -1;
+}
+
+}  // namespace nngp

@@ -1,0 +1,398 @@
+// nngp_host.hip — host helpers and the device-resident model of
+// libnngp_hip.so (see nngp_host.hpp).
+#include <algorithm>
+#include <climits>
+#include <cstring>
+
+#include "nngp_host.hpp"
+
+namespace nngp {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+int set_err(int code, const char *fmt, ...) {
+  char buf[768];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+void clear_err() { g_err.clear(); }
+const char *last_error() { return g_err.c_str(); }
+
+int use_device(const nngp_opts *opts) {
+  const int dev = opts ? opts->device : 0;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0)
+    return set_err(NNGP_ERR_HIP, "no HIP device available (%s)",
+                   e != hipSuccess ? hipGetErrorString(e) : "0 devices");
+  if (dev < 0 || dev >= count)
+    return set_err(NNGP_ERR_ARG, "device %d out of range (%d visible)", dev,
+                   count);
+  NNGP_HIP(hipSetDevice(dev));
+  return NNGP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// covariance parameters
+// ---------------------------------------------------------------------------
+
+// Coefficients of 1/Gamma(z) = sum_k c_k z^k (Abramowitz & Stegun 6.1.34),
+// even k, for Gamma_1(mu) = -(c2 + c4 mu^2 + ...) near mu = 0.
+static const double kRg[] = {0.5772156649015329,  -0.0420026350340952,
+                             -0.0421977345555443, 0.0072189432466630,
+                             -0.0002152416741149, -0.0000201348547807,
+                             0.0000011330272320};
+
+CovArgs make_cov(int cov, double s2, double t2, double phi, double nu) {
+  CovArgs c;
+  std::memset(&c, 0, sizeof c);
+  c.cov = cov;
+  c.s2 = s2;
+  c.t2 = t2;
+  c.phi = phi;
+  c.nu = nu;
+  c.diag = s2 * 1.0 + t2;
+  c.inv_phi = 1.0 / phi;
+  c.phi2 = phi * phi;
+  const long double ln2 = logl(2.0L);
+  double hi = (double)ln2;
+  uint64_t bits;
+  std::memcpy(&bits, &hi, 8);
+  bits &= 0xFFFFFFFF00000000ull;  // 21 significant bits: k*hi is exact
+  std::memcpy(&hi, &bits, 8);
+  c.ln2_hi = hi;
+  c.ln2_lo = (double)(ln2 - (long double)hi);
+  c.log2e = (double)(1.0L / ln2);
+  double fact = 1.0;
+  for (int k = 0; k <= kExpDeg; k++) {
+    if (k > 0) fact *= k;
+    c.ec[k] = s2 / fact;
+  }
+  if (cov == 2) {
+    const int nl = (int)(nu + 0.5);
+    const double mu = nu - nl;
+    c.nl = nl;
+    c.mu = mu;
+    const double rp = 1.0 / std::tgamma(1.0 + mu);
+    const double rm = 1.0 / std::tgamma(1.0 - mu);
+    c.rgp = rp;
+    c.rgm = rm;
+    c.g2 = 0.5 * (rm + rp);
+    if (std::fabs(mu) < 0.1) {
+      const double u = mu * mu;
+      double g = kRg[6];
+      for (int k = 5; k >= 0; k--) g = g * u + kRg[k];
+      c.g1 = -g;
+    } else {
+      c.g1 = (rm - rp) / (2.0 * mu);
+    }
+    c.mnorm = s2 / (std::pow(2.0, nu - 1.0) * std::tgamma(nu));
+  }
+  return c;
+}
+
+// ---------------------------------------------------------------------------
+// argument checks
+// ---------------------------------------------------------------------------
+long long n_indx(int n, int m) {
+  // src/rNNGP.cpp:218
+  return (long long)((double)(1 + m) / 2 * m + (double)(n - m - 1) * m);
+}
+
+int check_neighbors(int n, int m, const int *nnIndx, const int *lu) {
+  if (!nnIndx || !lu) return set_err(NNGP_ERR_ARG, "nnIndx/nnIndxLU is NULL");
+  const long long ni = n_indx(n, m);
+  for (int i = 0; i < n; i++) {
+    const long long off = lu[i], cnt = lu[(size_t)n + i];
+    if (cnt < 0 || cnt > m || off < 0 || off + cnt > ni)
+      return set_err(NNGP_ERR_ARG,
+                     "malformed nnIndxLU at site %d (offset %lld, count %lld)",
+                     i, off, cnt);
+    for (long long j = 0; j < cnt; j++) {
+      const int id = nnIndx[off + j];
+      if (id < 0 || id >= n)
+        return set_err(NNGP_ERR_ARG,
+                       "nnIndx entry %lld of site %d is %d, outside [0, n)",
+                       j, i, id);
+    }
+  }
+  return NNGP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// small dense algebra: LAPACK dpotf2 / dtrti2 + dlauu2, BLAS dsymv / dtrmv
+// loop orders, lower triangle, lda = n
+// ---------------------------------------------------------------------------
+#define A_(i, j) A[(size_t)(j) * n + (i)]
+int h_potrf_L(int n, double *A) {
+  for (int j = 0; j < n; j++) {
+    double dot = 0.0;
+    for (int k = 0; k < j; k++) dot += A_(j, k) * A_(j, k);
+    double ajj = A_(j, j) - dot;
+    if (ajj <= 0.0 || std::isnan(ajj)) return j + 1;
+    ajj = std::sqrt(ajj);
+    A_(j, j) = ajj;
+    for (int k = 0; k < j; k++) {
+      const double t = -A_(j, k);
+      for (int i = j + 1; i < n; i++) A_(i, j) += t * A_(i, k);
+    }
+    const double r = 1.0 / ajj;
+    for (int i = j + 1; i < n; i++) A_(i, j) *= r;
+  }
+  return 0;
+}
+
+int h_potri_L(int n, double *A) {
+  for (int j = 0; j < n; j++)
+    if (A_(j, j) == 0.0) return j + 1;
+  for (int j = n - 1; j >= 0; j--) {
+    A_(j, j) = 1.0 / A_(j, j);
+    const double ajj = -A_(j, j);
+    const int nn = n - j - 1;
+    for (int jj = nn - 1; jj >= 0; jj--) {
+      const double t = A_(j + 1 + jj, j);
+      if (t != 0.0) {
+        for (int ii = nn - 1; ii > jj; ii--)
+          A_(j + 1 + ii, j) += t * A_(j + 1 + ii, j + 1 + jj);
+        A_(j + 1 + jj, j) = t * A_(j + 1 + jj, j + 1 + jj);
+      }
+    }
+    for (int ii = 0; ii < nn; ii++) A_(j + 1 + ii, j) *= ajj;
+  }
+  for (int i = 0; i < n; i++) {
+    const double aii = A_(i, i);
+    if (i < n - 1) {
+      double dot = 0.0;
+      for (int k = i; k < n; k++) dot += A_(k, i) * A_(k, i);
+      A_(i, i) = dot;
+      for (int jj = 0; jj < i; jj++) {
+        double t = 0.0;
+        for (int k = i + 1; k < n; k++) t += A_(k, jj) * A_(k, i);
+        A_(i, jj) = t + aii * A_(i, jj);
+      }
+    } else {
+      for (int jj = 0; jj <= i; jj++) A_(i, jj) *= aii;
+    }
+  }
+  return 0;
+}
+
+void h_symv_L(int n, const double *A, const double *x, double *y) {
+  for (int i = 0; i < n; i++) y[i] = 0.0;
+  for (int j = 0; j < n; j++) {
+    const double t1 = x[j];
+    double t2 = 0.0;
+    y[j] += t1 * A_(j, j);
+    for (int i = j + 1; i < n; i++) {
+      y[i] += t1 * A_(i, j);
+      t2 += A_(i, j) * x[i];
+    }
+    y[j] += t2;
+  }
+}
+
+void h_trmv_L(int n, const double *A, double *x) {
+  for (int j = n - 1; j >= 0; j--) {
+    const double t = x[j];
+    if (t != 0.0) {
+      for (int i = n - 1; i > j; i--) x[i] += t * A_(i, j);
+      x[j] = t * A_(j, j);
+    }
+  }
+}
+#undef A_
+
+// ---------------------------------------------------------------------------
+// built-in generator
+// ---------------------------------------------------------------------------
+static uint64_t splitmix64(uint64_t &x) {
+  uint64_t z = (x += 0x9e3779b97f4a7c15ull);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+static inline uint64_t rotl(uint64_t x, int k) {
+  return (x << k) | (x >> (64 - k));
+}
+BuiltinRng::BuiltinRng(uint64_t seed) {
+  uint64_t x = seed;
+  for (int k = 0; k < 4; k++) s[k] = splitmix64(x);
+}
+uint64_t BuiltinRng::next() {
+  const uint64_t r = rotl(s[1] * 5, 7) * 9;
+  const uint64_t t = s[1] << 17;
+  s[2] ^= s[0];
+  s[3] ^= s[1];
+  s[1] ^= s[2];
+  s[0] ^= s[3];
+  s[2] ^= t;
+  s[3] = rotl(s[3], 45);
+  return r;
+}
+double BuiltinRng::unif() {
+  return ((double)(next() >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+double BuiltinRng::norm() {
+  if (have) {
+    have = 0;
+    return cached;
+  }
+  double u, v, r;
+  do {
+    u = 2.0 * unif() - 1.0;
+    v = 2.0 * unif() - 1.0;
+    r = u * u + v * v;
+  } while (r >= 1.0 || r == 0.0);
+  const double f = std::sqrt(-2.0 * std::log(r) / r);
+  cached = v * f;
+  have = 1;
+  return u * f;
+}
+
+// ---------------------------------------------------------------------------
+// model
+// ---------------------------------------------------------------------------
+__global__ void k_fill_int(int *p, int v) { *p = v; }
+
+int fill_int_launch(int *p, int v, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(1), 0, st, p, v);
+  return (int)hipGetLastError();
+}
+
+int Model::create(int n_, int m_, int p_, const double *coords,
+                  const int *nnI, const int *LU, const double *X,
+                  const double *y, int cov_, const nngp_opts *opts) {
+  n = n_;
+  m = m_;
+  p = p_;
+  cov = cov_;
+  precision = opts ? opts->precision : 0;
+  nz = (X && y) ? p + 1 : 0;
+  zs = (nz + 1) & ~1;
+  nIndx = n_indx(n, m);
+  NNGP_TRY(st.create());
+  {
+    DevBuf tmp;
+    NNGP_TRY(tmp.alloc(sizeof(double) * 2 * (size_t)n));
+    NNGP_HIP(hipMemcpyAsync(tmp.p, coords, sizeof(double) * 2 * (size_t)n,
+                            hipMemcpyHostToDevice, st.s));
+    NNGP_TRY(xy.alloc(sizeof(double2) * (size_t)n));
+    if (interleave_launch(tmp.as<double>(), n, xy.as<double2>(), st.s))
+      return set_err(NNGP_ERR_HIP, "coordinate upload kernel failed");
+    NNGP_HIP(hipStreamSynchronize(st.s));
+  }
+  // the index array carries 64 zero entries of padding: the per-lane kernel
+  // reads a full block of M <= 16 indices at every offset
+  NNGP_TRY(nnIndx.alloc(sizeof(int) * ((size_t)nIndx + 64)));
+  NNGP_HIP(hipMemsetAsync(nnIndx.p, 0, nnIndx.bytes, st.s));
+  NNGP_HIP(hipMemcpyAsync(nnIndx.p, nnI, sizeof(int) * (size_t)nIndx,
+                          hipMemcpyHostToDevice, st.s));
+  NNGP_TRY(lu.alloc(sizeof(int) * 2 * (size_t)n));
+  NNGP_HIP(hipMemcpyAsync(lu.p, LU, sizeof(int) * 2 * (size_t)n,
+                          hipMemcpyHostToDevice, st.s));
+  if (nz) {
+    std::vector<double> hz((size_t)n * zs, 0.0);
+    for (int i = 0; i < n; i++) {
+      for (int j = 0; j < p; j++) hz[(size_t)i * zs + j] = X[(size_t)j * n + i];
+      hz[(size_t)i * zs + p] = y[i];
+    }
+    NNGP_TRY(Z.alloc(sizeof(double) * hz.size()));
+    NNGP_HIP(hipMemcpyAsync(Z.p, hz.data(), sizeof(double) * hz.size(),
+                            hipMemcpyHostToDevice, st.s));
+    NNGP_HIP(hipStreamSynchronize(st.s));
+  }
+  for (int s = 0; s < 2; s++) {
+    NNGP_TRY(F[s].alloc(sizeof(double) * (size_t)n));
+    if (nz) NNGP_TRY(E[s].alloc(sizeof(double) * (size_t)nz * n));
+  }
+  NNGP_TRY(partials.alloc(sizeof(double) * (size_t)stats_blocks(n) *
+                          kStatsMax));
+  NNGP_TRY(sums.alloc(sizeof(double) * 2 * kStatsMax + 64));
+  NNGP_TRY(err.alloc(sizeof(int)));
+  NNGP_TRY(hsums.alloc(sizeof(double) * 2 * kStatsMax + 64));
+  NNGP_HIP(hipStreamSynchronize(st.s));
+  return NNGP_OK;
+}
+
+int Model::bf(const CovArgs &c, int slot, bool storeB) {
+  if (storeB && !B[slot].p)
+    NNGP_TRY(B[slot].alloc(sizeof(double) * (size_t)std::max(nIndx, 1LL)));
+  BfArgs a;
+  a.xy = xy.as<double2>();
+  a.nnIndx = nnIndx.as<int>();
+  a.lu = lu.as<int>();
+  a.n = n;
+  a.m = m;
+  a.B = storeB ? B[slot].as<double>() : nullptr;
+  a.F = F[slot].as<double>();
+  a.Z = nz ? Z.as<double>() : nullptr;
+  a.nz = nz;
+  a.zs = zs;
+  a.E = nz ? E[slot].as<double>() : nullptr;
+  a.err = err.as<int>();
+  a.c = c;
+  if (fill_int_launch(a.err, INT_MAX, st.s))
+    return set_err(NNGP_ERR_HIP, "flag reset kernel failed");
+  const int rc = bf_launch(a, st.s);
+  if (rc == -1)
+    return set_err(NNGP_ERR_UNSUPPORTED, "n.neighbors = %d exceeds 64", m);
+  if (rc) return set_err(NNGP_ERR_HIP, "updateBF launch failed: %s",
+                         hipGetErrorString((hipError_t)rc));
+  return NNGP_OK;
+}
+
+int Model::stats(int slot, const double *beta, int off) {
+  StatsArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.E = nz ? E[slot].as<double>() : nullptr;
+  a.F = F[slot].as<double>();
+  a.n = n;
+  a.nz = nz;
+  a.want_q = beta != nullptr && nz > 0;
+  if (beta)
+    for (int j = 0; j < p && j < kMaxP; j++) a.beta[j] = beta[j];
+  const int rc = stats_launch(a, partials.as<double>(),
+                              sums.as<double>() + off, st.s);
+  if (rc) return set_err(NNGP_ERR_HIP, "sum kernel launch failed");
+  return NNGP_OK;
+}
+
+int Model::check_err(const char *what) {
+  const int bad = *(reinterpret_cast<int *>(hsums.as<double>() +
+                                            2 * kStatsMax));
+  if (bad != INT_MAX)
+    return set_err(NNGP_ERR_NOT_PD,
+                   "c++ Rf_error: dpotrf failed\n(%s: the neighbor covariance"
+                   " of site %d is not positive definite)",
+                   what, bad);
+  return NNGP_OK;
+}
+
+void Model::unpack(const double *raw, PassOut *out) const {
+  out->logDet = raw[0];
+  out->q = raw[1];
+  for (int k = 0; k < kStatsMax; k++) out->gram[k] = raw[k];
+}
+
+int Model::pass(const CovArgs &c, int slot, const double *beta, bool storeB,
+                PassOut *out) {
+  NNGP_TRY(bf(c, slot, storeB));
+  NNGP_TRY(stats(slot, beta, 0));
+  NNGP_HIP(hipMemcpyAsync(hsums.p, sums.p, sizeof(double) * kStatsMax,
+                          hipMemcpyDeviceToHost, st.s));
+  NNGP_HIP(hipMemcpyAsync(hsums.as<double>() + 2 * kStatsMax, err.p,
+                          sizeof(int), hipMemcpyDeviceToHost, st.s));
+  NNGP_HIP(hipStreamSynchronize(st.s));
+  NNGP_TRY(check_err("updateBF"));
+  if (out) unpack(hsums.as<double>(), out);
+  return NNGP_OK;
+}
+
+}  // namespace nngp

@@ -1,0 +1,195 @@
+// nngp_host.hpp — host-side helpers of libnngp_hip.so: error state, device
+// buffers, covariance parameters, small dense algebra and the built-in
+// random number generator.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/nngp.h"
+#include "nngp_kernels.hpp"
+
+namespace nngp {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+int set_err(int code, const char *fmt, ...);
+void clear_err();
+
+#define NNGP_HIP(expr)                                                      \
+  do {                                                                      \
+    hipError_t e_ = (expr);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      return ::nngp::set_err(NNGP_ERR_HIP, "HIP error '%s' (%s:%d)",        \
+                             hipGetErrorString(e_), __FILE__, __LINE__);    \
+  } while (0)
+
+#define NNGP_TRY(expr)            \
+  do {                            \
+    int st_ = (expr);             \
+    if (st_ != NNGP_OK) return st_; \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// device memory
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  int alloc(size_t nbytes) {
+    release();
+    if (nbytes == 0) nbytes = 8;
+    hipError_t e = hipMalloc(&p, nbytes);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return set_err(e == hipErrorOutOfMemory ? NNGP_ERR_NOMEM : NNGP_ERR_HIP,
+                     "device allocation of %zu bytes failed: %s", nbytes,
+                     hipGetErrorString(e));
+    }
+    bytes = nbytes;
+    return NNGP_OK;
+  }
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct PinnedBuf {
+  void *p = nullptr;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf &) = delete;
+  PinnedBuf &operator=(const PinnedBuf &) = delete;
+  ~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  int alloc(size_t nbytes) {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    NNGP_HIP(hipHostMalloc(&p, nbytes ? nbytes : 8, hipHostMallocDefault));
+    return NNGP_OK;
+  }
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct Stream {
+  hipStream_t s = nullptr;
+  Stream() = default;
+  Stream(const Stream &) = delete;
+  Stream &operator=(const Stream &) = delete;
+  ~Stream() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+  int create() {
+    NNGP_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return NNGP_OK;
+  }
+};
+
+// Select the device of opts (default 0); fails cleanly without a GPU.
+int use_device(const nngp_opts *opts);
+
+// ---------------------------------------------------------------------------
+// parameters
+// ---------------------------------------------------------------------------
+CovArgs make_cov(int cov, double s2, double t2, double phi, double nu);
+
+// ---------------------------------------------------------------------------
+// argument checks
+// ---------------------------------------------------------------------------
+long long n_indx(int n, int m);
+// nnIndxLU consistent with nIndx, counts <= m, indices in [0, n).
+int check_neighbors(int n, int m, const int *nnIndx, const int *nnIndxLU);
+
+// ---------------------------------------------------------------------------
+// small dense algebra (lower triangle, column-major, the reference's
+// dpotrf / dpotri / dsymv / dtrmv sequence of src/rNNGP.cpp:288-294)
+// ---------------------------------------------------------------------------
+int h_potrf_L(int n, double *A);
+int h_potri_L(int n, double *A);
+void h_symv_L(int n, const double *A, const double *x, double *y);
+void h_trmv_L(int n, const double *A, double *x);
+
+// ---------------------------------------------------------------------------
+// random numbers with R's argument semantics
+// ---------------------------------------------------------------------------
+struct BuiltinRng {
+  // xoshiro256** (Blackman & Vigna) seeded through splitmix64; normals by
+  // the polar Box-Muller method, one cached.
+  uint64_t s[4];
+  int have = 0;
+  double cached = 0.0;
+  explicit BuiltinRng(uint64_t seed);
+  uint64_t next();
+  double unif();
+  double norm();
+};
+
+struct Rng {
+  const nngp_rng *ext = nullptr;
+  BuiltinRng own;
+  explicit Rng(const nngp_rng *r, uint64_t seed) : ext(r), own(seed) {}
+  double norm_rand() { return ext ? ext->norm_rand(ext->state) : own.norm(); }
+  double unif_rand() { return ext ? ext->unif_rand(ext->state) : own.unif(); }
+  // R's rnorm(mu, sd): sd == 0 or non-finite mu returns mu without a draw
+  double rnorm(double mu, double sd) {
+    if (std::isnan(mu) || !std::isfinite(sd) || sd < 0.0) return NAN;
+    if (sd == 0.0 || !std::isfinite(mu)) return mu;
+    return mu + sd * norm_rand();
+  }
+  // R's runif(a, b)
+  double runif(double a, double b) {
+    if (!std::isfinite(a) || !std::isfinite(b) || b < a) return NAN;
+    if (a == b) return a;
+    double u;
+    do {
+      u = unif_rand();
+    } while (u <= 0.0 || u >= 1.0);
+    return a + (b - a) * u;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// device-resident model
+// ---------------------------------------------------------------------------
+struct PassOut {
+  double logDet = 0.0, q = 0.0;
+  double gram[kStatsMax];  // lower triangle of E^T F^-1 E, row-packed
+};
+
+struct Model {
+  int n = 0, m = 0, p = 0, nz = 0, zs = 0, cov = 0, precision = 0;
+  long long nIndx = 0;
+  Stream st;
+  DevBuf xy, nnIndx, lu, Z;
+  DevBuf B[2], F[2], E[2];
+  DevBuf partials, sums, err;
+  PinnedBuf hsums;  // 2 * kStatsMax + 1 ints' worth, host mirror
+
+  int create(int n, int m, int p, const double *coords, const int *nnIndx,
+             const int *nnIndxLU, const double *X, const double *y, int cov,
+             const nngp_opts *opts);
+  // B/F/E of theta into slot (B stored only when storeB)
+  int bf(const CovArgs &c, int slot, bool storeB);
+  // launch the stats pass over slot into sums[off .. off + kStatsMax)
+  int stats(int slot, const double *beta, int off);
+  // one full pass: bf + stats, synchronized, results in out
+  int pass(const CovArgs &c, int slot, const double *beta, bool storeB,
+           PassOut *out);
+  int check_err(const char *what);
+  void unpack(const double *raw, PassOut *out) const;
+};
+
+}  // namespace nngp

@@ -1,0 +1,144 @@
+// nngp_kernels.hpp — launchers of the NNGP device kernels (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "nngp_device.hpp"
+
+namespace nngp {
+
+constexpr int kMaxP = 8;    // covariates supported by the fused passes
+constexpr int kMaxNZ = kMaxP + 1;
+constexpr int kStatsMax = 2 + kMaxNZ * (kMaxNZ + 1) / 2;  // logdet, Q, gram
+
+// One neighbor-system pass over sites [0, n): B, F of one theta and,
+// when E != nullptr, the residual vectors e_j(i) = Z(i,j) - B_i . Z(N(i),j).
+struct BfArgs {
+  const double2 *xy;     // n interleaved coordinates
+  const int *nnIndx;     // flat neighbor index
+  const int *lu;         // nnIndxLU, 2n
+  int n;
+  int m;                 // n.neighbors (max count of any site)
+  double *B;             // nIndx, or nullptr: not stored
+  double *F;             // n
+  const double *Z;       // n x zs row-major: X row then y (nz = p + 1 used)
+  int nz;
+  int zs;
+  double *E;             // nz x n (E[j*n + i]) or nullptr
+  int *err;              // smallest site whose Cholesky failed (INT_MAX = ok)
+  CovArgs c;
+};
+
+// Sums over sites for one slot: sum log F, Q(r, r) with r = X beta - y
+// (via the residual vectors), and the lower triangle of E^T diag(1/F) E.
+struct StatsArgs {
+  const double *E;       // nz x n, or nullptr (then only sum log F)
+  const double *F;
+  int n;
+  int nz;                // p + 1
+  double beta[kMaxP];
+  int want_q;
+};
+
+// Plain Q(u, v) of src/util.cpp:261-280 over stored B and F.
+struct QArgs {
+  const double *B, *F, *u, *v;
+  const int *nnIndx, *lu;
+  int n;
+};
+
+int bf_launch(const BfArgs &a, hipStream_t st);
+// Writes kStatsMax block partials per block into partials and the reduced
+// sums into out (device, kStatsMax doubles).  partials must hold
+// stats_blocks(n) * kStatsMax doubles.
+int stats_blocks(int n);
+int stats_launch(const StatsArgs &a, double *partials, double *out,
+                 hipStream_t st);
+int q_blocks(int n);
+int q_launch(const QArgs &a, double *partials, double *out, hipStream_t st);
+
+// *p = v on the stream (no memset node, safe under graph capture).
+int fill_int_launch(int *p, int v, hipStream_t st);
+
+// Interleave column-major coords (x block then y block) into xy.
+int interleave_launch(const double *coords_cm, int n, double2 *xy,
+                      hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// neighbor search
+// ---------------------------------------------------------------------------
+// Strip index: points sorted by x, cut into strips of S consecutive points,
+// each strip sorted by y.  Arrays are strip-major (strip t at [t*S, ...)).
+struct StripIndex {
+  int n, S, nstrip;
+  const double *sx, *sy;  // coordinates
+  const int *sid;         // point id (NNGP index, or fitted row for nn2)
+  const double *xlo, *xhi;  // x range of each strip
+};
+
+// mkNNIndx(CB): m nearest predecessors of every site i (ids < i).
+struct KnnArgs {
+  StripIndex s;
+  const double2 *xy;     // NNGP-order coordinates
+  const int *strip_of;   // strip holding each NNGP index
+  int n, m;
+  int sorted;            // 1: NNGP order equals the x order
+  int *nnIndx;           // outputs in the reference layout
+  double *nnDist;
+  int *lu;
+};
+int knn_launch(const KnnArgs &a, hipStream_t st);
+
+// nn2: k nearest fitted points of each query point.
+struct Nn2Args {
+  StripIndex s;
+  const double2 *q;      // query coordinates
+  int nq, k;
+  int *idx;              // nq x k column-major (0-based)
+  double *dist;          // nq x k column-major or nullptr
+};
+int nn2_launch(const Nn2Args &a, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// kriging
+// ---------------------------------------------------------------------------
+struct PredArgs {
+  const double2 *xy;     // fitted coordinates (original order)
+  const double *X;       // n x p column-major
+  const double *y;       // n
+  int n, p, m;
+  const double2 *xy0;    // q query coordinates
+  const double *X0;      // q x p column-major
+  int q;
+  const int *nn0;        // q x m column-major
+  const double *beta;    // p x S
+  const double *theta;   // nTheta x S
+  int S, nTheta, cov;
+  const double *z;       // q*S normals, z[i*S + s]
+  double *y0;            // q x S: y0[s*q + i]
+  int *err;
+  int site0;             // first site of this launch (sites [site0, site0+nsite))
+  int nsite;
+  CovArgs unit;          // scale-free exp coefficients (closed-form path)
+  const CovArgs *cov_s;  // per-sample parameters, device, S entries
+                         // (generic path)
+};
+int pred_launch(const PredArgs &a, hipStream_t st);
+
+struct ConjPredArgs {
+  const double2 *xy;
+  const double *X, *y;
+  int n, p, m;
+  const double2 *xy0;
+  const double *X0;
+  int n0;
+  const int *nn0;        // n0 x m column-major
+  const double *beta;    // p
+  const double *Vb;      // p x p (lower) = (X^T V^-1 X)^-1
+  double a, b, alpha;
+  double *y0, *y0var;    // n0
+  int *err;
+  CovArgs c;
+};
+int conj_pred_launch(const ConjPredArgs &a, hipStream_t st);
+
+}  // namespace nngp

@@ -1,0 +1,333 @@
+"""numpy interface to libnngp_hip.so, named after spNNGP's native routines.
+
+Each function takes the arguments of the reference routine it replaces, in
+the reference's layouts (column-major FP64 matrices, int32 flat neighbor
+index), and returns what the `.Call` returns, as numpy arrays.  All compute
+runs in the HIP library; errors raise NngpError with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import NngpOpts, NngpPassOut, NngpRng, check, load
+
+COV_MODELS = ("exponential", "spherical", "matern", "gaussian")
+
+
+def cov_index(cov) -> int:
+    """covModel index of src/util.cpp:192-206 from a name or an int."""
+    if isinstance(cov, str):
+        return COV_MODELS.index(cov)
+    return int(cov)
+
+
+def n_indx(n: int, m: int) -> int:
+    """Length of the flat neighbor arrays (src/rNNGP.cpp:218)."""
+    return int((1 + m) / 2 * m + (n - m - 1) * m)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _cm(a):
+    """Column-major flat copy of a 2-D array (or a flat array unchanged)."""
+    a = np.asarray(a, dtype=np.float64)
+    if a.ndim == 2:
+        return _f64(a.T.reshape(-1))
+    return _f64(a)
+
+
+def _opts(precision: int = 0, device: int = 0, seed: int = 1):
+    return NngpOpts(precision, device, seed, _lib._INT_FN(), None,
+                    (C.c_int * 8)())
+
+
+def device_count() -> int:
+    return load().nngp_device_count()
+
+
+# ---------------------------------------------------------------------------
+# neighbor search
+# ---------------------------------------------------------------------------
+def _knn(fn, coords, m):
+    coords = _cm(coords)
+    n = coords.size // 2
+    ni = n_indx(n, m)
+    nnIndx = np.zeros(ni, np.int32)
+    nnDist = np.zeros(ni, np.float64)
+    nnIndxLU = np.zeros(2 * n, np.int32)
+    check(fn(n, m, coords, nnIndx, nnDist, nnIndxLU, 1))
+    return {"nnIndx": nnIndx, "nnDist": nnDist, "nnIndxLU": nnIndxLU}
+
+
+def mkNNIndxCB(coords, m: int):
+    """mkNNIndxCB (R/nn.R:53-76, src/nn.cpp:214-257).  coords: n x 2."""
+    return _knn(load().nngp_mkNNIndxCB, coords, m)
+
+
+def mkNNIndx(coords, m: int):
+    """mkNNIndx (R/nn.R:27-51, src/nn.cpp:63-111)."""
+    return _knn(load().nngp_mkNNIndx, coords, m)
+
+
+def nn2(coords, coords0, k: int):
+    """RANN::nn2(coords, coords0, k) with 0-based indices (q x k)."""
+    coords = np.asarray(coords, np.float64)
+    coords0 = np.asarray(coords0, np.float64)
+    n, q = coords.shape[0], coords0.shape[0]
+    idx = np.zeros(q * k, np.int32)
+    dist = np.zeros(q * k, np.float64)
+    check(load().nngp_nn2(n, _cm(coords), q, _cm(coords0), k, idx,
+                          dist.ctypes.data_as(C.c_void_p)))
+    return idx.reshape(k, q).T.copy(), dist.reshape(k, q).T.copy()
+
+
+# ---------------------------------------------------------------------------
+# kernels
+# ---------------------------------------------------------------------------
+def updateBF(coords, nnIndx, nnIndxLU, m: int, theta, covModel,
+             precision: int = 0):
+    """updateBF (src/rNNGP.cpp:46-101).  theta = (sigma.sq, tau.sq, phi[, nu]).
+    Returns (B, F, logDet)."""
+    coords = _cm(coords)
+    n = coords.size // 2
+    B = np.zeros(max(n_indx(n, m), 1))
+    F = np.zeros(n)
+    th = np.zeros(4)
+    th[:len(theta)] = theta
+    ld = C.c_double()
+    opts = _opts(precision)
+    check(load().nngp_updateBF(B, F, coords, _i32(nnIndx), _i32(nnIndxLU), n,
+                               m, th, cov_index(covModel), C.byref(ld),
+                               C.byref(opts)))
+    return B, F, ld.value
+
+
+def updateConjBF(coords, nnIndx, nnIndxLU, m: int, phi, alpha, nu, covModel):
+    """updateConjBF (src/cNNGP.cpp:24-69).  Returns (B, F)."""
+    coords = _cm(coords)
+    n = coords.size // 2
+    B = np.zeros(max(n_indx(n, m), 1))
+    F = np.zeros(n)
+    opts = _opts()
+    check(load().nngp_updateConjBF(B, F, coords, _i32(nnIndx), _i32(nnIndxLU),
+                                   n, m, float(phi), float(alpha), float(nu),
+                                   cov_index(covModel), C.byref(opts)))
+    return B, F
+
+
+def Q(B, F, u, v, nnIndx, nnIndxLU, m: int) -> float:
+    """Q (src/util.cpp:261-280)."""
+    F = _f64(F)
+    q = C.c_double()
+    check(load().nngp_Q(_f64(B), F, _f64(u), _f64(v), F.size, m, _i32(nnIndx),
+                        _i32(nnIndxLU), C.byref(q)))
+    return q.value
+
+
+# ---------------------------------------------------------------------------
+# drivers
+# ---------------------------------------------------------------------------
+def _rng_struct(rng):
+    if rng is None:
+        return None
+    return C.byref(rng.struct(NngpRng))
+
+
+def rNNGP(y, X, p, n, m, coords, covModel, nnIndx, nnIndxLU, sigmaSqIG,
+          tauSqIG, phiUnif, nuUnif, betaStarting, sigmaSqStarting,
+          tauSqStarting, phiStarting, nuStarting, sigmaSqTuning, tauSqTuning,
+          phiTuning, nuTuning, nSamples, nThreads=1, verbose=0, nReport=100,
+          nRep=0, repIndx=None, rng=None, precision=0, seed=1):
+    """rNNGP (src/rNNGP.cpp:105-420), arguments in the .Call order of
+    R/spNNGP.R:401-405.  rng: object with .struct(cls) giving the
+    {state, norm_rand, unif_rand} callbacks, or None for the built-in
+    generator seeded by `seed`."""
+    cm = cov_index(covModel)
+    nTheta = 4 if cm == 2 else 3
+    beta = np.zeros(p * nSamples)
+    theta = np.zeros(nTheta * nSamples)
+    acc = C.c_int()
+    rep = np.zeros(max(n * nRep, 1))
+    ri = _i32(repIndx if repIndx is not None else np.zeros(max(nSamples, 1)))
+    opts = _opts(precision, 0, seed)
+    nu_u = _f64(nuUnif if nuUnif is not None else [0.0, 0.0])
+    check(load().nngp_rNNGP(
+        _f64(y), _cm(X), p, n, m, _cm(coords), cm, _i32(nnIndx),
+        _i32(nnIndxLU), _f64(sigmaSqIG), _f64(tauSqIG), _f64(phiUnif),
+        nu_u.ctypes.data_as(C.c_void_p), _f64(betaStarting),
+        sigmaSqStarting, tauSqStarting, phiStarting, nuStarting,
+        sigmaSqTuning, tauSqTuning, phiTuning, nuTuning, nSamples, nThreads,
+        verbose, nReport, nRep, ri.ctypes.data_as(C.c_void_p),
+        _rng_struct(rng), C.byref(opts), beta, theta,
+        rep.ctypes.data_as(C.c_void_p), C.byref(acc)))
+    out = {"p.beta.samples": beta.reshape(nSamples, p).T.copy(),
+           "p.theta.samples": theta.reshape(nSamples, nTheta).T.copy(),
+           "accept": acc.value}
+    if nRep:
+        out["y.rep.samples"] = rep.reshape(nRep, n).T.copy()
+    return out
+
+
+def cNNGP(y, X, p, n, coords, thetaAlpha, nnIndx, nnIndxLU, X0, coords0, n0,
+          nnIndx0, g, m, sigmaSqIG, covModel, nThreads=1, verbose=0,
+          precision=0):
+    """cNNGP (src/cNNGP.cpp:73-358), .Call order of R/spConjNNGP.R:336.
+    thetaAlpha: nTheta x g (rows phi, alpha[, nu]); nnIndx0: n0 x m."""
+    cm = cov_index(covModel)
+    betaHat = np.zeros(p * g)
+    ab = np.zeros(2 * g)
+    bBInv = np.zeros(p * p * g)
+    bb = np.zeros(p * g)
+    y0 = np.zeros(max(n0 * g, 1))
+    y0v = np.zeros(max(n0 * g, 1))
+    opts = _opts(precision)
+    keep = []
+
+    def vp(a, conv):
+        if a is None or n0 == 0:
+            return None
+        arr = conv(a)
+        keep.append(arr)
+        return arr.ctypes.data_as(C.c_void_p)
+
+    check(load().nngp_cNNGP(
+        _f64(y), _cm(X), p, n, _cm(coords), _cm(thetaAlpha), _i32(nnIndx),
+        _i32(nnIndxLU), vp(X0, _cm), vp(coords0, _cm), n0,
+        vp(nnIndx0, lambda a: _i32(np.asarray(a).T.reshape(-1))
+           if np.asarray(a).ndim == 2 else _i32(a)),
+        g, m, _f64(sigmaSqIG), cm, nThreads, verbose, C.byref(opts), betaHat,
+        ab, bBInv, bb, y0.ctypes.data_as(C.c_void_p),
+        y0v.ctypes.data_as(C.c_void_p)))
+    return {"beta.hat": betaHat.reshape(g, p).T.copy(),
+            "ab": ab.reshape(g, 2).T.copy(),
+            "bB.inv": bBInv.reshape(g, p * p).T.copy(),
+            "bb": bb.reshape(g, p).T.copy(),
+            "y.0.hat": y0[:n0 * g].reshape(g, n0).T.copy(),
+            "y.0.hat.var": y0v[:n0 * g].reshape(g, n0).T.copy()}
+
+
+def rNNGPPredict(X, y, coords, n, p, m, X0, coords0, q, nnIndx0,
+                 betaSamples, thetaSamples, nSamples, covModel, nThreads=1,
+                 verbose=0, nReport=100, z=None, rng=None, seed=1,
+                 precision=0):
+    """rNNGPPredict (src/rNNGPPredict.cpp:25-222), .Call order of
+    R/generics.R:674.  betaSamples p x S, thetaSamples nTheta x S, nnIndx0
+    q x m (0-based).  z: q*S normals with z[i*S + s], or None."""
+    out = np.zeros(q * nSamples)
+    opts = _opts(precision, 0, seed)
+    nn0 = np.asarray(nnIndx0)
+    nn0 = _i32(nn0.T.reshape(-1)) if nn0.ndim == 2 else _i32(nn0)
+    zz = None if z is None else _f64(z)
+    check(load().nngp_rNNGPPredict(
+        _cm(X), _f64(y), _cm(coords), n, p, m, _cm(X0), _cm(coords0), q, nn0,
+        _cm(betaSamples), _cm(thetaSamples), nSamples, cov_index(covModel),
+        nThreads, verbose, nReport,
+        None if zz is None else zz.ctypes.data_as(C.c_void_p),
+        _rng_struct(rng), C.byref(opts), out))
+    return {"p.y.0": out.reshape(nSamples, q).T.copy()}
+
+
+# ---------------------------------------------------------------------------
+# device-resident model
+# ---------------------------------------------------------------------------
+class Model:
+    """Coordinates, neighbor index, X and y kept in HBM (nngp_model_*)."""
+
+    def __init__(self, coords, nnIndx, nnIndxLU, m, covModel, X=None, y=None,
+                 precision=0):
+        coords = _cm(coords)
+        self.n = coords.size // 2
+        self.m = m
+        self.cov = cov_index(covModel)
+        self.p = 0 if X is None else np.asarray(X).shape[1]
+        self._h = C.c_void_p()
+        self._opts = _opts(precision)
+        Xc = None if X is None else _cm(X)
+        yc = None if y is None else _f64(y)
+        self._keep = (coords, Xc, yc)
+        check(load().nngp_model_create(
+            C.byref(self._h), self.n, m, self.p, coords, _i32(nnIndx),
+            _i32(nnIndxLU),
+            None if Xc is None else Xc.ctypes.data_as(C.c_void_p),
+            None if yc is None else yc.ctypes.data_as(C.c_void_p), self.cov,
+            C.byref(self._opts)))
+
+    def close(self):
+        if self._h:
+            load().nngp_model_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def update(self, theta, slot=0, beta=None, want_xprod=True):
+        th = np.zeros(4)
+        th[:len(theta)] = theta
+        out = NngpPassOut()
+        b = None if beta is None else _f64(beta)
+        check(load().nngp_model_update(
+            self._h, th, slot,
+            None if b is None else b.ctypes.data_as(C.c_void_p),
+            int(want_xprod), C.byref(out)))
+        p = self.p
+        res = {"logDet": out.logDet, "qResid": out.qResid}
+        if want_xprod and p:
+            X = np.array(out.XtViX[:p * p]).reshape(p, p).T
+            res["XtViX"] = np.tril(X) + np.tril(X, -1).T
+            res["XtViy"] = np.array(out.XtViy[:p])
+            res["ytViy"] = out.ytViy
+        return res
+
+    def qresid(self, beta, slot=0):
+        q = C.c_double()
+        check(load().nngp_model_qresid(self._h, slot, _f64(beta), C.byref(q)))
+        return q.value
+
+    def get_bf(self, slot=0):
+        B = np.zeros(max(n_indx(self.n, self.m), 1))
+        F = np.zeros(self.n)
+        check(load().nngp_model_get_bf(self._h, slot,
+                                       B.ctypes.data_as(C.c_void_p),
+                                       F.ctypes.data_as(C.c_void_p)))
+        return B, F
+
+    def rNNGP(self, sigmaSqIG, tauSqIG, phiUnif, nuUnif, betaStarting,
+              sigmaSqStarting, tauSqStarting, phiStarting, nuStarting,
+              sigmaSqTuning, tauSqTuning, phiTuning, nuTuning, nSamples,
+              verbose=0, nReport=100, rng=None, seed=1):
+        p = self.p
+        nTheta = 4 if self.cov == 2 else 3
+        beta = np.zeros(p * nSamples)
+        theta = np.zeros(nTheta * nSamples)
+        acc = C.c_int()
+        opts = _opts(0, 0, seed)
+        nu_u = _f64(nuUnif if nuUnif is not None else [0.0, 0.0])
+        check(load().nngp_model_rNNGP(
+            self._h, _f64(sigmaSqIG), _f64(tauSqIG), _f64(phiUnif),
+            nu_u.ctypes.data_as(C.c_void_p), _f64(betaStarting),
+            sigmaSqStarting, tauSqStarting, phiStarting, nuStarting,
+            sigmaSqTuning, tauSqTuning, phiTuning, nuTuning, nSamples,
+            verbose, nReport, _rng_struct(rng), C.byref(opts), beta, theta,
+            C.byref(acc)))
+        return {"p.beta.samples": beta.reshape(nSamples, p).T.copy(),
+                "p.theta.samples": theta.reshape(nSamples, nTheta).T.copy(),
+                "accept": acc.value}

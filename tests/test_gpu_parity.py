@@ -1,0 +1,336 @@
+"""GPU parity: libnngp_hip.so (through its C ABI) against the CPU oracle on
+the same seeded inputs.
+
+Bars: neighbor indices and distances bit-exact; B, F, log-det, Q, kriging
+and conjugate outputs within 1e-10 relative (FP64 path); MCMC chains with
+the same host random numbers accept and reject identically, samples within
+1e-9 relative.
+"""
+import numpy as np
+import pytest
+
+from conftest import uniform_coords
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+def rel_norm_err(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def per_site_B_err(B1, B0, lu, n):
+    worst = 0.0
+    for i in range(1, n):
+        off, cnt = lu[i], lu[n + i]
+        b0 = B0[off:off + cnt]
+        d = np.max(np.abs(B1[off:off + cnt] - b0)) / max(np.max(np.abs(b0)),
+                                                         1e-300)
+        worst = max(worst, d)
+    return worst
+
+
+# ---------------------------------------------------------------------------
+# neighbor search
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n,m,seed", [(2000, 10, 1), (5000, 15, 2),
+                                      (3000, 32, 3), (1500, 1, 4),
+                                      (700, 64, 5)])
+def test_knn_bit_exact(oracle, nngp, n, m, seed):
+    print(f"knn n={n} m={m} seed={seed}", flush=True)
+    c = uniform_coords(n, seed)
+    ref = oracle.mkNNIndx(c, m)
+    got = nngp.mkNNIndxCB(c, m)
+    assert np.array_equal(got["nnIndxLU"], ref["nnIndxLU"])
+    assert np.array_equal(got["nnIndx"], ref["nnIndx"])
+    assert np.array_equal(got["nnDist"], ref["nnDist"])  # bit for bit
+
+
+def test_knn_custom_order_bit_exact(oracle, nngp):
+    rng = np.random.default_rng(8)
+    c = rng.uniform(size=(2500, 2))  # not x-ordered
+    ref = oracle.mkNNIndx(c, 12)
+    got = nngp.mkNNIndx(c, 12)
+    assert np.array_equal(got["nnIndx"], ref["nnIndx"])
+    assert np.array_equal(got["nnDist"], ref["nnDist"])
+
+
+def test_knn_clustered_anisotropic(oracle, nngp):
+    rng = np.random.default_rng(12)
+    cen = rng.uniform(size=(10, 2))
+    pts = cen[rng.integers(0, 10, 3000)] + rng.normal(scale=0.02,
+                                                      size=(3000, 2))
+    pts[:, 0] *= 3.0
+    pts = pts[np.argsort(pts[:, 0], kind="stable")]
+    ref = oracle.mkNNIndx(pts, 20)
+    got = nngp.mkNNIndxCB(pts, 20)
+    assert np.array_equal(got["nnIndx"], ref["nnIndx"])
+    assert np.array_equal(got["nnDist"], ref["nnDist"])
+
+
+@pytest.mark.parametrize("k", [1, 10, 15, 40])
+def test_nn2_bit_exact(oracle, nngp, k):
+    rng = np.random.default_rng(20 + k)
+    c = rng.uniform(size=(4000, 2))
+    c0 = rng.uniform(size=(900, 2))
+    i_ref, d_ref = oracle.nn2(c, c0, k)
+    i_got, d_got = nngp.nn2(c, c0, k)
+    assert np.array_equal(i_got, i_ref)
+    assert np.array_equal(d_got, d_ref)
+
+
+# ---------------------------------------------------------------------------
+# updateBF / updateConjBF / Q
+# ---------------------------------------------------------------------------
+CASES = [
+    # (n, m, cov, theta) -- per-lane kernel for cov != 2 and m <= 16,
+    # wavefront kernel for matern or m > 16
+    (2000, 10, 0, [5.0, 1.0, 6.0, 0.0]),
+    (3000, 15, 0, [2.0, 0.2, 12.0, 0.0]),
+    (1500, 12, 1, [1.0, 0.1, 3.0, 0.0]),
+    (1500, 8, 3, [1.5, 0.5, 4.0, 0.0]),
+    (1200, 15, 2, [1.0, 0.1, 8.0, 1.5]),
+    (800, 10, 2, [1.0, 0.05, 20.0, 2.5]),
+    (900, 32, 0, [1.0, 0.05, 10.0, 0.0]),
+    (600, 32, 2, [1.0, 0.05, 20.0, 0.7]),
+    (400, 3, 0, [1.0, 0.5, 2.0, 0.0]),
+]
+
+
+@pytest.mark.parametrize("n,m,cov,theta", CASES)
+def test_updateBF_matches_oracle(oracle, nngp, n, m, cov, theta):
+    print(f"updateBF n={n} m={m} cov={cov} theta={theta}", flush=True)
+    c = uniform_coords(n, 100 + n + m)
+    nn = oracle.mkNNIndx(c, m)
+    flat = c.T.reshape(-1).copy()
+    B0, F0, ld0 = oracle.updateBF(flat, nn["nnIndx"], nn["nnIndxLU"], n, m,
+                                  theta, cov)
+    B1, F1, ld1 = nngp.updateBF(c, nn["nnIndx"], nn["nnIndxLU"], m, theta,
+                                cov)
+    assert per_site_B_err(B1, B0, nn["nnIndxLU"], n) < RTOL
+    assert rel_err(F1, F0) < RTOL
+    assert abs(ld1 - ld0) / abs(ld0) < RTOL
+
+
+@pytest.mark.parametrize("cov,nu", [(0, 0.0), (2, 1.5), (3, 0.0)])
+def test_updateConjBF_matches_oracle(oracle, nngp, cov, nu):
+    n, m = 1500, 15
+    c = uniform_coords(n, 77)
+    nn = oracle.mkNNIndx(c, m)
+    B0, F0 = oracle.updateConjBF(c.T.reshape(-1).copy(), nn["nnIndx"],
+                                 nn["nnIndxLU"], n, m, 8.0, 0.1, nu, cov)
+    B1, F1 = nngp.updateConjBF(c, nn["nnIndx"], nn["nnIndxLU"], m, 8.0, 0.1,
+                               nu, cov)
+    assert per_site_B_err(B1, B0, nn["nnIndxLU"], n) < RTOL
+    assert rel_err(F1, F0) < RTOL
+
+
+def test_Q_matches_oracle(oracle, nngp):
+    n, m = 4000, 15
+    c = uniform_coords(n, 5)
+    nn = oracle.mkNNIndx(c, m)
+    B, F, _ = oracle.updateBF(c.T.reshape(-1).copy(), nn["nnIndx"],
+                              nn["nnIndxLU"], n, m, [2.0, 0.2, 12.0], 0)
+    rng = np.random.default_rng(0)
+    u, v = rng.normal(size=n), rng.normal(size=n)
+    q0 = oracle.Q(B, F, u, v, nn["nnIndx"], nn["nnIndxLU"])
+    q1 = nngp.Q(B, F, u, v, nn["nnIndx"], nn["nnIndxLU"], m)
+    assert abs(q1 - q0) / abs(q0) < RTOL
+
+
+def test_model_pass_cross_products_match_oracle_Q(oracle, nngp):
+    """The fused pass's X^T V^-1 X, X^T V^-1 y, y^T V^-1 y and Q(r, r)
+    against the oracle's Q on the same B, F (src/rNNGP.cpp:281-286)."""
+    n, m, p = 3000, 10, 3
+    c = uniform_coords(n, 31)
+    rng = np.random.default_rng(31)
+    X = np.column_stack([np.ones(n), rng.normal(size=(n, p - 1))])
+    y = X @ [1.0, 2.0, -1.0] + rng.normal(size=n)
+    nn = oracle.mkNNIndx(c, m)
+    theta = [2.0, 0.3, 9.0]
+    B, F, ld = oracle.updateBF(c.T.reshape(-1).copy(), nn["nnIndx"],
+                               nn["nnIndxLU"], n, m, theta, 0)
+    beta = np.array([0.9, 2.1, -0.8])
+    with nngp.Model(c, nn["nnIndx"], nn["nnIndxLU"], m, 0, X, y) as md:
+        out = md.update(theta, slot=1, beta=beta)
+        q_again = md.qresid(beta, slot=1)
+        B1, F1 = md.get_bf(1)
+    L, lu = nn["nnIndx"], nn["nnIndxLU"]
+    Q = lambda u, v: oracle.Q(B, F, u, v, L, lu)
+    r = X @ beta - y
+    assert abs(out["logDet"] - ld) / abs(ld) < RTOL
+    assert abs(out["qResid"] - Q(r, r)) / Q(r, r) < RTOL
+    assert q_again == out["qResid"]
+    for a in range(p):
+        assert abs(out["XtViy"][a] - Q(X[:, a], y)) <= RTOL * abs(Q(y, y))
+        for b in range(p):
+            ref = Q(X[:, a], X[:, b])
+            assert abs(out["XtViX"][a, b] - ref) <= RTOL * Q(X[:, a], X[:, a])
+    assert abs(out["ytViy"] - Q(y, y)) / Q(y, y) < RTOL
+    assert per_site_B_err(B1, B, lu, n) < RTOL
+
+
+# ---------------------------------------------------------------------------
+# rNNGP chain identity
+# ---------------------------------------------------------------------------
+def make_response_data(n, p, seed, cov=0):
+    rng = np.random.default_rng(seed)
+    c = uniform_coords(n, seed)
+    X = np.column_stack([np.ones(n), rng.normal(size=(n, p - 1))])
+    D = np.sqrt(((c[:, None] - c[None]) ** 2).sum(-1))
+    C = 5.0 * np.exp(-6.0 * D)
+    w = np.linalg.cholesky(C + 1e-10 * np.eye(n)) @ rng.normal(size=n)
+    beta = np.array([1.0, 5.0] + [0.5] * (p - 2))[:p]
+    y = X @ beta + w + rng.normal(size=n)
+    return c, X, y
+
+
+@pytest.mark.parametrize("cov,nu", [(0, 0.0), (2, 1.0)])
+def test_rNNGP_chain_identical_to_oracle(oracle, nngp, cov, nu):
+    n, m, p, S = 400, 10, 2, 200
+    print(f"rNNGP chain cov={cov}", flush=True)
+    c, X, y = make_response_data(n, p, 42)
+    nn = oracle.mkNNIndx(c, m)
+    rng = np.random.default_rng(99)
+    normals = rng.normal(size=S * 10 + 100)
+    unifs = rng.uniform(size=S + 10)
+    beta0 = np.linalg.lstsq(X, y, rcond=None)[0]
+    args = dict(sigmaSqIG=[2.0, 5.0], tauSqIG=[2.0, 1.0], phiUnif=[3.0, 30.0],
+                nuUnif=[0.5, 2.5], betaStarting=beta0, sigmaSqStarting=5.0,
+                tauSqStarting=1.0, phiStarting=6.0, nuStarting=nu,
+                sigmaSqTuning=0.3, tauSqTuning=0.3, phiTuning=0.3,
+                nuTuning=0.2, nSamples=S)
+    flat = c.T.reshape(-1).copy()
+    ref = oracle.rNNGP(y, X, p, n, m, flat, cov, nn["nnIndx"],
+                       nn["nnIndxLU"], rng=oracle.TapeRng(normals, unifs),
+                       **args)
+    tape = oracle.TapeRng(normals, unifs)
+    got = nngp.rNNGP(y, X, p, n, m, c, cov, nn["nnIndx"], nn["nnIndxLU"],
+                     rng=tape, **args)
+    # same number of draws consumed, same accept count, same chain
+    assert got["accept"] == ref["accept"]
+    assert 0 < ref["accept"] < S
+    th0, th1 = ref["p.theta.samples"], got["p.theta.samples"]
+    acc0 = np.r_[True, np.any(th0[:, 1:] != th0[:, :-1], axis=0)]
+    acc1 = np.r_[True, np.any(th1[:, 1:] != th1[:, :-1], axis=0)]
+    assert np.array_equal(acc0, acc1)
+    assert rel_err(th1, th0) < 1e-9
+    assert rel_norm_err(got["p.beta.samples"], ref["p.beta.samples"]) < 1e-9
+
+
+def test_rNNGP_fit_rep_matches_oracle(oracle, nngp):
+    n, m, p, S = 200, 8, 2, 30
+    c, X, y = make_response_data(n, p, 7)
+    nn = oracle.mkNNIndx(c, m)
+    rng = np.random.default_rng(3)
+    normals = rng.normal(size=S * (n + 10) + 10)
+    unifs = rng.uniform(size=S + 5)
+    rep = np.zeros(S, np.int32)
+    rep[10::5] = 1
+    nrep = int(rep.sum())
+    args = dict(sigmaSqIG=[2.0, 5.0], tauSqIG=[2.0, 1.0], phiUnif=[3.0, 30.0],
+                nuUnif=None, betaStarting=np.zeros(p), sigmaSqStarting=5.0,
+                tauSqStarting=1.0, phiStarting=6.0, nuStarting=0.0,
+                sigmaSqTuning=0.3, tauSqTuning=0.3, phiTuning=0.3,
+                nuTuning=0.0, nSamples=S, nRep=nrep, repIndx=rep)
+    ref = oracle.rNNGP(y, X, p, n, m, c.T.reshape(-1).copy(), 0,
+                       nn["nnIndx"], nn["nnIndxLU"],
+                       rng=oracle.TapeRng(normals, unifs), **args)
+    got = nngp.rNNGP(y, X, p, n, m, c, 0, nn["nnIndx"], nn["nnIndxLU"],
+                     rng=oracle.TapeRng(normals, unifs), **args)
+    assert got["accept"] == ref["accept"]
+    assert rel_norm_err(got["y.rep.samples"], ref["y.rep.samples"]) < 1e-9
+
+
+# ---------------------------------------------------------------------------
+# kriging and the conjugate model
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("cov,m", [(0, 15), (0, 10), (3, 8), (1, 12),
+                                   (2, 15), (0, 20)])
+def test_rNNGPPredict_matches_oracle(oracle, nngp, cov, m):
+    n, p, q, S = 3000, 3, 300, 70
+    print(f"predict cov={cov} m={m}", flush=True)
+    rng = np.random.default_rng(m + cov)
+    c = rng.uniform(size=(n, 2))
+    X = np.column_stack([np.ones(n), rng.normal(size=(n, p - 1))])
+    y = X @ [1.0, 2.0, -1.0] + rng.normal(size=n)
+    c0 = rng.uniform(size=(q, 2))
+    X0 = np.column_stack([np.ones(q), rng.normal(size=(q, p - 1))])
+    nn0, _ = oracle.nn2(c, c0, m)
+    nTheta = 4 if cov == 2 else 3
+    beta = rng.normal([1.0, 2.0, -1.0], 0.05, size=(S, p)).T.copy()
+    th = np.vstack([np.exp(rng.normal(np.log(2.0), 0.1, S)),
+                    np.exp(rng.normal(np.log(0.2), 0.1, S)),
+                    np.exp(rng.normal(np.log(10.0), 0.1, S))]
+                   + ([rng.uniform(0.6, 2.2, S)] if cov == 2 else []))
+    z = rng.normal(size=q * S)
+    ref = oracle.rNNGPPredict(X.T.reshape(-1).copy(), y,
+                              c.T.reshape(-1).copy(), n, p, m,
+                              X0.T.reshape(-1).copy(),
+                              c0.T.reshape(-1).copy(), q,
+                              nn0.T.reshape(-1).copy(),
+                              beta.T.reshape(-1).copy(),
+                              th.T.reshape(-1).copy(), S, cov, z)
+    got = nngp.rNNGPPredict(X, y, c, n, p, m, X0, c0, q, nn0, beta, th, S,
+                            cov, z=z)["p.y.0"]
+    assert got.shape == (q, S)
+    assert rel_norm_err(got, ref) < RTOL
+
+
+@pytest.mark.parametrize("cov,nu", [(0, 0.0), (2, 1.5)])
+def test_cNNGP_matches_oracle(oracle, nngp, cov, nu):
+    n, m, p, n0, g = 2000, 15, 3, 400, 4
+    rng = np.random.default_rng(17)
+    c = uniform_coords(n, 17)
+    X = rng.normal(size=(n, p))
+    y = X @ [1.0, 2.0, -1.0] + rng.normal(size=n)
+    c0 = rng.uniform(size=(n0, 2))
+    X0 = rng.normal(size=(n0, p))
+    nn = oracle.mkNNIndx(c, m)
+    nn0, _ = oracle.nn2(c, c0, m)
+    grid = [(3.0, 0.05), (8.0, 0.1), (15.0, 0.5), (30.0, 1.0)]
+    ta = np.array([[ph, al] + ([nu] if cov == 2 else []) for ph, al in grid]).T
+    ref = oracle.cNNGP(y, X.T.reshape(-1).copy(), p, n,
+                       c.T.reshape(-1).copy(), ta.T.reshape(-1).copy(),
+                       nn["nnIndx"], nn["nnIndxLU"], X0.T.reshape(-1).copy(),
+                       c0.T.reshape(-1).copy(), n0,
+                       nn0.T.reshape(-1).copy(), g, m, [2.0, 1.0], cov)
+    got = nngp.cNNGP(y, X, p, n, c, ta, nn["nnIndx"], nn["nnIndxLU"], X0, c0,
+                     n0, nn0, g, m, [2.0, 1.0], cov)
+    for key in ("beta.hat", "ab", "bb", "y.0.hat", "y.0.hat.var"):
+        assert rel_norm_err(got[key], ref[key]) < RTOL, key
+    lower = np.tril_indices(p)
+    for k in range(g):
+        a = got["bB.inv"][:, k].reshape(p, p).T
+        b = ref["bB.inv"][:, k].reshape(p, p).T
+        assert rel_norm_err(a[lower], b[lower]) < RTOL
+
+
+# ---------------------------------------------------------------------------
+# error paths (forced from the host; every kernel still exits normally)
+# ---------------------------------------------------------------------------
+def test_not_positive_definite_is_reported(nngp, oracle):
+    # duplicated coordinates with tau.sq = 0 make a singular neighbor system
+    c = uniform_coords(300, 3)
+    c[150] = c[149]
+    nn = oracle.mkNNIndx(c, 5)
+    with pytest.raises(nngp.NngpError) as e:
+        nngp.updateBF(c, nn["nnIndx"], nn["nnIndxLU"], 5, [1.0, 0.0, 2.0], 0)
+    assert e.value.status == nngp.NNGP_ERR_NOT_PD
+    assert "dpotrf failed" in str(e.value)
+
+
+def test_bad_cov_model_is_reported(nngp, oracle):
+    c = uniform_coords(100, 3)
+    nn = oracle.mkNNIndx(c, 5)
+    with pytest.raises(nngp.NngpError) as e:
+        nngp.updateBF(c, nn["nnIndx"], nn["nnIndxLU"], 5, [1.0, 0.1, 2.0], 9)
+    assert e.value.status == nngp.NNGP_ERR_COV_MODEL
+    assert "cov.model is not correctly specified" in str(e.value)

@@ -1,0 +1,55 @@
+"""CPU checks of the C-ABI library: it loads, exports every symbol that
+include/nngp.h declares, and rejects bad arguments before touching a GPU."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "nngp.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(nngp_[A-Za-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_declares_what_the_binding_lists(nngp):
+    from spnngp_amd import _lib
+    assert sorted(_lib.EXPORTS) == declared_symbols()
+
+
+def test_library_exports_every_declared_symbol(nngp):
+    assert sorted(nngp.exported_symbols()) == declared_symbols()
+
+
+def test_version_and_device_count(nngp):
+    assert nngp.load().nngp_version() == 10000
+    assert nngp.device_count() >= 0
+
+
+def test_bad_arguments_rejected_without_gpu(nngp):
+    c = np.random.default_rng(0).uniform(size=(50, 2))
+    nnIndx = np.zeros(nngp.n_indx(50, 5), np.int32)
+    lu = np.zeros(100, np.int32)
+    # malformed nnIndxLU (count above m) is an argument error
+    lu[50 + 3] = 7
+    with pytest.raises(nngp.NngpError) as e:
+        nngp.updateBF(c, nnIndx, lu, 5, [1.0, 0.1, 1.0], 0)
+    assert e.value.status == nngp.NNGP_ERR_ARG
+    with pytest.raises(nngp.NngpError) as e:
+        nngp.updateBF(c, nnIndx, lu, 5, [1.0, 0.1, 1.0], 5)
+    assert e.value.status == nngp.NNGP_ERR_COV_MODEL
+    with pytest.raises(nngp.NngpError) as e:
+        nngp.mkNNIndxCB(c, 50)  # m must be below n
+    assert e.value.status == nngp.NNGP_ERR_ARG
+
+
+def test_no_device_is_a_clean_error(nngp):
+    if nngp.device_count() > 0:
+        pytest.skip("a device is visible")
+    c = np.random.default_rng(0).uniform(size=(50, 2))
+    with pytest.raises(nngp.NngpError) as e:
+        nngp.mkNNIndxCB(c, 5)
+    assert e.value.status == nngp.NNGP_ERR_HIP
