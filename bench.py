@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Headline benchmark: response-NNGP MCMC iterations per second on one GPU.
+
+Workload (the at-scale response configuration): n = 1e6 sites ~ U[0,1]^2
+ordered by x, X = [1, N(0,1), N(0,1)], beta = (1, 2, -1), w drawn from the
+NNGP prior (exponential, sigma.sq = 2, phi = 12, no nugget, m = 15,
+w_i = B_i w_N(i) + sqrt(F_i) z_i), tau.sq = 0.2.  A step is one
+Metropolis-within-Gibbs iteration of rNNGP (beta draw, Q(r, r) at the
+current theta, updateBF of the proposal with its log-det and quadratic
+forms, accept/reject).  Data, neighbor index and B/F buffers are resident
+in HBM before the timed region starts; the host keeps the sampler.
+
+Prints one JSON line.  --gpus > 1 is refused: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MCMC iter/s @ n=1e6,m=15 (FP64 vs mixed) + kriging p50/p99 latency"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def f_alg(m):
+    """Algorithmic flops of one site's updateBF with m neighbors: Cholesky
+    m^3/3, two triangular solves 2m^2, 7 flops per pair for the distance and
+    kernel, 2m + 2 for F, and one per exp and per sqrt (m(m+1)/2 each)."""
+    pairs = m * (m + 1) / 2
+    return m ** 3 / 3 + 2 * m * m + 7 * pairs + 2 * m + 2 + 2 * pairs
+
+
+def make_data(n, m, seed):
+    import spnngp_amd as nn
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(size=(n, 2))
+    c = c[np.argsort(c[:, 0], kind="stable")]
+    X = np.column_stack([np.ones(n), rng.normal(size=(n, 2))])
+    beta = np.array([1.0, 2.0, -1.0])
+    t0 = time.perf_counter()
+    idx = nn.mkNNIndxCB(c, m)
+    knn_s = time.perf_counter() - t0
+    # w from the NNGP prior itself, exponential, sigma.sq = 2, phi = 12
+    B, F, _ = nn.updateBF(c, idx["nnIndx"], idx["nnIndxLU"], m,
+                          [2.0, 0.0, 12.0], "exponential")
+    w = nn.prior_draw(B, F, idx["nnIndx"], idx["nnIndxLU"],
+                      rng.normal(size=n))
+    y = X @ beta + w + rng.normal(scale=np.sqrt(0.2), size=n)
+    return c, X, y, idx, knn_s
+
+
+def cores():
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(avail, int(env))) if env else avail
+
+
+def cpu_baseline(c, X, y, idx, m, iters, start):
+    """The oracle's loop-for-loop restatement of src/rNNGP.cpp (two updateBF
+    per iteration, OpenMP over sites) on this host, for a bounded sample."""
+    from oracle import oracle as O
+    n, p = X.shape
+    nt = cores()
+    rng = np.random.default_rng(7)
+    tape = O.TapeRng(rng.normal(size=iters * 8 + 16),
+                     rng.uniform(size=iters + 4))
+    t0 = time.perf_counter()
+    O.rNNGP(y, X, p, n, m, c.T.reshape(-1).copy(), 0, idx["nnIndx"],
+            idx["nnIndxLU"], rng=tape, nThreads=nt, nSamples=iters, **start)
+    dt = time.perf_counter() - t0
+    return {"value": iters / dt, "unit": "iter/s", "cores": nt,
+            "kind": "port",
+            "sample": f"{iters} MH iterations of the same n={n}, m={m} "
+                      f"chain (oracle rNNGP, OpenMP, {nt} threads), "
+                      f"{dt:.1f} s"}
+
+
+def kriging_latency(c, X, y, m, nreq, q, S, seed):
+    """Latency of predict requests of q new sites against the fitted data
+    (nn2 + rNNGPPredict with host buffers), S posterior samples."""
+    import spnngp_amd as nn
+    rng = np.random.default_rng(seed)
+    n, p = X.shape
+    beta = rng.normal([1.0, 2.0, -1.0], 0.01, size=(S, p)).T.copy()
+    th = np.vstack([np.exp(np.log(2.0) + rng.normal(0, 0.02, S)),
+                    np.exp(np.log(0.2) + rng.normal(0, 0.02, S)),
+                    3 + 297 / (1 + np.exp(-(np.log((12 - 3) / (300 - 12))
+                                            + rng.normal(0, 0.02, S))))])
+    z = rng.normal(size=q * S)
+    lat = []
+    for r in range(nreq + 1):
+        c0 = rng.uniform(size=(q, 2))
+        X0 = np.column_stack([np.ones(q), rng.normal(size=(q, 2))])
+        t0 = time.perf_counter()
+        nn0, _ = nn.nn2(c, c0, m)
+        nn.rNNGPPredict(X, y, c, n, p, m, X0, c0, q, nn0, beta, th, S,
+                        "exponential", z=z)
+        dt = time.perf_counter() - t0
+        if r > 0:  # first request warms up
+            lat.append(dt * 1e3)
+    lat = np.array(lat)
+    return {"q": q, "samples": S, "requests": nreq,
+            "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)),
+            "site_samples_per_s": float(q * S / (np.median(lat) * 1e-3)),
+            "note": "sequential requests, host buffers in and out "
+                    "(nn2 + upload + kriging + download); no batching yet"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--m", type=int, default=15)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--cpu-iters", type=int, default=6)
+    ap.add_argument("--kriging-requests", type=int, default=10)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is "
+              "not supported (the multi-GPU sharding is a design only)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import spnngp_amd as nn
+    if nn.device_count() < 1:
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    n, m = args.n, args.m
+    print(f"[bench] generating n={n} m={m}", file=sys.stderr, flush=True)
+    c, X, y, idx, knn_s = make_data(n, m, args.seed)
+    p = X.shape[1]
+    beta0 = np.linalg.lstsq(X, y, rcond=None)[0]
+    start = dict(sigmaSqIG=[2.0, 2.0], tauSqIG=[2.0, 0.2], phiUnif=[3.0, 300.0],
+                 nuUnif=None, betaStarting=beta0, sigmaSqStarting=2.0,
+                 tauSqStarting=0.2, phiStarting=12.0, nuStarting=0.0,
+                 sigmaSqTuning=0.02, tauSqTuning=0.02, phiTuning=0.02,
+                 nuTuning=0.0)
+    md = nn.Model(c, idx["nnIndx"], idx["nnIndxLU"], m, "exponential", X, y)
+    print(f"[bench] warmup {args.warmup}", file=sys.stderr, flush=True)
+    w = md.rNNGP(nSamples=args.warmup, seed=11, **start)
+    th_last = w["p.theta.samples"][:, -1]
+    start_t = dict(start, sigmaSqStarting=th_last[0], tauSqStarting=th_last[1],
+                   phiStarting=th_last[2],
+                   betaStarting=w["p.beta.samples"][:, -1])
+    md.profile(True)
+    md.timing()  # drains the stream: nothing in flight when the clock starts
+    print(f"[bench] timing {args.steps} MH iterations", file=sys.stderr,
+          flush=True)
+    t0 = time.perf_counter()
+    out = md.rNNGP(nSamples=args.steps, seed=12, **start_t)
+    tm = md.timing()  # synchronizes the model's stream
+    dt = time.perf_counter() - t0
+    iters_s = args.steps / dt
+    bf_ms = tm["bf_ms"] / max(tm["bf_launches"], 1)
+    flops = sum(f_alg(min(i, m)) for i in range(1, min(n, m))) \
+        + (n - m) * f_alg(m)
+    achieved = flops / (bf_ms * 1e-3) / 1e12
+    md.close()
+
+    print("[bench] cpu baseline", file=sys.stderr, flush=True)
+    cpu = cpu_baseline(c, X, y, idx, m, args.cpu_iters, start) \
+        if args.cpu_iters > 0 else None
+    print("[bench] kriging", file=sys.stderr, flush=True)
+    krig = kriging_latency(c, X, y, m, args.kriging_requests, 10_000, 500,
+                           4) if args.kriging_requests > 0 else None
+
+    line = {
+        "metric": METRIC, "value": iters_s, "unit": "iter/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: seeded U[0,1]^2 coords, Gaussian X, w from the "
+                "NNGP prior",
+        "config": {"workload": "response NNGP MCMC (rNNGP), n=1e6, m=15, "
+                               "exponential, p=3, FP64",
+                   "n": n, "m": m, "p": p, "cov_model": "exponential",
+                   "precision": "fp64", "tuning": 0.02},
+        "acceptance": out["accept"] / args.steps,
+        "roofline": {"bound": "fp64-vector", "kernel": "updateBF (k_bf_lane<15,exp>)",
+                     "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": None,
+                     "flops_per_launch": flops,
+                     "launch_ms": bf_ms,
+                     "launches": tm["bf_launches"]},
+        "breakdown_ms_per_step": {
+            "updateBF": tm["bf_ms"] / args.steps,
+            "sums": tm["sum_ms"] / args.steps,
+            "host_and_sync": dt * 1e3 / args.steps
+                             - (tm["bf_ms"] + tm["sum_ms"]) / args.steps},
+        "cpu_baseline": cpu,
+        "kriging": krig,
+        "mixed": None,
+        "knn_s": knn_s,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

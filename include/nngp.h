@@ -133,6 +133,13 @@ int nngp_Q(const double *B, const double *F, const double *u,
            const double *v, int n, int m, const int *nnIndx,
            const int *nnIndxLU, double *q);
 
+/* Sequential NNGP draw w_i = B_i . w_N(i) + sqrt(F_i) z_i over i = 0..n-1
+ * — replaces updateRep, src/rNNGP.cpp:24-42 (also
+ * src/rNNGPReplicated.cpp:183-199), with the n standard normals z supplied
+ * by the caller.  Host code (each w_i depends on earlier ones). */
+int nngp_prior_draw(const double *B, const double *F, const int *nnIndx,
+                    const int *nnIndxLU, int n, const double *z, double *w);
+
 /* ------------------------------------------------------------------------
  * .Call drivers (the argument lists follow the SEXP order of
  * src/spNNGP.h; outputs are caller-allocated)
@@ -241,6 +248,15 @@ int nngp_model_rNNGP(nngp_model *model, const double *sigmaSqIG,
                      int verbose, int nReport, const nngp_rng *rng,
                      const nngp_opts *opts, double *pBetaSamples,
                      double *pThetaSamples, int *nAccept);
+
+/* Kernel timing on the model's stream (HIP events around each launch), for
+ * the run-time breakdown the reference reports as run.time
+ * (R/spNNGP.R:395,437).  enable != 0 starts (and zeroes) the counters.
+ * nngp_model_timing fills t[0] = total updateBF kernel ms, t[1] = number of
+ * updateBF launches, t[2] = total ms of the sum kernels, t[3] = their
+ * launches. */
+int nngp_model_profile(nngp_model *model, int enable);
+int nngp_model_timing(nngp_model *model, double *t);
 
 #ifdef __cplusplus
 }

@@ -16,13 +16,14 @@ from ._lib import (NNGP_ERR_ARG, NNGP_ERR_COV_MODEL, NNGP_ERR_HIP,
                    NNGP_ERR_UNSUPPORTED, NNGP_FP64, NNGP_MIXED, NNGP_OK,
                    NngpError, exported_symbols, load)
 from .nngp import (COV_MODELS, Model, Q, cNNGP, cov_index, device_count,
-                   mkNNIndx, mkNNIndxCB, n_indx, nn2, rNNGP, rNNGPPredict,
-                   updateBF, updateConjBF)
+                   mkNNIndx, mkNNIndxCB, n_indx, nn2, prior_draw, rNNGP,
+                   rNNGPPredict, updateBF, updateConjBF)
 
 __all__ = [
     "COV_MODELS", "Model", "NngpError", "Q", "cNNGP", "cov_index",
     "device_count", "exported_symbols", "load", "mkNNIndx", "mkNNIndxCB",
-    "n_indx", "nn2", "rNNGP", "rNNGPPredict", "updateBF", "updateConjBF",
+    "n_indx", "nn2", "prior_draw", "rNNGP", "rNNGPPredict", "updateBF",
+    "updateConjBF",
     "NNGP_OK", "NNGP_ERR_ARG", "NNGP_ERR_NOT_PD", "NNGP_ERR_COV_MODEL",
     "NNGP_ERR_HIP", "NNGP_ERR_NOMEM", "NNGP_ERR_INTERRUPT",
     "NNGP_ERR_UNSUPPORTED", "NNGP_FP64", "NNGP_MIXED",

@@ -33,6 +33,7 @@ EXPORTS = (
     "nngp_rNNGP", "nngp_cNNGP", "nngp_rNNGPPredict",
     "nngp_model_create", "nngp_model_destroy", "nngp_model_update",
     "nngp_model_qresid", "nngp_model_get_bf", "nngp_model_rNNGP",
+    "nngp_prior_draw", "nngp_model_profile", "nngp_model_timing",
 )
 
 _RNG_FN = C.CFUNCTYPE(C.c_double, C.c_void_p)
@@ -120,6 +121,9 @@ def load():
         [_vp, _dp, _dp, _dp, _vp, _dp] + [_d] * 8
         + [_i, _i, _i, C.POINTER(NngpRng), C.POINTER(NngpOpts), _dp, _dp,
            C.POINTER(_i)])
+    L.nngp_prior_draw.argtypes = [_dp, _dp, _ip, _ip, _i, _dp, _dp]
+    L.nngp_model_profile.argtypes = [_vp, _i]
+    L.nngp_model_timing.argtypes = [_vp, _dp]
     _lib = L
     return _lib
 

@@ -300,6 +300,7 @@ static int mcmc_loop(Model &md, const McmcSpec &S, Rng &R,
     NNGP_HIP(hipMemcpyAsync(hs + 2 * kStatsMax, md.err.p, sizeof(int),
                             hipMemcpyDeviceToHost, md.st.s));
     NNGP_HIP(hipStreamSynchronize(md.st.s));
+    if (md.prof) NNGP_TRY(md.collect());
     NNGP_TRY(md.check_err("updateBF"));
     const double QCurrent = hs[kStatsMax + 1];
     const double lpCur = log_post(theta, pc.logDet, QCurrent);
@@ -517,6 +518,28 @@ int nngp_Q(const double *B, const double *F, const double *u,
     return set_err(NNGP_ERR_HIP, "Q launch failed");
   NNGP_HIP(hipMemcpyAsync(q, out.p, sizeof(double), hipMemcpyDeviceToHost, st.s));
   NNGP_HIP(hipStreamSynchronize(st.s));
+  return NNGP_OK;
+}
+
+// src/rNNGP.cpp:24-42 with the normals supplied
+int nngp_prior_draw(const double *B, const double *F, const int *nnIndx,
+                    const int *nnIndxLU, int n, const double *z, double *w) {
+  clear_err();
+  if (n < 1 || !B || !F || !nnIndx || !nnIndxLU || !z || !w)
+    return set_err(NNGP_ERR_ARG, "NULL buffer or n < 1");
+  for (int i = 0; i < n; i++) {
+    double acc = 0.0;
+    const int off = nnIndxLU[i], cnt = nnIndxLU[(size_t)n + i];
+    for (int j = 0; j < cnt; j++) {
+      const int id = nnIndx[off + j];
+      if (id < 0 || id >= i)
+        return set_err(NNGP_ERR_ARG,
+                       "site %d has neighbor %d that is not a predecessor", i,
+                       id);
+      acc += B[off + j] * w[id];
+    }
+    w[i] = acc + std::sqrt(F[i]) * z[i];
+  }
   return NNGP_OK;
 }
 
@@ -960,6 +983,30 @@ int nngp_model_rNNGP(nngp_model *model, const double *sigmaSqIG,
   S.pThetaSamples = pThetaSamples;
   S.nAccept = nAccept;
   return mcmc_loop(md, S, R, opts);
+}
+
+int nngp_model_profile(nngp_model *model, int enable) {
+  clear_err();
+  if (!model) return set_err(NNGP_ERR_ARG, "model is NULL");
+  Model &md = model->md;
+  md.prof = enable != 0;
+  md.nev = 0;
+  md.tms[0] = md.tms[1] = 0.0;
+  md.tcount[0] = md.tcount[1] = 0;
+  return NNGP_OK;
+}
+
+int nngp_model_timing(nngp_model *model, double *t) {
+  clear_err();
+  if (!model || !t) return set_err(NNGP_ERR_ARG, "bad arguments");
+  Model &md = model->md;
+  NNGP_HIP(hipStreamSynchronize(md.st.s));
+  NNGP_TRY(md.collect());
+  t[0] = md.tms[0];
+  t[1] = (double)md.tcount[0];
+  t[2] = md.tms[1];
+  t[3] = (double)md.tcount[1];
+  return NNGP_OK;
 }
 
 int nngp_model_get_bf(nngp_model *model, int slot, double *B, double *F) {

@@ -340,7 +340,9 @@ int Model::bf(const CovArgs &c, int slot, bool storeB) {
   a.c = c;
   if (fill_int_launch(a.err, INT_MAX, st.s))
     return set_err(NNGP_ERR_HIP, "flag reset kernel failed");
+  const int ev = prof ? ev_begin(0) : -1;
   const int rc = bf_launch(a, st.s);
+  if (ev >= 0) ev_end(ev);
   if (rc == -1)
     return set_err(NNGP_ERR_UNSUPPORTED, "n.neighbors = %d exceeds 64", m);
   if (rc) return set_err(NNGP_ERR_HIP, "updateBF launch failed: %s",
@@ -358,10 +360,49 @@ int Model::stats(int slot, const double *beta, int off) {
   a.want_q = beta != nullptr && nz > 0;
   if (beta)
     for (int j = 0; j < p && j < kMaxP; j++) a.beta[j] = beta[j];
+  const int ev = prof ? ev_begin(1) : -1;
   const int rc = stats_launch(a, partials.as<double>(),
                               sums.as<double>() + off, st.s);
+  if (ev >= 0) ev_end(ev);
   if (rc) return set_err(NNGP_ERR_HIP, "sum kernel launch failed");
   return NNGP_OK;
+}
+
+int Model::ev_begin(int kind) {
+  if (nev == (int)evs.size()) {
+    EvPair e;
+    if (hipEventCreate(&e.a) != hipSuccess) return -1;
+    if (hipEventCreate(&e.b) != hipSuccess) {
+      (void)hipEventDestroy(e.a);
+      return -1;
+    }
+    evs.push_back(e);
+  }
+  EvPair &e = evs[nev];
+  e.kind = kind;
+  if (hipEventRecord(e.a, st.s) != hipSuccess) return -1;
+  return nev++;
+}
+
+void Model::ev_end(int idx) { (void)hipEventRecord(evs[idx].b, st.s); }
+
+int Model::collect() {
+  for (int k = 0; k < nev; k++) {
+    float ms = 0.0f;
+    NNGP_HIP(hipEventSynchronize(evs[k].b));
+    NNGP_HIP(hipEventElapsedTime(&ms, evs[k].a, evs[k].b));
+    tms[evs[k].kind] += ms;
+    tcount[evs[k].kind] += 1;
+  }
+  nev = 0;
+  return NNGP_OK;
+}
+
+Model::~Model() {
+  for (auto &e : evs) {
+    if (e.a) (void)hipEventDestroy(e.a);
+    if (e.b) (void)hipEventDestroy(e.b);
+  }
 }
 
 int Model::check_err(const char *what) {
@@ -390,6 +431,7 @@ int Model::pass(const CovArgs &c, int slot, const double *beta, bool storeB,
   NNGP_HIP(hipMemcpyAsync(hsums.as<double>() + 2 * kStatsMax, err.p,
                           sizeof(int), hipMemcpyDeviceToHost, st.s));
   NNGP_HIP(hipStreamSynchronize(st.s));
+  if (prof) NNGP_TRY(collect());
   NNGP_TRY(check_err("updateBF"));
   if (out) unpack(hsums.as<double>(), out);
   return NNGP_OK;

@@ -190,6 +190,22 @@ struct Model {
            PassOut *out);
   int check_err(const char *what);
   void unpack(const double *raw, PassOut *out) const;
+
+  // optional kernel timing with HIP events on st (kind 0: updateBF,
+  // kind 1: sum kernels); collect() after each stream synchronization
+  struct EvPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    int kind = 0;
+  };
+  bool prof = false;
+  std::vector<EvPair> evs;
+  int nev = 0;
+  double tms[2] = {0.0, 0.0};
+  long long tcount[2] = {0, 0};
+  int ev_begin(int kind);
+  void ev_end(int idx);
+  int collect();
+  ~Model();
 };
 
 }  // namespace nngp

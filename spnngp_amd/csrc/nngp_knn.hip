@@ -10,12 +10,15 @@
 // fl(dx^2) + fl(dy^2) rounds monotonically, nothing that could enter the
 // list is ever skipped, so the result is the exact m smallest set in the
 // reference's own arithmetic.  Products and sums here must not contract
-// into FMAs: they use __dmul_rn / __dadd_rn.
+// into FMAs (the reference rounds dx*dx, dy*dy and their sum separately), so
+// contraction is switched off for this whole file.
 //
 // Insertion is strict (d2 < current m-th), an equal distance goes after the
 // ones already held.  Exactly tied distances may be ordered differently from
 // the reference (whose order depends on its walk and its shell sort).
 #include "nngp_kernels.hpp"
+
+#pragma clang fp contract(off)
 
 namespace nngp {
 
@@ -75,26 +78,26 @@ __device__ inline void scan_strip(const StripIndex &S, int t, double qx,
   const double *sx = S.sx + base;
   const int *sid = S.sid + base;
   const int p = lower_bound_y(sy, len, qy);
-  for (int j = p; j < len; j++) {
-    const double dy = __dsub_rn(qy, sy[j]);
-    const double dy2 = __dmul_rn(dy, dy);
-    if (dy2 > top.worst()) break;
+  // candidate j: squared distance rounded as the reference rounds it,
+  // (x0-c0)^2 + (x1-c1)^2 with each square and the sum rounded; returns
+  // false once dy^2 alone exceeds the current m-th distance
+  auto visit = [&](int j) {
+    const double dy = qy - sy[j];
+    const double dy2 = dy * dy;
+    if (dy2 > top.worst()) return false;
     const int id = sid[j];
-    if (self >= 0 && id >= self) continue;
-    const double dx = __dsub_rn(qx, sx[j]);
-    const double d2 = __dadd_rn(__dmul_rn(dx, dx), dy2);
-    if (d2 < top.worst()) top.insert(d2, id);
-  }
-  for (int j = p - 1; j >= 0; j--) {
-    const double dy = __dsub_rn(qy, sy[j]);
-    const double dy2 = __dmul_rn(dy, dy);
-    if (dy2 > top.worst()) break;
-    const int id = sid[j];
-    if (self >= 0 && id >= self) continue;
-    const double dx = __dsub_rn(qx, sx[j]);
-    const double d2 = __dadd_rn(__dmul_rn(dx, dx), dy2);
-    if (d2 < top.worst()) top.insert(d2, id);
-  }
+    if (self < 0 || id < self) {
+      const double dx = qx - sx[j];
+      const double dx2 = dx * dx;
+      const double d2 = dx2 + dy2;
+      if (d2 < top.worst()) top.insert(d2, id);
+    }
+    return true;
+  };
+  for (int j = p; j < len; j++)
+    if (!visit(j)) break;
+  for (int j = p - 1; j >= 0; j--)
+    if (!visit(j)) break;
 }
 
 template <int M>
@@ -103,14 +106,14 @@ __device__ inline void search(const StripIndex &S, int t0, double qx,
                               TopM<M> &top) {
   scan_strip<M>(S, t0, qx, qy, self, top);
   for (int t = t0 - 1; t >= 0; t--) {
-    const double gap = __dsub_rn(qx, S.xhi[t]);
-    if (gap > 0.0 && __dmul_rn(gap, gap) > top.worst()) break;
+    const double gap = qx - S.xhi[t];
+    if (gap > 0.0 && gap * gap > top.worst()) break;
     scan_strip<M>(S, t, qx, qy, self, top);
   }
   if (left_only) return;
   for (int t = t0 + 1; t < S.nstrip; t++) {
-    const double gap = __dsub_rn(S.xlo[t], qx);
-    if (gap > 0.0 && __dmul_rn(gap, gap) > top.worst()) break;
+    const double gap = S.xlo[t] - qx;
+    if (gap > 0.0 && gap * gap > top.worst()) break;
     scan_strip<M>(S, t, qx, qy, self, top);
   }
 }

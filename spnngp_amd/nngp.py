@@ -134,6 +134,16 @@ def Q(B, F, u, v, nnIndx, nnIndxLU, m: int) -> float:
     return q.value
 
 
+def prior_draw(B, F, nnIndx, nnIndxLU, z):
+    """Sequential NNGP draw w_i = B_i.w_N(i) + sqrt(F_i) z_i
+    (updateRep, src/rNNGP.cpp:24-42)."""
+    F = _f64(F)
+    w = np.zeros(F.size)
+    check(load().nngp_prior_draw(_f64(B), F, _i32(nnIndx), _i32(nnIndxLU),
+                                 F.size, _f64(z), w))
+    return w
+
+
 # ---------------------------------------------------------------------------
 # drivers
 # ---------------------------------------------------------------------------
@@ -296,6 +306,17 @@ class Model:
             res["XtViy"] = np.array(out.XtViy[:p])
             res["ytViy"] = out.ytViy
         return res
+
+    def profile(self, enable=True):
+        """Start (and zero) or stop per-kernel event timing."""
+        check(load().nngp_model_profile(self._h, int(enable)))
+
+    def timing(self):
+        """{'bf_ms', 'bf_launches', 'sum_ms', 'sum_launches'} so far."""
+        t = np.zeros(4)
+        check(load().nngp_model_timing(self._h, t))
+        return {"bf_ms": t[0], "bf_launches": int(t[1]), "sum_ms": t[2],
+                "sum_launches": int(t[3])}
 
     def qresid(self, beta, slot=0):
         q = C.c_double()
