@@ -295,13 +295,8 @@ static int mcmc_loop(Model &md, const McmcSpec &S, Rng &R,
     NNGP_TRY(md.stats(cur, beta.data(), kStatsMax));
     NNGP_TRY(md.bf(cov_of(thetaCand), 1 - cur, false));
     NNGP_TRY(md.stats(1 - cur, beta.data(), 0));
-    NNGP_HIP(hipMemcpyAsync(hs, md.sums.p, sizeof(double) * 2 * kStatsMax,
-                            hipMemcpyDeviceToHost, md.st.s));
-    NNGP_HIP(hipMemcpyAsync(hs + 2 * kStatsMax, md.err.p, sizeof(int),
-                            hipMemcpyDeviceToHost, md.st.s));
-    NNGP_HIP(hipStreamSynchronize(md.st.s));
-    if (md.prof) NNGP_TRY(md.collect());
-    NNGP_TRY(md.check_err("updateBF"));
+    NNGP_TRY(md.fetch_sums());
+    NNGP_TRY(md.check_err("updateBF", 0));
     const double QCurrent = hs[kStatsMax + 1];
     const double lpCur = log_post(theta, pc.logDet, QCurrent);
     PassOut pn;
@@ -497,14 +492,16 @@ int nngp_Q(const double *B, const double *F, const double *u,
   Stream st;
   NNGP_TRY(st.create());
   const long long ni = n_indx(n, m);
-  DevBuf dB, dF, du, dv, dI, dL, part, out;
+  DevBuf dB, dF, du, dv, dI, dL, part, cnt, out;
   NNGP_TRY(dB.alloc(sizeof(double) * (size_t)ni));
   NNGP_TRY(dF.alloc(sizeof(double) * (size_t)n));
   NNGP_TRY(du.alloc(sizeof(double) * (size_t)n));
   NNGP_TRY(dv.alloc(sizeof(double) * (size_t)n));
   NNGP_TRY(dI.alloc(sizeof(int) * (size_t)ni));
   NNGP_TRY(dL.alloc(sizeof(int) * 2 * (size_t)n));
-  NNGP_TRY(part.alloc(sizeof(double) * (size_t)q_blocks(n)));
+  NNGP_TRY(part.alloc(sizeof(double) * (size_t)reduce_blocks() * kStatsMax));
+  NNGP_TRY(cnt.alloc(sizeof(unsigned int)));
+  NNGP_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned int), st.s));
   NNGP_TRY(out.alloc(sizeof(double)));
   NNGP_HIP(hipMemcpyAsync(dB.p, B, dB.bytes, hipMemcpyHostToDevice, st.s));
   NNGP_HIP(hipMemcpyAsync(dF.p, F, dF.bytes, hipMemcpyHostToDevice, st.s));
@@ -514,7 +511,8 @@ int nngp_Q(const double *B, const double *F, const double *u,
   NNGP_HIP(hipMemcpyAsync(dL.p, nnIndxLU, dL.bytes, hipMemcpyHostToDevice, st.s));
   QArgs a{dB.as<double>(), dF.as<double>(), du.as<double>(), dv.as<double>(),
           dI.as<int>(), dL.as<int>(), n};
-  if (q_launch(a, part.as<double>(), out.as<double>(), st.s))
+  if (q_launch(a, part.as<double>(), cnt.as<unsigned int>(), out.as<double>(),
+               st.s))
     return set_err(NNGP_ERR_HIP, "Q launch failed");
   NNGP_HIP(hipMemcpyAsync(q, out.p, sizeof(double), hipMemcpyDeviceToHost, st.s));
   NNGP_HIP(hipStreamSynchronize(st.s));
@@ -641,7 +639,10 @@ int nngp_cNNGP(const double *y, const double *X, int p, int n,
   const int nTheta = covModel == 2 ? 3 : 2;
   Model md;
   NNGP_TRY(md.create(n, m, p, coords, nnIndx, nnIndxLU, X, y, covModel, opts));
-  DevBuf dX, dy, dxy0, dX0, dnn0, dbeta, dVb, dy0, dy0v;
+  DevBuf dX, dy, dxy0, dX0, dnn0, dbeta, dVb, dy0, dy0v, perr;
+  NNGP_TRY(perr.alloc(sizeof(int)));
+  if (fill_int_launch(perr.as<int>(), INT_MAX, md.st.s))
+    return set_err(NNGP_ERR_HIP, "flag reset kernel failed");
   hipStream_t st = md.st.s;
   if (n0 > 0) {
     NNGP_TRY(dX.alloc(sizeof(double) * (size_t)n * p));
@@ -710,7 +711,7 @@ int nngp_cNNGP(const double *y, const double *X, int p, int n,
       pa.alpha = alpha;
       pa.y0 = dy0.as<double>() + (size_t)k * n0;
       pa.y0var = dy0v.as<double>() + (size_t)k * n0;
-      pa.err = md.err.as<int>();
+      pa.err = perr.as<int>();
       pa.c = c;
       if (conj_pred_launch(pa, st))
         return set_err(NNGP_ERR_HIP, "prediction launch failed");
@@ -727,12 +728,15 @@ int nngp_cNNGP(const double *y, const double *X, int p, int n,
                      k + 1, g);
   }
   if (n0 > 0) {
+    int bad = INT_MAX;
     NNGP_HIP(hipMemcpyAsync(y0Hat, dy0.p, dy0.bytes, hipMemcpyDeviceToHost, st));
     NNGP_HIP(hipMemcpyAsync(y0HatVar, dy0v.p, dy0v.bytes, hipMemcpyDeviceToHost, st));
-    NNGP_HIP(hipMemcpyAsync(md.hsums.as<double>() + 2 * kStatsMax, md.err.p,
-                            sizeof(int), hipMemcpyDeviceToHost, st));
+    NNGP_HIP(hipMemcpyAsync(&bad, perr.p, sizeof(int), hipMemcpyDeviceToHost, st));
     NNGP_HIP(hipStreamSynchronize(st));
-    NNGP_TRY(md.check_err("prediction"));
+    if (bad != INT_MAX)
+      return set_err(NNGP_ERR_NOT_PD,
+                     "c++ Rf_error: dpotrf failed\n(prediction system of "
+                     "held-out site %d is not positive definite)", bad);
   }
   return NNGP_OK;
 }

@@ -24,8 +24,8 @@ namespace nngp {
 constexpr int kLaneBlock = 256;
 constexpr int kCoopWaves = 4;
 constexpr int kCoopMaxM = 64;
-constexpr int kRedBlock = 256;
-constexpr int kRedMaxBlocks = 1024;
+constexpr int kRedThreads = 512;  // threads per block of the sum kernels
+constexpr int kRedBlocks = 512;   // at most this many partial rows
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -288,53 +288,77 @@ int bf_launch(const BfArgs &a, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------
-// deterministic sums: fixed grid, fixed-order tree inside the block, one
-// partial per block, then a single block adds the partials in block order
+// deterministic sums.  A fixed grid of kRedBlocks blocks; every thread adds
+// a fixed set of sites in a fixed order, each block reduces its threads with
+// a fixed butterfly per wavefront and the wavefronts in order, and writes one
+// partial row.  The last block to finish (found with an atomic counter, which
+// only decides who adds, not in what order) adds the partial rows: thread b
+// holds row b, the same butterfly, the waves in order.  Results are
+// bit-reproducible from run to run.
 // ---------------------------------------------------------------------------
 template <int NV>
-__device__ inline void block_sum_store(double (&acc)[NV], double *dst) {
-  __shared__ double red[kRedBlock / 64][NV];
+__device__ inline void block_reduce(double (&acc)[NV], double *dst) {
+  __shared__ double red[kRedThreads / 64][NV];
   const int w = (int)threadIdx.x >> 6, t = (int)threadIdx.x & 63;
 #pragma unroll
   for (int v = 0; v < NV; v++) {
     double x = acc[v];
+#pragma unroll
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
     if (t == 0) red[w][v] = x;
   }
   __syncthreads();
   if ((int)threadIdx.x < NV) {
     double s = 0.0;
-    for (int k = 0; k < kRedBlock / 64; k++) s += red[k][threadIdx.x];
+#pragma unroll
+    for (int k = 0; k < kRedThreads / 64; k++) s += red[k][threadIdx.x];
     dst[threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
+// Called by every block after its partial row is stored.
+template <int NV>
+__device__ inline void last_block_sum(const double *partials,
+                                      unsigned int *counter, double *out,
+                                      int *err, double *err_out) {
+  __shared__ bool is_last;
+  __threadfence();
+  if (threadIdx.x == 0)
+    is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  double acc[NV];
+  const int b = (int)threadIdx.x;
+#pragma unroll
+  for (int v = 0; v < NV; v++)
+    acc[v] = b < (int)gridDim.x ? partials[(size_t)b * kStatsMax + v] : 0.0;
+  block_reduce<NV>(acc, out);
+  if (threadIdx.x == 0) {
+    *counter = 0u;  // ready for the next launch
+    if (err) {
+      *err_out = (double)*err;
+      *err = INT_MAX;
+    }
   }
 }
 
-__global__ void k_sum_partials(const double *partials, int nblocks, int nv,
-                               int stride, double *out) {
-  const int v = (int)threadIdx.x;
-  if (v >= nv) return;
-  double s = 0.0;
-  for (int b = 0; b < nblocks; b++) s += partials[(size_t)b * stride + v];
-  out[v] = s;
-}
-
-static int red_blocks(int n) {
-  int b = (n + kRedBlock - 1) / kRedBlock;
-  return b < 1 ? 1 : (b > kRedMaxBlocks ? kRedMaxBlocks : b);
-}
-int stats_blocks(int n) { return red_blocks(n); }
-int q_blocks(int n) { return red_blocks(n); }
+int reduce_blocks() { return kRedBlocks; }
 
 template <int NZ>
-__global__ __launch_bounds__(kRedBlock) void k_stats(StatsArgs A,
-                                                     double *partials) {
+__global__ __launch_bounds__(kRedThreads) void k_stats(StatsArgs A,
+                                                       double *partials,
+                                                       unsigned int *counter,
+                                                       double *out) {
   constexpr int NG = NZ * (NZ + 1) / 2;
   constexpr int NV = 2 + NG;
   double acc[NV];
 #pragma unroll
   for (int v = 0; v < NV; v++) acc[v] = 0.0;
-  for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < A.n;
-       i += gridDim.x * kRedBlock) {
+  const int stride = (int)gridDim.x * kRedThreads;
+#pragma unroll 4
+  for (int i = blockIdx.x * kRedThreads + threadIdx.x; i < A.n; i += stride) {
     const double f = A.F[i];
     acc[0] += log(f);
     if constexpr (NZ > 0) {
@@ -354,16 +378,25 @@ __global__ __launch_bounds__(kRedBlock) void k_stats(StatsArgs A,
         for (int q = 0; q <= p; q++) acc[g++] += e[p] * e[q] / f;
     }
   }
-  block_sum_store<NV>(acc, partials + (size_t)blockIdx.x * kStatsMax);
+  block_reduce<NV>(acc, partials + (size_t)blockIdx.x * kStatsMax);
+  last_block_sum<NV>(partials, counter, out, A.err, A.err_out);
 }
 
-int stats_launch(const StatsArgs &a, double *partials, double *out,
-                 hipStream_t st) {
-  const int nb = red_blocks(a.n);
+static int red_grid(int n) {
+  const int b = (n + kRedThreads - 1) / kRedThreads;
+  return b < 1 ? 1 : (b > kRedBlocks ? kRedBlocks : b);
+}
+
+int stats_launch(const StatsArgs &a, double *partials, unsigned int *counter,
+                 double *out, hipStream_t st) {
+  const int nb = red_grid(a.n);
   const int nz = a.E ? a.nz : 0;
   switch (nz) {
-#define NNGP_STATS_CASE(K) \
-  case K: hipLaunchKernelGGL(k_stats<K>, dim3(nb), dim3(kRedBlock), 0, st, a, partials); break;
+#define NNGP_STATS_CASE(K)                                                 \
+  case K:                                                                 \
+    hipLaunchKernelGGL(k_stats<K>, dim3(nb), dim3(kRedThreads), 0, st, a, \
+                       partials, counter, out);                           \
+    break;
     NNGP_STATS_CASE(0)
     NNGP_STATS_CASE(2)
     NNGP_STATS_CASE(3)
@@ -376,17 +409,14 @@ int stats_launch(const StatsArgs &a, double *partials, double *out,
 #undef NNGP_STATS_CASE
     default: return -1;
   }
-  const int nv = 2 + nz * (nz + 1) / 2;
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, partials, nb,
-                     nv, kStatsMax, out);
   return (int)hipGetLastError();
 }
 
-__global__ __launch_bounds__(kRedBlock) void k_quadform(QArgs A,
-                                                        double *partials) {
+__global__ __launch_bounds__(kRedThreads) void k_quadform(
+    QArgs A, double *partials, unsigned int *counter, double *out) {
   double acc[1] = {0.0};
-  for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < A.n;
-       i += gridDim.x * kRedBlock) {
+  const int stride = (int)gridDim.x * kRedThreads;
+  for (int i = blockIdx.x * kRedThreads + threadIdx.x; i < A.n; i += stride) {
     const int cnt = A.lu[A.n + i], off = A.lu[i];
     double a = 0.0, b = 0.0;
     for (int j = 0; j < cnt; j++) {
@@ -397,15 +427,14 @@ __global__ __launch_bounds__(kRedBlock) void k_quadform(QArgs A,
     }
     acc[0] += (A.u[i] - a) * (A.v[i] - b) / A.F[i];
   }
-  block_sum_store<1>(acc, partials + blockIdx.x);
+  block_reduce<1>(acc, partials + (size_t)blockIdx.x * kStatsMax);
+  last_block_sum<1>(partials, counter, out, nullptr, nullptr);
 }
 
-int q_launch(const QArgs &a, double *partials, double *out, hipStream_t st) {
-  const int nb = red_blocks(a.n);
-  hipLaunchKernelGGL(k_quadform, dim3(nb), dim3(kRedBlock), 0, st, a,
-                     partials);
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, partials, nb, 1,
-                     1, out);
+int q_launch(const QArgs &a, double *partials, unsigned int *counter,
+             double *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_quadform, dim3(red_grid(a.n)), dim3(kRedThreads), 0,
+                     st, a, partials, counter, out);
   return (int)hipGetLastError();
 }
 

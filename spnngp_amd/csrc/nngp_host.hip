@@ -312,11 +312,15 @@ int Model::create(int n_, int m_, int p_, const double *coords,
     NNGP_TRY(F[s].alloc(sizeof(double) * (size_t)n));
     if (nz) NNGP_TRY(E[s].alloc(sizeof(double) * (size_t)nz * n));
   }
-  NNGP_TRY(partials.alloc(sizeof(double) * (size_t)stats_blocks(n) *
+  NNGP_TRY(partials.alloc(sizeof(double) * (size_t)reduce_blocks() *
                           kStatsMax));
-  NNGP_TRY(sums.alloc(sizeof(double) * 2 * kStatsMax + 64));
+  NNGP_TRY(counter.alloc(sizeof(unsigned int)));
+  NNGP_HIP(hipMemsetAsync(counter.p, 0, sizeof(unsigned int), st.s));
+  NNGP_TRY(sums.alloc(sizeof(double) * kSumsLen));
   NNGP_TRY(err.alloc(sizeof(int)));
-  NNGP_TRY(hsums.alloc(sizeof(double) * 2 * kStatsMax + 64));
+  if (fill_int_launch(err.as<int>(), INT_MAX, st.s))
+    return set_err(NNGP_ERR_HIP, "flag reset kernel failed");
+  NNGP_TRY(hsums.alloc(sizeof(double) * kSumsLen));
   NNGP_HIP(hipStreamSynchronize(st.s));
   return NNGP_OK;
 }
@@ -336,10 +340,8 @@ int Model::bf(const CovArgs &c, int slot, bool storeB) {
   a.nz = nz;
   a.zs = zs;
   a.E = nz ? E[slot].as<double>() : nullptr;
-  a.err = err.as<int>();
+  a.err = err.as<int>();  // INT_MAX here: every stats pass resets it
   a.c = c;
-  if (fill_int_launch(a.err, INT_MAX, st.s))
-    return set_err(NNGP_ERR_HIP, "flag reset kernel failed");
   const int ev = prof ? ev_begin(0) : -1;
   const int rc = bf_launch(a, st.s);
   if (ev >= 0) ev_end(ev);
@@ -360,11 +362,22 @@ int Model::stats(int slot, const double *beta, int off) {
   a.want_q = beta != nullptr && nz > 0;
   if (beta)
     for (int j = 0; j < p && j < kMaxP; j++) a.beta[j] = beta[j];
+  a.err = err.as<int>();
+  a.err_out = sums.as<double>() + 2 * kStatsMax + (off ? 1 : 0);
   const int ev = prof ? ev_begin(1) : -1;
   const int rc = stats_launch(a, partials.as<double>(),
+                              counter.as<unsigned int>(),
                               sums.as<double>() + off, st.s);
   if (ev >= 0) ev_end(ev);
   if (rc) return set_err(NNGP_ERR_HIP, "sum kernel launch failed");
+  return NNGP_OK;
+}
+
+int Model::fetch_sums() {
+  NNGP_HIP(hipMemcpyAsync(hsums.p, sums.p, sizeof(double) * kSumsLen,
+                          hipMemcpyDeviceToHost, st.s));
+  NNGP_HIP(hipStreamSynchronize(st.s));
+  if (prof) NNGP_TRY(collect());
   return NNGP_OK;
 }
 
@@ -405,13 +418,12 @@ Model::~Model() {
   }
 }
 
-int Model::check_err(const char *what) {
-  const int bad = *(reinterpret_cast<int *>(hsums.as<double>() +
-                                            2 * kStatsMax));
-  if (bad != INT_MAX)
+int Model::check_err(const char *what, int which) {
+  const double bad = hsums.as<double>()[2 * kStatsMax + which];
+  if (bad != (double)INT_MAX)
     return set_err(NNGP_ERR_NOT_PD,
                    "c++ Rf_error: dpotrf failed\n(%s: the neighbor covariance"
-                   " of site %d is not positive definite)",
+                   " of site %.0f is not positive definite)",
                    what, bad);
   return NNGP_OK;
 }
@@ -426,13 +438,8 @@ int Model::pass(const CovArgs &c, int slot, const double *beta, bool storeB,
                 PassOut *out) {
   NNGP_TRY(bf(c, slot, storeB));
   NNGP_TRY(stats(slot, beta, 0));
-  NNGP_HIP(hipMemcpyAsync(hsums.p, sums.p, sizeof(double) * kStatsMax,
-                          hipMemcpyDeviceToHost, st.s));
-  NNGP_HIP(hipMemcpyAsync(hsums.as<double>() + 2 * kStatsMax, err.p,
-                          sizeof(int), hipMemcpyDeviceToHost, st.s));
-  NNGP_HIP(hipStreamSynchronize(st.s));
-  if (prof) NNGP_TRY(collect());
-  NNGP_TRY(check_err("updateBF"));
+  NNGP_TRY(fetch_sums());
+  NNGP_TRY(check_err("updateBF", 0));
   if (out) unpack(hsums.as<double>(), out);
   return NNGP_OK;
 }

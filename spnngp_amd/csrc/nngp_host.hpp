@@ -164,6 +164,10 @@ struct Rng {
 // ---------------------------------------------------------------------------
 // device-resident model
 // ---------------------------------------------------------------------------
+// device sums buffer: two passes of kStatsMax values, then their two
+// updateBF failure flags
+constexpr int kSumsLen = 2 * kStatsMax + 2;
+
 struct PassOut {
   double logDet = 0.0, q = 0.0;
   double gram[kStatsMax];  // lower triangle of E^T F^-1 E, row-packed
@@ -175,8 +179,10 @@ struct Model {
   Stream st;
   DevBuf xy, nnIndx, lu, Z;
   DevBuf B[2], F[2], E[2];
-  DevBuf partials, sums, err;
-  PinnedBuf hsums;  // 2 * kStatsMax + 1 ints' worth, host mirror
+  // sums: [0, kStatsMax) pass at off 0, [kStatsMax, 2 kStatsMax) pass at
+  // off kStatsMax, then the updateBF failure flag of each (as doubles)
+  DevBuf partials, counter, sums, err;
+  PinnedBuf hsums;  // host mirror of sums
 
   int create(int n, int m, int p, const double *coords, const int *nnIndx,
              const int *nnIndxLU, const double *X, const double *y, int cov,
@@ -188,7 +194,11 @@ struct Model {
   // one full pass: bf + stats, synchronized, results in out
   int pass(const CovArgs &c, int slot, const double *beta, bool storeB,
            PassOut *out);
-  int check_err(const char *what);
+  // copy sums (and the failure flags) to hsums and synchronize
+  int fetch_sums();
+  // failure flag of the pass whose sums sit at off 0 (which = 0) or at off
+  // kStatsMax (which = 1), as last fetched
+  int check_err(const char *what, int which);
   void unpack(const double *raw, PassOut *out) const;
 
   // optional kernel timing with HIP events on st (kind 0: updateBF,

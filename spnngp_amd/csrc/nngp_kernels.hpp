@@ -37,6 +37,8 @@ struct StatsArgs {
   int nz;                // p + 1
   double beta[kMaxP];
   int want_q;
+  int *err;              // optional: the updateBF failure flag; its value is
+  double *err_out;       // copied to *err_out and the flag reset to INT_MAX
 };
 
 // Plain Q(u, v) of src/util.cpp:261-280 over stored B and F.
@@ -47,14 +49,14 @@ struct QArgs {
 };
 
 int bf_launch(const BfArgs &a, hipStream_t st);
-// Writes kStatsMax block partials per block into partials and the reduced
-// sums into out (device, kStatsMax doubles).  partials must hold
-// stats_blocks(n) * kStatsMax doubles.
-int stats_blocks(int n);
-int stats_launch(const StatsArgs &a, double *partials, double *out,
-                 hipStream_t st);
-int q_blocks(int n);
-int q_launch(const QArgs &a, double *partials, double *out, hipStream_t st);
+// One launch each: block partials go to `partials` (reduce_blocks() *
+// kStatsMax doubles), the final sums to `out` (device).  `counter` is one
+// unsigned int, zero before the first launch; every launch leaves it zero.
+int reduce_blocks();
+int stats_launch(const StatsArgs &a, double *partials, unsigned int *counter,
+                 double *out, hipStream_t st);
+int q_launch(const QArgs &a, double *partials, unsigned int *counter,
+             double *out, hipStream_t st);
 
 // *p = v on the stream (no memset node, safe under graph capture).
 int fill_int_launch(int *p, int v, hipStream_t st);
