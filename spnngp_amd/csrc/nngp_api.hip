@@ -13,12 +13,6 @@
 namespace nngp {
 const char *last_error();
 
-static double gram_at(const double *raw, int a, int b) {
-  // lower triangle of E^T F^-1 E, packed by rows after logDet and Q
-  if (a < b) std::swap(a, b);
-  return raw[2 + a * (a + 1) / 2 + b];
-}
-
 static int check_cov(int covModel) {
   if (covModel < 0 || covModel > 3)
     return set_err(NNGP_ERR_COV_MODEL,
@@ -228,12 +222,11 @@ static int mcmc_loop(Model &md, const McmcSpec &S, Rng &R,
 
   int cur = 0;
   PassOut pc;  // current slot's logDet and cross-products
-  NNGP_TRY(md.pass(cov_of(theta), cur, nullptr, false, &pc));
+  NNGP_TRY(md.pass(cov_of(theta), cur, false, &pc));
   std::vector<double> tmp_pp((size_t)p * p, 0.0), tmp_p(p), tmp_p2(p);
   std::vector<double> hB, hF, wrep;
   bool thetaUpdate = true;
   int accept = 0, batchAccept = 0, status = 0, repCnt = 0;
-  double *hs = md.hsums.as<double>();
 
   for (int s = 0; s < S.nSamples; s++) {
     if (thetaUpdate) {
@@ -261,7 +254,7 @@ static int mcmc_loop(Model &md, const McmcSpec &S, Rng &R,
       for (int i = 0; i < n; i++) yr[i] = 0.0;
       for (int j = 0; j < p; j++)
         for (int i = 0; i < n; i++) yr[i] += beta[j] * S.X[(size_t)j * n + i];
-      NNGP_TRY(md.pass(cov_of(theta), cur, nullptr, true, nullptr));
+      NNGP_TRY(md.pass(cov_of(theta), cur, true, nullptr));
       hB.resize((size_t)md.nIndx);
       hF.resize((size_t)n);
       NNGP_HIP(hipMemcpy(hB.data(), md.B[cur].p, sizeof(double) * hB.size(),
@@ -291,17 +284,15 @@ static int mcmc_loop(Model &md, const McmcSpec &S, Rng &R,
       thetaCand[iN] =
           logit_inv(R.rnorm(logit_ab(theta[iN], nA, nB), tuning[iN]), nA, nB);
 
-    // Q(r, r) at the current theta for the new beta, then the candidate
-    NNGP_TRY(md.stats(cur, beta.data(), kStatsMax));
-    NNGP_TRY(md.bf(cov_of(thetaCand), 1 - cur, false));
-    NNGP_TRY(md.stats(1 - cur, beta.data(), 0));
-    NNGP_TRY(md.fetch_sums());
-    NNGP_TRY(md.check_err("updateBF", 0));
-    const double QCurrent = hs[kStatsMax + 1];
-    const double lpCur = log_post(theta, pc.logDet, QCurrent);
+    // the candidate's B, F, log-det and cross-products in one pass; both
+    // quadratic forms Q(r, r), r = X beta - y, follow from cross-products
+    // (src/rNNGP.cpp:299-315, 336-338)
     PassOut pn;
-    md.unpack(hs, &pn);
-    const double lpCand = log_post(thetaCand, pn.logDet, pn.q);
+    NNGP_TRY(md.pass(cov_of(thetaCand), 1 - cur, false, &pn));
+    const double QCurrent = quad_resid(pc, beta.data(), p);
+    const double lpCur = log_post(theta, pc.logDet, QCurrent);
+    const double QCand = quad_resid(pn, beta.data(), p);
+    const double lpCand = log_post(thetaCand, pn.logDet, QCand);
 
     if (R.runif(0.0, 1.0) <= std::exp(lpCand - lpCur)) {
       thetaUpdate = true;
@@ -449,7 +440,7 @@ static int bf_host(double *B, double *F, const double *coords,
   NNGP_TRY(md.create(n, m, 0, coords, nnIndx, nnIndxLU, nullptr, nullptr,
                      c.cov, opts));
   PassOut out;
-  NNGP_TRY(md.pass(c, 0, nullptr, true, &out));
+  NNGP_TRY(md.pass(c, 0, true, &out));
   // B for site 0 is never written (it has no neighbors)
   NNGP_HIP(hipMemcpy(B, md.B[0].p, sizeof(double) * (size_t)md.nIndx,
                      hipMemcpyDeviceToHost));
@@ -492,16 +483,14 @@ int nngp_Q(const double *B, const double *F, const double *u,
   Stream st;
   NNGP_TRY(st.create());
   const long long ni = n_indx(n, m);
-  DevBuf dB, dF, du, dv, dI, dL, part, cnt, out;
+  DevBuf dB, dF, du, dv, dI, dL, part, out;
   NNGP_TRY(dB.alloc(sizeof(double) * (size_t)ni));
   NNGP_TRY(dF.alloc(sizeof(double) * (size_t)n));
   NNGP_TRY(du.alloc(sizeof(double) * (size_t)n));
   NNGP_TRY(dv.alloc(sizeof(double) * (size_t)n));
   NNGP_TRY(dI.alloc(sizeof(int) * (size_t)ni));
   NNGP_TRY(dL.alloc(sizeof(int) * 2 * (size_t)n));
-  NNGP_TRY(part.alloc(sizeof(double) * (size_t)reduce_blocks() * kStatsMax));
-  NNGP_TRY(cnt.alloc(sizeof(unsigned int)));
-  NNGP_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned int), st.s));
+  NNGP_TRY(part.alloc(sizeof(double) * (size_t)rows_capacity(n)));
   NNGP_TRY(out.alloc(sizeof(double)));
   NNGP_HIP(hipMemcpyAsync(dB.p, B, dB.bytes, hipMemcpyHostToDevice, st.s));
   NNGP_HIP(hipMemcpyAsync(dF.p, F, dF.bytes, hipMemcpyHostToDevice, st.s));
@@ -510,10 +499,8 @@ int nngp_Q(const double *B, const double *F, const double *u,
   NNGP_HIP(hipMemcpyAsync(dI.p, nnIndx, dI.bytes, hipMemcpyHostToDevice, st.s));
   NNGP_HIP(hipMemcpyAsync(dL.p, nnIndxLU, dL.bytes, hipMemcpyHostToDevice, st.s));
   QArgs a{dB.as<double>(), dF.as<double>(), du.as<double>(), dv.as<double>(),
-          dI.as<int>(), dL.as<int>(), n};
-  if (q_launch(a, part.as<double>(), cnt.as<unsigned int>(), out.as<double>(),
-               st.s))
-    return set_err(NNGP_ERR_HIP, "Q launch failed");
+          dI.as<int>(), dL.as<int>(), n, part.as<double>(), out.as<double>()};
+  if (q_launch(a, st.s)) return set_err(NNGP_ERR_HIP, "Q launch failed");
   NNGP_HIP(hipMemcpyAsync(q, out.p, sizeof(double), hipMemcpyDeviceToHost, st.s));
   NNGP_HIP(hipStreamSynchronize(st.s));
   return NNGP_OK;
@@ -666,7 +653,7 @@ int nngp_cNNGP(const double *y, const double *X, int p, int n,
     const double nu = covModel == 2 ? thetaAlpha[(size_t)k * nTheta + 2] : 0.0;
     const CovArgs c = make_cov(covModel, 1.0, alpha, phi, nu);
     PassOut po;
-    NNGP_TRY(md.pass(c, 0, nullptr, false, &po));
+    NNGP_TRY(md.pass(c, 0, false, &po));
     std::fill(tmp_pp.begin(), tmp_pp.end(), 0.0);
     for (int i = 0; i < p; i++) {
       tmp_p[i] = gram_at(po.gram, p, i);
@@ -909,11 +896,11 @@ int nngp_model_update(nngp_model *model, const double *theta, int slot,
   const CovArgs c = make_cov(md.cov, theta[0], theta[1], theta[2],
                              md.cov == 2 ? theta[3] : 0.0);
   PassOut po;
-  NNGP_TRY(md.pass(c, slot, md.nz ? beta : nullptr, true, &po));
+  NNGP_TRY(md.pass(c, slot, true, &po));
   if (out) {
     std::memset(out, 0, sizeof *out);
     out->logDet = po.logDet;
-    out->qResid = po.q;
+    out->qResid = (md.nz && beta) ? quad_resid(po, beta, md.p) : 0.0;
     if (want_xprod && md.nz) {
       const int p = md.p;
       for (int i = 0; i < p; i++) {
@@ -934,11 +921,7 @@ int nngp_model_qresid(nngp_model *model, int slot, const double *beta,
     return set_err(NNGP_ERR_ARG, "bad arguments");
   Model &md = model->md;
   if (!md.nz) return set_err(NNGP_ERR_ARG, "model was created without X, y");
-  NNGP_TRY(md.stats(slot, beta, 0));
-  NNGP_HIP(hipMemcpyAsync(md.hsums.p, md.sums.p, sizeof(double) * 2,
-                          hipMemcpyDeviceToHost, md.st.s));
-  NNGP_HIP(hipStreamSynchronize(md.st.s));
-  *q = md.hsums.as<double>()[1];
+  *q = quad_resid(md.slot_out[slot], beta, md.p);
   return NNGP_OK;
 }
 

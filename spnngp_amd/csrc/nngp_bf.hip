@@ -1,6 +1,6 @@
 // nngp_bf.hip — neighbor-system kernels: updateBF / updateConjBF
 // (src/rNNGP.cpp:46-101, src/cNNGP.cpp:24-69), the quadratic form Q
-// (src/util.cpp:261-280) and the fused sums the MCMC driver needs.
+// (src/util.cpp:261-280) and the site sums the samplers need.
 //
 // Two updateBF kernels:
 //  * k_bf_lane<M, COV>: one site per lane, the whole M x M system in
@@ -8,13 +8,22 @@
 //    with fewer than M neighbors (the first m sites of the NNGP order) are
 //    padded with far-away points: their rows/columns of C are exactly zero
 //    off the diagonal, so the real block factors bit-for-bit as if unpadded
-//    and the padded coefficients come out exactly 0.
+//    and the padded coefficients come out exactly 0.  The kernel also adds
+//    up its block's site terms (log F_i and e_i e_i^T / F_i, with e_i the
+//    residual vector of X and y at site i), so one launch gives everything
+//    a Metropolis step needs.
 //  * k_bf_coop: one site per wavefront, the matrix in LDS, one row per lane
-//    (any m <= 64, any covariance including the Bessel-K matern).
+//    (any m <= 64, any covariance including the Bessel-K matern); it stores
+//    the residual vectors and k_stats adds them up.
 // Both compute C = s2 R(N,N) + t2 I, c = s2 R(N,i), the Cholesky factor of C
 // (pivots kept as reciprocal square roots), v = L^-1 c, B = L^-T v and
 // F = s2 + t2 - v.v, i.e. the reference's potrf -> potri -> symv result
 // without forming the inverse.
+//
+// Sums are deterministic: a fixed grid, every thread adds a fixed set of
+// sites in a fixed order, each block reduces with a fixed butterfly per
+// wavefront and its wavefronts in order into one partial row, and
+// k_rows_sum adds the rows of each column in a fixed order.
 #include <climits>
 
 #include "nngp_kernels.hpp"
@@ -24,8 +33,8 @@ namespace nngp {
 constexpr int kLaneBlock = 256;
 constexpr int kCoopWaves = 4;
 constexpr int kCoopMaxM = 64;
-constexpr int kRedThreads = 512;  // threads per block of the sum kernels
-constexpr int kRedBlocks = 512;   // at most this many partial rows
+constexpr int kRedThreads = 256;  // threads per block of the sum kernels
+constexpr int kRedBlocks = 1024;  // grid of the sum kernels (at most)
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -39,93 +48,147 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
   return __hiloint2double(hi, lo);
 }
 
-// Site with no neighbors (site 0): F = s2 + t2, e_j(i) = Z(i, j).
-__device__ inline void site_without_neighbors(const BfArgs &A, int i) {
-  A.F[i] = A.c.diag;
-  if (A.E)
-    for (int j = 0; j < A.nz; j++)
-      A.E[(size_t)j * A.n + i] = A.Z[(size_t)i * A.zs + j];
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
+}
+
+int rows_capacity(int n) {
+  const int lane_rows = (n + kLaneBlock - 1) / kLaneBlock;
+  return lane_rows > kRedBlocks ? lane_rows : kRedBlocks;
 }
 
 // ---------------------------------------------------------------------------
 // one site per lane
 // ---------------------------------------------------------------------------
+
+// The block's partial row of one pass: column 0 sum log F_i, column 1 zero
+// (quadratic forms are formed from the cross-products), then the lower
+// triangle of e e^T / F row by row.  Called by every thread of the block.
+template <int NT>
+__device__ inline void block_site_terms(const BfArgs &A, bool valid,
+                                        double F, const double (&e)[kMaxNZ],
+                                        int row) {
+  __shared__ double red[NT / 64][kStatsMax];
+  const int w = (int)threadIdx.x >> 6, t = (int)threadIdx.x & 63;
+  const double rf = valid ? 1.0 / F : 0.0;
+  double x = wave_sum(valid ? log(F) : 0.0);
+  if (t == 0) red[w][0] = x;
+  int g = 2;
+#pragma unroll
+  for (int p = 0; p < kMaxNZ; p++) {
+    if (p < A.nz) {
+#pragma unroll
+      for (int q = 0; q <= p; q++) {
+        x = wave_sum(e[p] * e[q] * rf);
+        if (t == 0) red[w][g + q] = x;
+      }
+    }
+    g += p + 1;
+  }
+  __syncthreads();
+  const int nv = 2 + A.nz * (A.nz + 1) / 2;
+  const int k = (int)threadIdx.x;
+  if (k < nv) {
+    double s = 0.0;
+    if (k != 1)
+#pragma unroll
+      for (int ww = 0; ww < NT / 64; ww++) s += red[ww][k];
+    A.rows[(size_t)k * A.rows_cap + row] = s;
+  }
+}
+
 template <int M, int COV>
 __global__ __launch_bounds__(kLaneBlock) void k_bf_lane(BfArgs A) {
   const int b = xcd_block((A.n + kLaneBlock - 1) / kLaneBlock);
-  if (b < 0) return;
+  if (b < 0) return;  // whole padding block
   const int i = b * kLaneBlock + (int)threadIdx.x;
-  if (i >= A.n) return;
-  const int cnt = A.lu[A.n + i];
-  if (cnt == 0) {
-    site_without_neighbors(A, i);
-    return;
-  }
-  const int off = A.lu[i];
+  const bool valid = i < A.n;
   const CovArgs &c = A.c;
+  double Fi = 1.0;
+  double e[kMaxNZ];
+#pragma unroll
+  for (int j = 0; j < kMaxNZ; j++) e[j] = 0.0;
 
-  // The index array carries 64 padding entries, so off + k is in bounds;
-  // entries past this site's count are not its neighbors and are replaced
-  // by index 0 before any use.
-  int nb[M];
+  if (valid) {
+    const int cnt = A.lu[A.n + i];
+    const int off = A.lu[i];
+    if (cnt == 0) {  // site 0: F = s2 + t2, residual = the data
+      Fi = c.diag;
 #pragma unroll
-  for (int k = 0; k < M; k++) nb[k] = A.nnIndx[off + k];
-  if (cnt < M) {
+      for (int j = 0; j < kMaxNZ; j++)
+        if (j < A.nz) e[j] = A.Z[(size_t)i * A.zs + j];
+    } else {
+      // The index array carries 64 padding entries, so off + k is in
+      // bounds; entries past this site's count are not its neighbors and
+      // are replaced by index 0 before any use.
+      int nb[M];
 #pragma unroll
-    for (int k = 0; k < M; k++)
-      if (k >= cnt) nb[k] = 0;
-  }
-  double px[M], py[M];
+      for (int k = 0; k < M; k++) nb[k] = A.nnIndx[off + k];
+      if (cnt < M) {
 #pragma unroll
-  for (int k = 0; k < M; k++) {
-    const double2 q = A.xy[nb[k]];
-    px[k] = q.x;
-    py[k] = q.y;
-  }
-  if (cnt < M) {
-#pragma unroll
-    for (int k = 0; k < M; k++) {
-      if (k >= cnt) {  // far away and far from each other: zero correlation
-        px[k] = 1e100 * (k + 1);
-        py[k] = -1e100 * (k + 1);
+        for (int k = 0; k < M; k++)
+          if (k >= cnt) nb[k] = 0;
       }
+      double px[M], py[M];
+#pragma unroll
+      for (int k = 0; k < M; k++) {
+        const double2 q = A.xy[nb[k]];
+        px[k] = q.x;
+        py[k] = q.y;
+      }
+      if (cnt < M) {
+#pragma unroll
+        for (int k = 0; k < M; k++) {
+          if (k >= cnt) {  // far away and far apart: zero correlation
+            px[k] = 1e100 * (k + 1);
+            py[k] = -1e100 * (k + 1);
+          }
+        }
+      }
+      const double2 s = A.xy[i];
+
+      constexpr int T = M * (M + 1) / 2;
+      double a[T];  // packed lower triangle
+      double v[M];
+#pragma unroll
+      for (int k = 0; k < M; k++) {
+        v[k] = s2_corr<COV>(sqdist(s.x, s.y, px[k], py[k]), c);
+#pragma unroll
+        for (int l = 0; l < k; l++)
+          a[tri(k, l)] = s2_corr<COV>(sqdist(px[k], py[k], px[l], py[l]), c);
+        a[tri(k, k)] = c.diag;
+      }
+      double vv;
+      const bool ok = chol_solve<M>(a, v, vv);  // v <- B_i = C^-1 c
+      if (A.B) {
+#pragma unroll
+        for (int k = 0; k < M; k++)
+          if (k < cnt) A.B[off + k] = v[k];
+      }
+      Fi = c.diag - vv;
+      // residual vectors; padded coefficients are exactly 0
+#pragma unroll
+      for (int j = 0; j < kMaxNZ; j++) {
+        if (j < A.nz) {
+          double ej = A.Z[(size_t)i * A.zs + j];
+#pragma unroll
+          for (int k = 0; k < M; k++)
+            ej = fma(-v[k], A.Z[(size_t)nb[k] * A.zs + j], ej);
+          e[j] = ej;
+        }
+      }
+      if (!ok) atomicMin(A.err, i);
+    }
+    A.F[i] = Fi;
+    if (A.E) {
+#pragma unroll
+      for (int j = 0; j < kMaxNZ; j++)
+        if (j < A.nz) A.E[(size_t)j * A.n + i] = e[j];
     }
   }
-  const double2 s = A.xy[i];
-
-  constexpr int T = M * (M + 1) / 2;
-  double a[T];  // packed lower triangle, a[k(k+1)/2 + l], l <= k
-  double v[M];
-#pragma unroll
-  for (int k = 0; k < M; k++) {
-    v[k] = s2_corr<COV>(sqdist(s.x, s.y, px[k], py[k]), c);
-#pragma unroll
-    for (int l = 0; l < k; l++)
-      a[k * (k + 1) / 2 + l] =
-          s2_corr<COV>(sqdist(px[k], py[k], px[l], py[l]), c);
-    a[k * (k + 1) / 2 + k] = c.diag;
-  }
-
-  double vv;
-  const bool ok = chol_solve<M>(a, v, vv);  // v <- B_i = C^-1 c
-
-  if (A.B) {
-#pragma unroll
-    for (int k = 0; k < M; k++)
-      if (k < cnt) A.B[off + k] = v[k];
-  }
-  A.F[i] = c.diag - vv;
-  if (A.E) {
-    // padded coefficients are exactly 0, so their (valid) rows add nothing
-    for (int j = 0; j < A.nz; j++) {
-      double e = A.Z[(size_t)i * A.zs + j];
-#pragma unroll
-      for (int k = 0; k < M; k++)
-        e = fma(-v[k], A.Z[(size_t)nb[k] * A.zs + j], e);
-      A.E[(size_t)j * A.n + i] = e;
-    }
-  }
-  if (!ok) atomicMin(A.err, i);
+  if (A.rows) block_site_terms<kLaneBlock>(A, valid, Fi, e, b);
 }
 
 // ---------------------------------------------------------------------------
@@ -170,7 +233,12 @@ __global__ __launch_bounds__(64 * kCoopWaves) void k_bf_coop(BfArgs A) {
   const CovArgs &c = A.c;
   const int cnt = A.lu[A.n + i];
   if (cnt == 0) {
-    if (t == 0) site_without_neighbors(A, i);
+    if (t == 0) {
+      A.F[i] = c.diag;
+      if (A.E)
+        for (int j = 0; j < A.nz; j++)
+          A.E[(size_t)j * A.n + i] = A.Z[(size_t)i * A.zs + j];
+    }
     return;
   }
   const int off = A.lu[i];
@@ -222,7 +290,12 @@ __global__ __launch_bounds__(64 * kCoopWaves) void k_bf_coop(BfArgs A) {
     wave_sync();
   }
   if (!ok) {
-    if (t == 0) atomicMin(A.err, i);
+    if (t == 0) {
+      atomicMin(A.err, i);
+      A.F[i] = 1.0;  // keeps the sums finite; the pass reports the failure
+      if (A.E)
+        for (int j = 0; j < A.nz; j++) A.E[(size_t)j * A.n + i] = 0.0;
+    }
     return;
   }
   // forward solve: lane t holds x_t
@@ -232,8 +305,7 @@ __global__ __launch_bounds__(64 * kCoopWaves) void k_bf_coop(BfArgs A) {
     if (t == k) x = vk;
     if (t > k && t < cnt) x = fma(-Am[t * ld + k], vk, x);
   }
-  double vv = t < cnt ? x * x : 0.0;
-  for (int o = 32; o > 0; o >>= 1) vv += __shfl_xor(vv, o);
+  const double vv = wave_sum(t < cnt ? x * x : 0.0);
   // back solve B = L^-T v
   for (int k = cnt - 1; k >= 0; k--) {
     const double bk = readlane_d(x, k) * Am[k * ld + k];
@@ -244,8 +316,8 @@ __global__ __launch_bounds__(64 * kCoopWaves) void k_bf_coop(BfArgs A) {
   if (t == 0) A.F[i] = c.diag - vv;
   if (A.E) {
     for (int j = 0; j < A.nz; j++) {
-      double part = t < cnt ? x * A.Z[(size_t)myid * A.zs + j] : 0.0;
-      for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+      const double part =
+          wave_sum(t < cnt ? x * A.Z[(size_t)myid * A.zs + j] : 0.0);
       if (t == 0)
         A.E[(size_t)j * A.n + i] = A.Z[(size_t)i * A.zs + j] - part;
     }
@@ -253,7 +325,113 @@ __global__ __launch_bounds__(64 * kCoopWaves) void k_bf_coop(BfArgs A) {
 }
 
 // ---------------------------------------------------------------------------
-// dispatch
+// sums
+// ---------------------------------------------------------------------------
+// out[k] = sum over rows r < nrows of rows[k*cap + r] (one block per column
+// k, fixed per-thread order, fixed butterfly); block 0 also hands over the
+// updateBF failure flag: *err_out = *err, *err = INT_MAX.
+__global__ __launch_bounds__(kRedThreads) void k_rows_sum(
+    const double *rows, int cap, int nrows, double *out, int *err,
+    double *err_out) {
+  __shared__ double red[kRedThreads / 64];
+  const int k = (int)blockIdx.x;
+  const int w = (int)threadIdx.x >> 6, t = (int)threadIdx.x & 63;
+  double s = 0.0;
+  const double *col = rows + (size_t)k * cap;
+  for (int r = (int)threadIdx.x; r < nrows; r += kRedThreads) s += col[r];
+  s = wave_sum(s);
+  if (t == 0) red[w] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = 0.0;
+#pragma unroll
+    for (int ww = 0; ww < kRedThreads / 64; ww++) tot += red[ww];
+    out[k] = tot;
+    if (k == 0 && err) {
+      *err_out = (double)*err;
+      *err = INT_MAX;
+    }
+  }
+}
+
+static int rows_sum_launch(const double *rows, int cap, int nrows, int nv,
+                           double *out, int *err, double *err_out,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(k_rows_sum, dim3(nv), dim3(kRedThreads), 0, st, rows,
+                     cap, nrows, out, err, err_out);
+  return (int)hipGetLastError();
+}
+
+static int red_grid(int n) {
+  const int b = (n + kRedThreads - 1) / kRedThreads;
+  return b < 1 ? 1 : (b > kRedBlocks ? kRedBlocks : b);
+}
+
+template <int NZ>
+__global__ __launch_bounds__(kRedThreads) void k_stats(StatsArgs A) {
+  constexpr int NV = 2 + NZ * (NZ + 1) / 2;
+  __shared__ double red[kRedThreads / 64][NV];
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) acc[v] = 0.0;
+  const int stride = (int)gridDim.x * kRedThreads;
+  for (int i = blockIdx.x * kRedThreads + threadIdx.x; i < A.n; i += stride) {
+    const double f = A.F[i];
+    acc[0] += log(f);
+    if constexpr (NZ > 0) {
+      const double rf = 1.0 / f;
+      double e[NZ];
+#pragma unroll
+      for (int j = 0; j < NZ; j++) e[j] = A.E[(size_t)j * A.n + i];
+      int g = 2;
+#pragma unroll
+      for (int p = 0; p < NZ; p++)
+#pragma unroll
+        for (int q = 0; q <= p; q++) acc[g++] += e[p] * e[q] * rf;
+    }
+  }
+  const int w = (int)threadIdx.x >> 6, t = (int)threadIdx.x & 63;
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    const double x = wave_sum(acc[v]);
+    if (t == 0) red[w][v] = x;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < NV) {
+    double s = 0.0;
+#pragma unroll
+    for (int ww = 0; ww < kRedThreads / 64; ww++) s += red[ww][threadIdx.x];
+    A.rows[(size_t)threadIdx.x * A.rows_cap + blockIdx.x] = s;
+  }
+}
+
+int stats_launch(const StatsArgs &a, hipStream_t st) {
+  const int nb = red_grid(a.n);
+  const int nz = a.E ? a.nz : 0;
+  switch (nz) {
+#define NNGP_STATS_CASE(K)                                                  \
+  case K:                                                                  \
+    hipLaunchKernelGGL(k_stats<K>, dim3(nb), dim3(kRedThreads), 0, st, a); \
+    break;
+    NNGP_STATS_CASE(0)
+    NNGP_STATS_CASE(2)
+    NNGP_STATS_CASE(3)
+    NNGP_STATS_CASE(4)
+    NNGP_STATS_CASE(5)
+    NNGP_STATS_CASE(6)
+    NNGP_STATS_CASE(7)
+    NNGP_STATS_CASE(8)
+    NNGP_STATS_CASE(9)
+#undef NNGP_STATS_CASE
+    default: return -1;
+  }
+  const int nv = 2 + nz * (nz + 1) / 2;
+  return rows_sum_launch(a.rows, a.rows_cap, nb, nv, a.out, a.err,
+                         a.err_out, st);
+}
+
+// ---------------------------------------------------------------------------
+// dispatch of one pass
 // ---------------------------------------------------------------------------
 template <int M>
 static void launch_lane_m(const BfArgs &a, int cov, hipStream_t st) {
@@ -266,10 +444,12 @@ static void launch_lane_m(const BfArgs &a, int cov, hipStream_t st) {
   }
 }
 
+bool bf_fuses_sums(int m, int cov) { return cov != 2 && m <= 16; }
+
 int bf_launch(const BfArgs &a, hipStream_t st) {
   if (a.n <= 0) return 0;
   const int cov = a.c.cov;
-  if (cov != 2 && a.m <= 16) {
+  if (bf_fuses_sums(a.m, cov)) {
     if (a.m <= 4) launch_lane_m<4>(a, cov, st);
     else if (a.m <= 8) launch_lane_m<8>(a, cov, st);
     else if (a.m <= 10) launch_lane_m<10>(a, cov, st);
@@ -287,134 +467,20 @@ int bf_launch(const BfArgs &a, hipStream_t st) {
   return (int)hipGetLastError();
 }
 
+int bf_sums_launch(const BfArgs &a, double *out, double *err_out,
+                   hipStream_t st) {
+  const int nv = 2 + a.nz * (a.nz + 1) / 2;
+  const int nrows = (a.n + kLaneBlock - 1) / kLaneBlock;
+  return rows_sum_launch(a.rows, a.rows_cap, nrows, nv, out, a.err, err_out,
+                         st);
+}
+
 // ---------------------------------------------------------------------------
-// deterministic sums.  A fixed grid of kRedBlocks blocks; every thread adds
-// a fixed set of sites in a fixed order, each block reduces its threads with
-// a fixed butterfly per wavefront and the wavefronts in order, and writes one
-// partial row.  The last block to finish (found with an atomic counter, which
-// only decides who adds, not in what order) adds the partial rows: thread b
-// holds row b, the same butterfly, the waves in order.  Results are
-// bit-reproducible from run to run.
+// plain Q(u, v)
 // ---------------------------------------------------------------------------
-template <int NV>
-__device__ inline void block_reduce(double (&acc)[NV], double *dst) {
-  __shared__ double red[kRedThreads / 64][NV];
-  const int w = (int)threadIdx.x >> 6, t = (int)threadIdx.x & 63;
-#pragma unroll
-  for (int v = 0; v < NV; v++) {
-    double x = acc[v];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-    if (t == 0) red[w][v] = x;
-  }
-  __syncthreads();
-  if ((int)threadIdx.x < NV) {
-    double s = 0.0;
-#pragma unroll
-    for (int k = 0; k < kRedThreads / 64; k++) s += red[k][threadIdx.x];
-    dst[threadIdx.x] = s;
-  }
-  __syncthreads();
-}
-
-// Called by every block after its partial row is stored.
-template <int NV>
-__device__ inline void last_block_sum(const double *partials,
-                                      unsigned int *counter, double *out,
-                                      int *err, double *err_out) {
-  __shared__ bool is_last;
-  __threadfence();
-  if (threadIdx.x == 0)
-    is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  double acc[NV];
-  const int b = (int)threadIdx.x;
-#pragma unroll
-  for (int v = 0; v < NV; v++)
-    acc[v] = b < (int)gridDim.x ? partials[(size_t)b * kStatsMax + v] : 0.0;
-  block_reduce<NV>(acc, out);
-  if (threadIdx.x == 0) {
-    *counter = 0u;  // ready for the next launch
-    if (err) {
-      *err_out = (double)*err;
-      *err = INT_MAX;
-    }
-  }
-}
-
-int reduce_blocks() { return kRedBlocks; }
-
-template <int NZ>
-__global__ __launch_bounds__(kRedThreads) void k_stats(StatsArgs A,
-                                                       double *partials,
-                                                       unsigned int *counter,
-                                                       double *out) {
-  constexpr int NG = NZ * (NZ + 1) / 2;
-  constexpr int NV = 2 + NG;
-  double acc[NV];
-#pragma unroll
-  for (int v = 0; v < NV; v++) acc[v] = 0.0;
-  const int stride = (int)gridDim.x * kRedThreads;
-#pragma unroll 4
-  for (int i = blockIdx.x * kRedThreads + threadIdx.x; i < A.n; i += stride) {
-    const double f = A.F[i];
-    acc[0] += log(f);
-    if constexpr (NZ > 0) {
-      double e[NZ];
-#pragma unroll
-      for (int j = 0; j < NZ; j++) e[j] = A.E[(size_t)j * A.n + i];
-      if (A.want_q) {
-        double er = -e[NZ - 1];
-#pragma unroll
-        for (int j = 0; j < NZ - 1; j++) er = fma(A.beta[j], e[j], er);
-        acc[1] += er * er / f;
-      }
-      int g = 2;
-#pragma unroll
-      for (int p = 0; p < NZ; p++)
-#pragma unroll
-        for (int q = 0; q <= p; q++) acc[g++] += e[p] * e[q] / f;
-    }
-  }
-  block_reduce<NV>(acc, partials + (size_t)blockIdx.x * kStatsMax);
-  last_block_sum<NV>(partials, counter, out, A.err, A.err_out);
-}
-
-static int red_grid(int n) {
-  const int b = (n + kRedThreads - 1) / kRedThreads;
-  return b < 1 ? 1 : (b > kRedBlocks ? kRedBlocks : b);
-}
-
-int stats_launch(const StatsArgs &a, double *partials, unsigned int *counter,
-                 double *out, hipStream_t st) {
-  const int nb = red_grid(a.n);
-  const int nz = a.E ? a.nz : 0;
-  switch (nz) {
-#define NNGP_STATS_CASE(K)                                                 \
-  case K:                                                                 \
-    hipLaunchKernelGGL(k_stats<K>, dim3(nb), dim3(kRedThreads), 0, st, a, \
-                       partials, counter, out);                           \
-    break;
-    NNGP_STATS_CASE(0)
-    NNGP_STATS_CASE(2)
-    NNGP_STATS_CASE(3)
-    NNGP_STATS_CASE(4)
-    NNGP_STATS_CASE(5)
-    NNGP_STATS_CASE(6)
-    NNGP_STATS_CASE(7)
-    NNGP_STATS_CASE(8)
-    NNGP_STATS_CASE(9)
-#undef NNGP_STATS_CASE
-    default: return -1;
-  }
-  return (int)hipGetLastError();
-}
-
-__global__ __launch_bounds__(kRedThreads) void k_quadform(
-    QArgs A, double *partials, unsigned int *counter, double *out) {
-  double acc[1] = {0.0};
+__global__ __launch_bounds__(kRedThreads) void k_quadform(QArgs A) {
+  __shared__ double red[kRedThreads / 64];
+  double acc = 0.0;
   const int stride = (int)gridDim.x * kRedThreads;
   for (int i = blockIdx.x * kRedThreads + threadIdx.x; i < A.n; i += stride) {
     const int cnt = A.lu[A.n + i], off = A.lu[i];
@@ -425,17 +491,25 @@ __global__ __launch_bounds__(kRedThreads) void k_quadform(
       a += bj * A.u[id];
       b += bj * A.v[id];
     }
-    acc[0] += (A.u[i] - a) * (A.v[i] - b) / A.F[i];
+    acc += (A.u[i] - a) * (A.v[i] - b) / A.F[i];
   }
-  block_reduce<1>(acc, partials + (size_t)blockIdx.x * kStatsMax);
-  last_block_sum<1>(partials, counter, out, nullptr, nullptr);
+  const int w = (int)threadIdx.x >> 6, t = (int)threadIdx.x & 63;
+  const double x = wave_sum(acc);
+  if (t == 0) red[w] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+#pragma unroll
+    for (int ww = 0; ww < kRedThreads / 64; ww++) s += red[ww];
+    A.rows[blockIdx.x] = s;
+  }
 }
 
-int q_launch(const QArgs &a, double *partials, unsigned int *counter,
-             double *out, hipStream_t st) {
-  hipLaunchKernelGGL(k_quadform, dim3(red_grid(a.n)), dim3(kRedThreads), 0,
-                     st, a, partials, counter, out);
-  return (int)hipGetLastError();
+int q_launch(const QArgs &a, hipStream_t st) {
+  const int nb = red_grid(a.n);
+  hipLaunchKernelGGL(k_quadform, dim3(nb), dim3(kRedThreads), 0, st, a);
+  return rows_sum_launch(a.rows, kRedBlocks, nb, 1, a.out, nullptr, nullptr,
+                         st);
 }
 
 __global__ void k_interleave(const double *cm, int n, double2 *xy) {

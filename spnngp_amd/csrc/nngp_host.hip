@@ -308,14 +308,12 @@ int Model::create(int n_, int m_, int p_, const double *coords,
                             hipMemcpyHostToDevice, st.s));
     NNGP_HIP(hipStreamSynchronize(st.s));
   }
+  fused = bf_fuses_sums(m, cov);
   for (int s = 0; s < 2; s++) {
     NNGP_TRY(F[s].alloc(sizeof(double) * (size_t)n));
-    if (nz) NNGP_TRY(E[s].alloc(sizeof(double) * (size_t)nz * n));
+    if (nz && !fused) NNGP_TRY(E[s].alloc(sizeof(double) * (size_t)nz * n));
   }
-  NNGP_TRY(partials.alloc(sizeof(double) * (size_t)reduce_blocks() *
-                          kStatsMax));
-  NNGP_TRY(counter.alloc(sizeof(unsigned int)));
-  NNGP_HIP(hipMemsetAsync(counter.p, 0, sizeof(unsigned int), st.s));
+  NNGP_TRY(rows.alloc(sizeof(double) * (size_t)rows_capacity(n) * kStatsMax));
   NNGP_TRY(sums.alloc(sizeof(double) * kSumsLen));
   NNGP_TRY(err.alloc(sizeof(int)));
   if (fill_int_launch(err.as<int>(), INT_MAX, st.s))
@@ -339,9 +337,12 @@ int Model::bf(const CovArgs &c, int slot, bool storeB) {
   a.Z = nz ? Z.as<double>() : nullptr;
   a.nz = nz;
   a.zs = zs;
-  a.E = nz ? E[slot].as<double>() : nullptr;
-  a.err = err.as<int>();  // INT_MAX here: every stats pass resets it
+  a.E = (nz && !fused) ? E[slot].as<double>() : nullptr;
+  a.err = err.as<int>();  // INT_MAX here: every sums pass resets it
+  a.rows = fused ? rows.as<double>() : nullptr;
+  a.rows_cap = rows_capacity(n);
   a.c = c;
+  last_bf = a;
   const int ev = prof ? ev_begin(0) : -1;
   const int rc = bf_launch(a, st.s);
   if (ev >= 0) ev_end(ev);
@@ -352,25 +353,46 @@ int Model::bf(const CovArgs &c, int slot, bool storeB) {
   return NNGP_OK;
 }
 
-int Model::stats(int slot, const double *beta, int off) {
-  StatsArgs a;
-  std::memset(&a, 0, sizeof a);
-  a.E = nz ? E[slot].as<double>() : nullptr;
-  a.F = F[slot].as<double>();
-  a.n = n;
-  a.nz = nz;
-  a.want_q = beta != nullptr && nz > 0;
-  if (beta)
-    for (int j = 0; j < p && j < kMaxP; j++) a.beta[j] = beta[j];
-  a.err = err.as<int>();
-  a.err_out = sums.as<double>() + 2 * kStatsMax + (off ? 1 : 0);
+int Model::sums_launch(int slot, int off) {
+  double *out = sums.as<double>() + off;
+  double *err_out = sums.as<double>() + 2 * kStatsMax + (off ? 1 : 0);
   const int ev = prof ? ev_begin(1) : -1;
-  const int rc = stats_launch(a, partials.as<double>(),
-                              counter.as<unsigned int>(),
-                              sums.as<double>() + off, st.s);
+  int rc;
+  if (fused) {
+    rc = bf_sums_launch(last_bf, out, err_out, st.s);
+  } else {
+    StatsArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.E = nz ? E[slot].as<double>() : nullptr;
+    a.F = F[slot].as<double>();
+    a.n = n;
+    a.nz = nz;
+    a.rows = rows.as<double>();
+    a.rows_cap = rows_capacity(n);
+    a.out = out;
+    a.err = err.as<int>();
+    a.err_out = err_out;
+    rc = stats_launch(a, st.s);
+  }
   if (ev >= 0) ev_end(ev);
   if (rc) return set_err(NNGP_ERR_HIP, "sum kernel launch failed");
   return NNGP_OK;
+}
+
+double gram_at(const double *raw, int a, int b) {
+  if (a < b) std::swap(a, b);
+  return raw[2 + a * (a + 1) / 2 + b];
+}
+
+double quad_resid(const PassOut &po, const double *beta, int p) {
+  // sum_i (beta . e_X(i) - e_y(i))^2 / F_i from the cross-products
+  double q = gram_at(po.gram, p, p);
+  for (int a = 0; a < p; a++) {
+    double ga = -2.0 * gram_at(po.gram, p, a);
+    for (int b = 0; b < p; b++) ga += beta[b] * gram_at(po.gram, a, b);
+    q += beta[a] * ga;
+  }
+  return q;
 }
 
 int Model::fetch_sums() {
@@ -434,13 +456,13 @@ void Model::unpack(const double *raw, PassOut *out) const {
   for (int k = 0; k < kStatsMax; k++) out->gram[k] = raw[k];
 }
 
-int Model::pass(const CovArgs &c, int slot, const double *beta, bool storeB,
-                PassOut *out) {
+int Model::pass(const CovArgs &c, int slot, bool storeB, PassOut *out) {
   NNGP_TRY(bf(c, slot, storeB));
-  NNGP_TRY(stats(slot, beta, 0));
+  NNGP_TRY(sums_launch(slot, 0));
   NNGP_TRY(fetch_sums());
   NNGP_TRY(check_err("updateBF", 0));
-  if (out) unpack(hsums.as<double>(), out);
+  unpack(hsums.as<double>(), &slot_out[slot]);
+  if (out) *out = slot_out[slot];
   return NNGP_OK;
 }
 

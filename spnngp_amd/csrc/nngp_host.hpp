@@ -170,8 +170,14 @@ constexpr int kSumsLen = 2 * kStatsMax + 2;
 
 struct PassOut {
   double logDet = 0.0, q = 0.0;
-  double gram[kStatsMax];  // lower triangle of E^T F^-1 E, row-packed
+  double gram[kStatsMax];  // the raw sums: [0] logDet, then from [2] the
+                           // lower triangle of E^T F^-1 E, row-packed
 };
+
+// X^T V^-1 X entry (a, b), X^T V^-1 y (a, p) or y^T V^-1 y (p, p) of a pass
+double gram_at(const double *raw, int a, int b);
+// Q(r, r) for r = X beta - y: beta' G beta - 2 beta' h + y' V^-1 y
+double quad_resid(const PassOut &po, const double *beta, int p);
 
 struct Model {
   int n = 0, m = 0, p = 0, nz = 0, zs = 0, cov = 0, precision = 0;
@@ -181,19 +187,22 @@ struct Model {
   DevBuf B[2], F[2], E[2];
   // sums: [0, kStatsMax) pass at off 0, [kStatsMax, 2 kStatsMax) pass at
   // off kStatsMax, then the updateBF failure flag of each (as doubles)
-  DevBuf partials, counter, sums, err;
+  DevBuf rows, sums, err;
   PinnedBuf hsums;  // host mirror of sums
+  bool fused = true;   // the updateBF kernel adds the site terms itself
+  BfArgs last_bf;      // arguments of the last bf() launch
+  PassOut slot_out[2]; // sums of the last completed pass into each slot
 
   int create(int n, int m, int p, const double *coords, const int *nnIndx,
              const int *nnIndxLU, const double *X, const double *y, int cov,
              const nngp_opts *opts);
-  // B/F/E of theta into slot (B stored only when storeB)
+  // B/F (and E when the sums are not fused) of theta into slot; B stored
+  // only when storeB
   int bf(const CovArgs &c, int slot, bool storeB);
-  // launch the stats pass over slot into sums[off .. off + kStatsMax)
-  int stats(int slot, const double *beta, int off);
-  // one full pass: bf + stats, synchronized, results in out
-  int pass(const CovArgs &c, int slot, const double *beta, bool storeB,
-           PassOut *out);
+  // sums of the last bf() into sums[off .. off + kStatsMax)
+  int sums_launch(int slot, int off);
+  // one full pass: bf + sums, synchronized; out (and slot_out[slot])
+  int pass(const CovArgs &c, int slot, bool storeB, PassOut *out);
   // copy sums (and the failure flags) to hsums and synchronize
   int fetch_sums();
   // failure flag of the pass whose sums sit at off 0 (which = 0) or at off

@@ -25,18 +25,23 @@ struct BfArgs {
   int zs;
   double *E;             // nz x n (E[j*n + i]) or nullptr
   int *err;              // smallest site whose Cholesky failed (INT_MAX = ok)
+  double *rows;          // partial-row buffer (kStatsMax columns x rows_cap)
+  int rows_cap;          // or nullptr when the kernel does not add sums
   CovArgs c;
 };
 
-// Sums over sites for one slot: sum log F, Q(r, r) with r = X beta - y
-// (via the residual vectors), and the lower triangle of E^T diag(1/F) E.
+// Per-site terms of a pass, summed: column 0 sum_i log F_i, column 1 zero,
+// then the lower triangle (row by row) of sum_i e_i e_i^T / F_i where e_i is
+// the residual vector (X row, y) - B_i (X, y)_N(i).  Quadratic forms of any
+// r = X beta - y follow from these cross-products.
 struct StatsArgs {
   const double *E;       // nz x n, or nullptr (then only sum log F)
   const double *F;
   int n;
   int nz;                // p + 1
-  double beta[kMaxP];
-  int want_q;
+  double *rows;          // partial rows, kStatsMax x rows_cap
+  int rows_cap;
+  double *out;           // sums (device)
   int *err;              // optional: the updateBF failure flag; its value is
   double *err_out;       // copied to *err_out and the flag reset to INT_MAX
 };
@@ -46,17 +51,21 @@ struct QArgs {
   const double *B, *F, *u, *v;
   const int *nnIndx, *lu;
   int n;
+  double *rows;          // kRedBlocks partial rows
+  double *out;
 };
 
+// Rows a partial-row buffer needs for n sites (times kStatsMax columns).
+int rows_capacity(int n);
+// True when bf_launch adds the site terms itself (per-lane kernel); else the
+// pass stores E and stats_launch adds them.
+bool bf_fuses_sums(int m, int cov);
 int bf_launch(const BfArgs &a, hipStream_t st);
-// One launch each: block partials go to `partials` (reduce_blocks() *
-// kStatsMax doubles), the final sums to `out` (device).  `counter` is one
-// unsigned int, zero before the first launch; every launch leaves it zero.
-int reduce_blocks();
-int stats_launch(const StatsArgs &a, double *partials, unsigned int *counter,
-                 double *out, hipStream_t st);
-int q_launch(const QArgs &a, double *partials, unsigned int *counter,
-             double *out, hipStream_t st);
+// Sums of a fused bf_launch (same BfArgs) into out; hands over the flag.
+int bf_sums_launch(const BfArgs &a, double *out, double *err_out,
+                   hipStream_t st);
+int stats_launch(const StatsArgs &a, hipStream_t st);
+int q_launch(const QArgs &a, hipStream_t st);
 
 // *p = v on the stream (no memset node, safe under graph capture).
 int fill_int_launch(int *p, int v, hipStream_t st);
