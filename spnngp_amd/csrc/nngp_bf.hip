@@ -54,6 +54,29 @@ __device__ __forceinline__ double wave_sum(double x) {
   return x;
 }
 
+// DPP lane move of a double (two 32-bit moves); lanes outside ROWMASK get 0.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL,
+                                             ROWMASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL,
+                                             ROWMASK, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes with DPP moves only (no LDS round trips); the
+// total is valid in lane 63.  Fixed order: pairs, quads, half rows, rows,
+// then rows 0+1 and 2+3, then all four.
+__device__ __forceinline__ double wave_sum_to63(double x) {
+  x += dpp_mov<0xB1, 0xF>(x);   // quad_perm [1,0,3,2]
+  x += dpp_mov<0x4E, 0xF>(x);   // quad_perm [2,3,0,1]
+  x += dpp_mov<0x141, 0xF>(x);  // row_half_mirror
+  x += dpp_mov<0x140, 0xF>(x);  // row_mirror
+  x += dpp_mov<0x142, 0xA>(x);  // row_bcast:15 into rows 1 and 3
+  x += dpp_mov<0x143, 0xC>(x);  // row_bcast:31 into rows 2 and 3
+  return x;
+}
+
 int rows_capacity(int n) {
   const int lane_rows = (n + kLaneBlock - 1) / kLaneBlock;
   return lane_rows > kRedBlocks ? lane_rows : kRedBlocks;
@@ -73,16 +96,16 @@ __device__ inline void block_site_terms(const BfArgs &A, bool valid,
   __shared__ double red[NT / 64][kStatsMax];
   const int w = (int)threadIdx.x >> 6, t = (int)threadIdx.x & 63;
   const double rf = valid ? 1.0 / F : 0.0;
-  double x = wave_sum(valid ? log(F) : 0.0);
-  if (t == 0) red[w][0] = x;
+  double x = wave_sum_to63(valid ? log(F) : 0.0);
+  if (t == 63) red[w][0] = x;
   int g = 2;
 #pragma unroll
   for (int p = 0; p < kMaxNZ; p++) {
     if (p < A.nz) {
 #pragma unroll
       for (int q = 0; q <= p; q++) {
-        x = wave_sum(e[p] * e[q] * rf);
-        if (t == 0) red[w][g + q] = x;
+        x = wave_sum_to63(e[p] * e[q] * rf);
+        if (t == 63) red[w][g + q] = x;
       }
     }
     g += p + 1;
