@@ -4,7 +4,7 @@
  * Drop-in boundary for spNNGP 1.0.1's NNGP hot path.  Every entry point takes
  * plain pointers and sizes (no R, no torch types).  Matrices keep the
  * reference's column-major FP64 layouts and int32 index layouts, so an R
- * `.Call` shim is pure pointer pass-through (see INTEGRATION.md).
+ * `.Call` shim is pure pointer pass-through.
  *
  * Conventions shared by all entry points
  *   - n sites, m = n.neighbors, p covariates, m_i = min(i, m).
