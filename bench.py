@@ -86,36 +86,115 @@ def cpu_baseline(c, X, y, idx, m, iters, start):
                       f"{dt:.1f} s"}
 
 
-def kriging_latency(c, X, y, m, nreq, q, S, seed):
-    """Latency of predict requests of q new sites against the fitted data
-    (nn2 + rNNGPPredict with host buffers), S posterior samples."""
+def posterior_samples(S, seed):
+    """S stored posterior draws around the truth: beta ~ N(truth, 0.01^2),
+    log sigma.sq, log tau.sq and logit phi (on [3, 300]) ~ truth +
+    N(0, 0.02^2)."""
+    rng = np.random.default_rng(seed)
+    beta = rng.normal([1.0, 2.0, -1.0], 0.01, size=(S, 3)).T.copy()
+    lphi = np.log((12 - 3) / (300 - 12))
+    th = np.vstack([np.exp(np.log(2.0) + rng.normal(0, 0.02, S)),
+                    np.exp(np.log(0.2) + rng.normal(0, 0.02, S)),
+                    300 - (300 - 3) / (1 + np.exp(lphi
+                                                  + rng.normal(0, 0.02, S)))])
+    return beta, th
+
+
+def kriging_serving(c, X, y, m, seconds, rate, S, seed):
+    """Kriging server under load: requests of q ~ floor(exp(U[0, ln 1e5]))
+    new sites, Poisson arrivals at `rate` per second for `seconds`, against
+    the resident fitted model with S posterior samples.  Latency is result
+    on the host minus the scheduled arrival time (so a late client counts
+    against the server)."""
+    import threading
     import spnngp_amd as nn
     rng = np.random.default_rng(seed)
     n, p = X.shape
-    beta = rng.normal([1.0, 2.0, -1.0], 0.01, size=(S, p)).T.copy()
-    th = np.vstack([np.exp(np.log(2.0) + rng.normal(0, 0.02, S)),
-                    np.exp(np.log(0.2) + rng.normal(0, 0.02, S)),
-                    3 + 297 / (1 + np.exp(-(np.log((12 - 3) / (300 - 12))
-                                            + rng.normal(0, 0.02, S))))])
-    z = rng.normal(size=q * S)
-    lat = []
-    for r in range(nreq + 1):
-        c0 = rng.uniform(size=(q, 2))
-        X0 = np.column_stack([np.ones(q), rng.normal(size=(q, 2))])
-        t0 = time.perf_counter()
-        nn0, _ = nn.nn2(c, c0, m)
-        nn.rNNGPPredict(X, y, c, n, p, m, X0, c0, q, nn0, beta, th, S,
-                        "exponential", z=z)
-        dt = time.perf_counter() - t0
-        if r > 0:  # first request warms up
-            lat.append(dt * 1e3)
-    lat = np.array(lat)
-    return {"q": q, "samples": S, "requests": nreq,
+    beta, th = posterior_samples(S, seed)
+    POOL = 100_000
+    pool_c = rng.uniform(size=(2, POOL))
+    pool_X = np.vstack([np.ones(POOL), rng.normal(size=(p - 1, POOL))])
+    t0 = time.perf_counter()
+    srv = nn.Server(X, y, c, m, beta, th, "exponential",
+                    device_bytes=16 << 30, host_bytes=6 << 30,
+                    batch_sites=16384, depth=3)
+    setup_s = time.perf_counter() - t0
+
+    def req(q, off):
+        return (np.ascontiguousarray(pool_X[:, off:off + q].T),
+                np.ascontiguousarray(pool_c[:, off:off + q].T))
+
+    # unloaded latency of a single 10k-site request (same delivery as under
+    # load: result in pinned host memory)
+    alone = []
+    for r in range(11):
+        X0, c0 = req(10_000, 37 * r)
+        t = srv.submit(X0, c0, 10_000_000 + r)
+        _, info = srv.wait(t, timeout=60.0, copy=False)
+        srv.release(t)
+        if r:
+            alone.append((info["t_done"] - info["t_submit"]) * 1e3)
+
+    nreq = rng.poisson(rate * seconds)
+    arrivals = np.sort(rng.uniform(0, seconds, nreq))
+    sizes = np.floor(np.exp(rng.uniform(0, np.log(1e5), nreq))).astype(int)
+    sizes = np.clip(sizes, 1, POOL)
+    offs = rng.integers(0, POOL - sizes + 1)
+    tickets = [None] * nreq
+    done = {}
+    late = [0.0]
+    base = time.monotonic() + 0.2
+
+    def producer():
+        for k in range(nreq):
+            X0, c0 = req(int(sizes[k]), int(offs[k]))
+            dt = base + arrivals[k] - time.monotonic()
+            if dt > 0:
+                time.sleep(dt)
+            else:
+                late[0] = max(late[0], -dt)
+            tickets[k] = srv.submit(X0, c0, k)
+
+    def consumer():
+        for _ in range(nreq):
+            t, info = srv.wait(-1, timeout=120.0, copy=False)
+            done[t] = info
+            srv.release(t)
+
+    th_p = threading.Thread(target=producer)
+    th_c = threading.Thread(target=consumer)
+    wall0 = time.monotonic()
+    th_p.start()
+    th_c.start()
+    th_p.join()
+    th_c.join()
+    wall = time.monotonic() - wall0
+    st = srv.stats()
+    srv.close()
+    lat = np.empty(nreq)
+    ok = 0
+    for k in range(nreq):
+        info = done[tickets[k]]
+        ok += info["status"] == 0
+        lat[k] = (info["t_done"] - (base + arrivals[k])) * 1e3
+    mid = (sizes >= 5_000) & (sizes <= 20_000)
+    return {"requests": int(nreq), "ok": int(ok), "rate_per_s": rate,
+            "seconds": seconds, "samples": S,
+            "q": "floor(exp(U[0, ln 1e5])), mean %.0f" % sizes.mean(),
             "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99)),
-            "site_samples_per_s": float(q * S / (np.median(lat) * 1e-3)),
-            "note": "sequential requests, host buffers in and out "
-                    "(nn2 + upload + kriging + download); no batching yet"}
+            "p50_ms_q5e3_2e4": float(np.percentile(lat[mid], 50)),
+            "p99_ms_q5e3_2e4": float(np.percentile(lat[mid], 99)),
+            "site_samples_per_s": float(st["site_samples"] / wall),
+            "launches": int(st["launches"]),
+            "device_bytes_peak": int(st["device_bytes_peak"]),
+            "client_late_ms_max": late[0] * 1e3,
+            "alone_q1e4_p50_ms": float(np.percentile(alone, 50)),
+            "alone_q1e4_p99_ms": float(np.percentile(alone, 99)),
+            "setup_s": setup_s,
+            "note": "resident model (n=1e6, m=15, exponential), nn2 + "
+                    "kriging on the GPU, results delivered to pinned host "
+                    "memory; latency = result on host - scheduled arrival"}
 
 
 def main():
@@ -126,8 +205,9 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--m", type=int, default=15)
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--cpu-iters", type=int, default=6)
-    ap.add_argument("--kriging-requests", type=int, default=10)
+    ap.add_argument("--cpu-iters", type=int, default=40)
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is "
@@ -175,8 +255,8 @@ def main():
     cpu = cpu_baseline(c, X, y, idx, m, args.cpu_iters, start) \
         if args.cpu_iters > 0 else None
     print("[bench] kriging", file=sys.stderr, flush=True)
-    krig = kriging_latency(c, X, y, m, args.kriging_requests, 10_000, 500,
-                           4) if args.kriging_requests > 0 else None
+    krig = kriging_serving(c, X, y, m, args.serve_seconds, args.serve_rate,
+                           500, 4) if args.serve_seconds > 0 else None
 
     line = {
         "metric": METRIC, "value": iters_s, "unit": "iter/s", "n_gpus": 1,

@@ -41,7 +41,8 @@ enum nngp_status {
   NNGP_ERR_HIP = 4,         /* HIP runtime error (no device, launch, copy)  */
   NNGP_ERR_NOMEM = 5,       /* device or host allocation failed             */
   NNGP_ERR_INTERRUPT = 6,   /* the caller's interrupt callback said stop    */
-  NNGP_ERR_UNSUPPORTED = 7  /* configuration the library does not handle    */
+  NNGP_ERR_UNSUPPORTED = 7, /* configuration the library does not handle    */
+  NNGP_ERR_TIMEOUT = 8      /* nngp_server_wait: nothing finished in time   */
 };
 
 enum nngp_precision {
@@ -257,6 +258,88 @@ int nngp_model_rNNGP(nngp_model *model, const double *sigmaSqIG,
  * launches. */
 int nngp_model_profile(nngp_model *model, int enable);
 int nngp_model_timing(nngp_model *model, double *t);
+
+/* ------------------------------------------------------------------------
+ * Kriging server: rNNGPPredict (src/rNNGPPredict.cpp:25-222) together with
+ * its nn2 neighbor query (R/generics.R:642) as a resident service.  The
+ * fitted response model (coordinates, X, y, posterior samples) and its
+ * neighbor-search index stay in HBM; requests of any size are submitted
+ * from any host thread and answered as they finish.
+ *
+ * Continuous batching: a request is admitted as soon as the device arena
+ * (16q + 8pq + 4mq + 8qS bytes) and the pinned host arena (16q + 8pq + 8qS
+ * bytes) have room for it; its sites then share kriging launches with the
+ * sites of every other admitted request (at most batch_sites per launch,
+ * shortest remaining request first, requests older than 0.25 s first), and
+ * several launches and result copies are in flight at once.
+ *
+ * The standard normals of a request are z(i, s) = Philox4x32-10 of counter
+ * (i, s, 0, 0) under key `seed`, turned into a normal by Box-Muller (see
+ * philox_normal in the library sources), so a request's result is the same
+ * whether it runs alone or among other requests.  Its neighbors are the m
+ * nearest fitted sites (nngp_nn2).  The result has the layout of p.y.0:
+ * q x nSamples column-major.
+ *
+ * Timestamps are CLOCK_MONOTONIC seconds.
+ * --------------------------------------------------------------------- */
+
+typedef struct nngp_server nngp_server;
+
+typedef struct nngp_server_opts {
+  unsigned long long device_bytes; /* request arena in HBM (0: 8 GiB)         */
+  unsigned long long host_bytes;   /* pinned host arena (0: 4 GiB)            */
+  int batch_sites;                 /* most new sites per launch (0: 16384)    */
+  int depth;                       /* launches in flight, 1..8 (0: 3)         */
+  int device;                      /* HIP device ordinal                      */
+  int reserved[8];
+} nngp_server_opts;
+
+typedef struct nngp_request_info {
+  int q, nSamples;
+  int status;                 /* NNGP_OK or the request's error           */
+  double t_submit, t_admit, t_done;
+  const double *y0;           /* q x nSamples result in the server's pinned
+                                 arena, valid until nngp_server_release;
+                                 NULL when status != NNGP_OK               */
+} nngp_request_info;
+
+typedef struct nngp_server_stats {
+  long long submitted, completed, launches;
+  int pending, active;        /* waiting for room / admitted, unfinished  */
+  double site_samples;        /* q * nSamples summed over finished requests */
+  unsigned long long device_bytes_used, device_bytes_peak;
+  unsigned long long host_bytes_used, host_bytes_peak;
+} nngp_server_stats;
+
+/* X n x p and coords n x 2 column-major (the fitted rows in any fixed
+ * order), y n, betaSamples p x nSamples, thetaSamples nTheta x nSamples
+ * (the p.beta.samples / p.theta.samples columns of a fit). */
+int nngp_server_create(nngp_server **out, const double *X, const double *y,
+                       const double *coords, int n, int p, int m,
+                       const double *betaSamples, const double *thetaSamples,
+                       int nSamples, int covModel,
+                       const nngp_server_opts *opts);
+/* Cancels requests not yet admitted (status NNGP_ERR_INTERRUPT), lets
+ * admitted ones finish, then frees everything. */
+void nngp_server_destroy(nngp_server *srv);
+/* Queue a request of q new sites (X0 q x p, coords0 q x 2, column-major;
+ * both copied before returning).  Fails with NNGP_ERR_NOMEM when the request
+ * alone exceeds an arena. */
+int nngp_server_submit(nngp_server *srv, int q, const double *X0,
+                       const double *coords0, unsigned long long seed,
+                       long long *ticket);
+/* Block until request *ticket finishes, or, with *ticket == -1, until any
+ * finished request not yet reported (its ticket is stored in *ticket).
+ * timeout_s < 0 waits without limit; on expiry NNGP_ERR_TIMEOUT. */
+int nngp_server_wait(nngp_server *srv, long long *ticket, double timeout_s,
+                     nngp_request_info *info);
+/* Give the finished request's result memory back to the server. */
+int nngp_server_release(nngp_server *srv, long long ticket);
+/* submit + wait + copy the result into pY0 (q x nSamples) + release. */
+int nngp_server_predict(nngp_server *srv, int q, const double *X0,
+                        const double *coords0, unsigned long long seed,
+                        double *pY0);
+int nngp_server_get_stats(nngp_server *srv, nngp_server_stats *st);
 
 #ifdef __cplusplus
 }

@@ -994,3 +994,36 @@ int or_cNNGP(const double *y, const double *X, int p, int n,
   free(tmp_p);
   return st;
 }
+
+/* ------------------------------------------------------------------------
+ * Philox4x32-10 and the kriging server's normals (test infrastructure for
+ * the server; the reference itself draws z from R's RNG)
+ * --------------------------------------------------------------------- */
+void or_philox4x32_10(unsigned int c[4], unsigned int k0, unsigned int k1) {
+  for (int r = 0; r < 10; r++) {
+    unsigned long long lo = (unsigned long long)0xD2511F53u * c[0];
+    unsigned long long hi = (unsigned long long)0xCD9E8D57u * c[2];
+    unsigned int x0 = (unsigned int)(hi >> 32) ^ c[1] ^ k0;
+    unsigned int x1 = (unsigned int)hi;
+    unsigned int x2 = (unsigned int)(lo >> 32) ^ c[3] ^ k1;
+    unsigned int x3 = (unsigned int)lo;
+    c[0] = x0;
+    c[1] = x1;
+    c[2] = x2;
+    c[3] = x3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+double or_philox_normal(unsigned long long seed, unsigned int i,
+                        unsigned int s) {
+  unsigned int c[4] = {i, s, 0u, 0u};
+  or_philox4x32_10(c, (unsigned int)(seed & 0xffffffffu),
+                   (unsigned int)(seed >> 32));
+  unsigned long long a = ((unsigned long long)c[0] << 32) | c[1];
+  unsigned long long b = ((unsigned long long)c[2] << 32) | c[3];
+  double u1 = ((double)(a >> 11) + 0.5) / 9007199254740992.0;
+  double u2 = ((double)(b >> 11) + 0.5) / 9007199254740992.0;
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}

@@ -24,6 +24,7 @@
  *                     recovery; dense-GP likelihood at m >= n-1
  *   or_rNNGPPredict   dense GP kriging when m = n (z = 0 isolates the mean)
  *   or_cNNGP          dense GLS when m >= n-1
+ *   or_philox*        Random123 known-answer vectors; moments and a KS test
  * Parity unpinned (no R in this image): R's RNG stream, the exact bits of
  * Rmath's bessel_k_ex / gammafn, R's LAPACK rounding order, and the order
  * rsort_with_index gives to exactly tied distances.
@@ -135,6 +136,17 @@ int or_cNNGP(const double *y, const double *X, int p, int n,
  * (src/rNNGP.cpp:24-42, src/rNNGPReplicated.cpp:183-199). */
 void or_nngp_draw(const double *B, const double *F, const int *nnIndx,
                   const int *nnIndxLU, int n, const double *z, double *w);
+
+/* Philox4x32-10 (Salmon, Moraes, Dror & Shaw, SC'11) in place on the
+ * counter c[4] under key (k0, k1), and the standard normal the kriging
+ * server draws for site i, sample s of a request with key `seed`
+ * (counter (i, s, 0, 0), two 53-bit uniforms, Box-Muller cosine branch).
+ * Not a reference function: the reference draws z from R's RNG
+ * (src/rNNGPPredict.cpp:164).  Pinned by Random123's known-answer
+ * vectors and by normality of its output. */
+void or_philox4x32_10(unsigned int c[4], unsigned int k0, unsigned int k1);
+double or_philox_normal(unsigned long long seed, unsigned int i,
+                        unsigned int s);
 
 #ifdef __cplusplus
 }

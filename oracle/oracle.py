@@ -82,6 +82,10 @@ def lib():
                                _dp, _dp, C.c_int, _ip, C.c_int, C.c_int, _dp,
                                C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
         L.or_nngp_draw.argtypes = [_dp, _dp, _ip, _ip, C.c_int, _dp, _dp]
+        L.or_philox4x32_10.argtypes = [C.POINTER(C.c_uint), C.c_uint,
+                                       C.c_uint]
+        L.or_philox_normal.restype = C.c_double
+        L.or_philox_normal.argtypes = [C.c_ulonglong, C.c_uint, C.c_uint]
         _lib = L
     return _lib
 
@@ -315,3 +319,20 @@ def cNNGP(y, X, p, n, coords_flat, thetaAlpha, nnIndx, nnIndxLU, X0,
             "bb": bb.reshape(g, p).T.copy(),
             "y.0.hat": y0[:n0 * g].reshape(g, n0).T.copy(),
             "y.0.hat.var": y0v[:n0 * g].reshape(g, n0).T.copy()}
+
+
+def philox4x32_10(ctr, key):
+    """Philox4x32-10 block of counter ctr[4] under key[2] (four uint32)."""
+    c = (C.c_uint * 4)(*ctr)
+    lib().or_philox4x32_10(c, key[0], key[1])
+    return [int(v) for v in c]
+
+
+def philox_normals(seed, q, S):
+    """The kriging server's z for a request: q*S normals, z[i*S + s]."""
+    L = lib()
+    z = np.empty(q * S)
+    for i in range(q):
+        for s in range(S):
+            z[i * S + s] = L.or_philox_normal(seed, i, s)
+    return z

@@ -21,6 +21,7 @@ NNGP_ERR_HIP = 4
 NNGP_ERR_NOMEM = 5
 NNGP_ERR_INTERRUPT = 6
 NNGP_ERR_UNSUPPORTED = 7
+NNGP_ERR_TIMEOUT = 8
 
 NNGP_FP64 = 0
 NNGP_MIXED = 1
@@ -34,6 +35,9 @@ EXPORTS = (
     "nngp_model_create", "nngp_model_destroy", "nngp_model_update",
     "nngp_model_qresid", "nngp_model_get_bf", "nngp_model_rNNGP",
     "nngp_prior_draw", "nngp_model_profile", "nngp_model_timing",
+    "nngp_server_create", "nngp_server_destroy", "nngp_server_submit",
+    "nngp_server_wait", "nngp_server_release", "nngp_server_predict",
+    "nngp_server_get_stats",
 )
 
 _RNG_FN = C.CFUNCTYPE(C.c_double, C.c_void_p)
@@ -55,6 +59,28 @@ class NngpPassOut(C.Structure):
     _fields_ = [("logDet", C.c_double), ("qResid", C.c_double),
                 ("XtViX", C.c_double * 64), ("XtViy", C.c_double * 8),
                 ("ytViy", C.c_double)]
+
+
+class NngpServerOpts(C.Structure):
+    _fields_ = [("device_bytes", C.c_ulonglong), ("host_bytes", C.c_ulonglong),
+                ("batch_sites", C.c_int), ("depth", C.c_int),
+                ("device", C.c_int), ("reserved", C.c_int * 8)]
+
+
+class NngpRequestInfo(C.Structure):
+    _fields_ = [("q", C.c_int), ("nSamples", C.c_int), ("status", C.c_int),
+                ("t_submit", C.c_double), ("t_admit", C.c_double),
+                ("t_done", C.c_double), ("y0", C.POINTER(C.c_double))]
+
+
+class NngpServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_longlong), ("completed", C.c_longlong),
+                ("launches", C.c_longlong), ("pending", C.c_int),
+                ("active", C.c_int), ("site_samples", C.c_double),
+                ("device_bytes_used", C.c_ulonglong),
+                ("device_bytes_peak", C.c_ulonglong),
+                ("host_bytes_used", C.c_ulonglong),
+                ("host_bytes_peak", C.c_ulonglong)]
 
 
 class NngpError(RuntimeError):
@@ -124,6 +150,18 @@ def load():
     L.nngp_prior_draw.argtypes = [_dp, _dp, _ip, _ip, _i, _dp, _dp]
     L.nngp_model_profile.argtypes = [_vp, _i]
     L.nngp_model_timing.argtypes = [_vp, _dp]
+    L.nngp_server_create.argtypes = [C.POINTER(_vp), _dp, _dp, _dp, _i, _i,
+                                     _i, _dp, _dp, _i, _i,
+                                     C.POINTER(NngpServerOpts)]
+    L.nngp_server_destroy.argtypes = [_vp]
+    L.nngp_server_destroy.restype = None
+    L.nngp_server_submit.argtypes = [_vp, _i, _dp, _dp, C.c_ulonglong,
+                                     C.POINTER(C.c_longlong)]
+    L.nngp_server_wait.argtypes = [_vp, C.POINTER(C.c_longlong), _d,
+                                   C.POINTER(NngpRequestInfo)]
+    L.nngp_server_release.argtypes = [_vp, C.c_longlong]
+    L.nngp_server_predict.argtypes = [_vp, _i, _dp, _dp, C.c_ulonglong, _dp]
+    L.nngp_server_get_stats.argtypes = [_vp, C.POINTER(NngpServerStats)]
     _lib = L
     return _lib
 

@@ -62,7 +62,8 @@ def build(verbose: bool = True, jobs: int | None = None) -> str:
                 print(f"[nngp build] compiled {os.path.basename(src)}",
                       flush=True)
     tmp = LIB + ".tmp"
-    cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp,
+    cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread",
+           "-o", tmp,
            *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -232,6 +232,41 @@ __device__ __forceinline__ bool chol_solve(double (&a)[M * (M + 1) / 2],
   return ok;
 }
 
+// Philox4x32-10 (Salmon, Moraes, Dror & Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11): ten rounds of the two 32x32->64 multiplies with
+// the published multipliers and Weyl key increments.
+__host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0,
+                                              uint32_t k1) {
+  for (int r = 0; r < 10; r++) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+    c[1] = (uint32_t)p1;
+    c[3] = (uint32_t)p0;
+    c[0] = n0;
+    c[2] = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+// Standard normal keyed by (seed, site i, sample s): Philox4x32-10 of the
+// counter (i, s, 0, 0) under key seed, two 53-bit uniforms in (0, 1), and
+// the cosine branch of Box-Muller.  A draw depends only on its key and
+// counter, so a kriging request gives the same draws however its sites are
+// split into launches.
+__host__ __device__ inline double philox_normal(uint64_t seed, uint32_t i,
+                                                uint32_t s) {
+  uint32_t c[4] = {i, s, 0u, 0u};
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const uint64_t a = ((uint64_t)c[0] << 32) | c[1];
+  const uint64_t b = ((uint64_t)c[2] << 32) | c[3];
+  const double u1 = ((double)(a >> 11) + 0.5) * 0x1p-53;
+  const double u2 = ((double)(b >> 11) + 0.5) * 0x1p-53;
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs of an MI355X; give each
 // XCD a contiguous range of logical blocks so that neighbor gathers of one
 // XCD stay in its own L2.  grid = 8 * per; returns -1 for padding blocks.

@@ -112,6 +112,19 @@ int nn2_launch(const Nn2Args &a, hipStream_t st);
 // ---------------------------------------------------------------------------
 // kriging
 // ---------------------------------------------------------------------------
+// One run of consecutive sites of one kriging request inside a batched launch
+// (the serving loop): blocks [block0, block0 + count) of the launch handle
+// the request's sites first .. first + count - 1.
+struct PredSeg {
+  const double2 *xy0;    // the request's q coordinates
+  const double *X0;      // q x p column-major
+  const int *nn0;        // q x m column-major
+  double *y0;            // q x S: y0[s*q + i]
+  int *err;              // the request's failure flag
+  unsigned long long seed;  // z(i, s) = philox_normal(seed, i, s)
+  int q, first, block0, count;
+};
+
 struct PredArgs {
   const double2 *xy;     // fitted coordinates (original order)
   const double *X;       // n x p column-major
@@ -132,6 +145,11 @@ struct PredArgs {
   CovArgs unit;          // scale-free exp coefficients (closed-form path)
   const CovArgs *cov_s;  // per-sample parameters, device, S entries
                          // (generic path)
+  // Batched form: when segs != nullptr the launch covers nsite blocks split
+  // into nseg runs (sorted by block0); xy0, X0, nn0, q, z, y0, err and site0
+  // above are then unused and z comes from each run's seed.
+  const PredSeg *segs = nullptr;
+  int nseg = 0;
 };
 int pred_launch(const PredArgs &a, hipStream_t st);
 

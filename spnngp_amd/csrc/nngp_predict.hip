@@ -37,6 +37,56 @@ __device__ __forceinline__ double corr_d(double D, double phi, double ip,
   }
 }
 
+// The new site a block works on, with the arrays of its request.
+struct SiteRef {
+  const double2 *xy0;
+  const double *X0;
+  const int *nn0;
+  double *y0;
+  const double *z;  // nullptr: z from philox_normal(seed, i, s)
+  int *err;
+  unsigned long long seed;
+  int q, i;
+};
+
+__device__ __forceinline__ SiteRef pred_site(const PredArgs &A, int b) {
+  SiteRef r;
+  if (A.segs == nullptr) {
+    r.xy0 = A.xy0;
+    r.X0 = A.X0;
+    r.nn0 = A.nn0;
+    r.y0 = A.y0;
+    r.z = A.z;
+    r.err = A.err;
+    r.seed = 0;
+    r.q = A.q;
+    r.i = A.site0 + b;
+    return r;
+  }
+  int lo = 0, hi = A.nseg - 1;  // last run with block0 <= b
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (A.segs[mid].block0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const PredSeg &g = A.segs[lo];
+  r.xy0 = g.xy0;
+  r.X0 = g.X0;
+  r.nn0 = g.nn0;
+  r.y0 = g.y0;
+  r.z = nullptr;
+  r.err = g.err;
+  r.seed = g.seed;
+  r.q = g.q;
+  r.i = g.first + (b - g.block0);
+  return r;
+}
+
+__device__ __forceinline__ double pred_z(const SiteRef &R, int S, int s) {
+  return R.z ? R.z[(size_t)R.i * S + s]
+             : philox_normal(R.seed, (uint32_t)R.i, (uint32_t)s);
+}
+
 // One block = one site x kPredBlock samples.  Grid: (nsite, ceil(S/256)).
 // M >= m; rows m..M-1 are padding at infinite distance (exactly zero
 // correlation), which leaves the real block's arithmetic unchanged.
@@ -48,13 +98,14 @@ __global__ __launch_bounds__(kPredBlock) void k_pred_lane(PredArgs A) {
   __shared__ double sX[M * kPredMaxP];        // X rows of the neighbors
   __shared__ double sX0[kPredMaxP];
   __shared__ double sPx[M], sPy[M];
-  const int i = A.site0 + (int)blockIdx.x;
+  const SiteRef R = pred_site(A, (int)blockIdx.x);
+  const int i = R.i;
   const int t = (int)threadIdx.x;
   const int m = A.m;
   if (t < M) {
     double px = 0.0, py = 0.0, yy = 0.0;
     if (t < m) {
-      const int id = A.nn0[(size_t)t * A.q + i];
+      const int id = R.nn0[(size_t)t * R.q + i];
       const double2 pt = A.xy[id];
       px = pt.x;
       py = pt.y;
@@ -68,9 +119,9 @@ __global__ __launch_bounds__(kPredBlock) void k_pred_lane(PredArgs A) {
     sPy[t] = py;
     sY[t] = yy;
   }
-  if (t < A.p) sX0[t] = A.X0[(size_t)t * A.q + i];
+  if (t < A.p) sX0[t] = R.X0[(size_t)t * R.q + i];
   __syncthreads();
-  const double2 s0 = A.xy0[i];
+  const double2 s0 = R.xy0[i];
   for (int p = t; p < M * (M - 1) / 2; p += kPredBlock) {
     int k = 1;
     while ((k + 1) * k / 2 <= p) k++;  // p = k(k-1)/2 + l, l < k
@@ -108,9 +159,9 @@ __global__ __launch_bounds__(kPredBlock) void k_pred_lane(PredArgs A) {
     for (int j = 0; j < A.p; j++) xb += sX[k * kPredMaxP + j] * bs[j];
     d += v[k] * (sY[k] - xb);
   }
-  const double zz = A.z[(size_t)i * A.S + s];
-  A.y0[(size_t)s * A.q + i] = x0b + d + sqrt(s2 + t2 - vv) * zz;
-  if (!ok) atomicMin(A.err, i);
+  const double zz = pred_z(R, A.S, s);
+  R.y0[(size_t)s * R.q + i] = x0b + d + sqrt(s2 + t2 - vv) * zz;
+  if (!ok) atomicMin(R.err, i);
 }
 
 // Generic path (any m <= 64, any covariance): one wavefront per
@@ -182,7 +233,8 @@ __global__ __launch_bounds__(64 * kPredCoopWaves) void k_pred_coop(
   double *Am = smem + (size_t)w * (m * ld + 2 * m);
   double *px = Am + m * ld;
   double *py = px + m;
-  const int i = A.site0 + (int)blockIdx.x;
+  const SiteRef R = pred_site(A, (int)blockIdx.x);
+  const int i = R.i;
   const int s = (int)blockIdx.y * ((int)blockDim.x >> 6) + w;
   if (s >= A.S) return;
   const double *th = A.theta + (size_t)s * A.nTheta;
@@ -190,13 +242,13 @@ __global__ __launch_bounds__(64 * kPredCoopWaves) void k_pred_coop(
   const CovArgs &c = A.cov_s[s];
   int id = 0;
   if (t < m) {
-    id = A.nn0[(size_t)t * A.q + i];
+    id = R.nn0[(size_t)t * R.q + i];
     const double2 pt = A.xy[id];
     px[t] = pt.x;
     py[t] = pt.y;
   }
   wave_sync_p();
-  const double2 s0 = A.xy0[i];
+  const double2 s0 = R.xy0[i];
   const int np = m * (m + 1) / 2;
   for (int p = t; p < np; p += 64) {
     int k = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
@@ -212,7 +264,7 @@ __global__ __launch_bounds__(64 * kPredCoopWaves) void k_pred_coop(
   double vv;
   const bool ok = wave_chol_solve(Am, ld, m, t, x, vv);
   if (!ok) {
-    if (t == 0) atomicMin(A.err, i);
+    if (t == 0) atomicMin(R.err, i);
     return;
   }
   const double *bs = A.beta + (size_t)s * A.p;
@@ -225,9 +277,9 @@ __global__ __launch_bounds__(64 * kPredCoopWaves) void k_pred_coop(
   for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
   if (t == 0) {
     double x0b = 0.0;
-    for (int j = 0; j < A.p; j++) x0b += A.X0[(size_t)j * A.q + i] * bs[j];
-    A.y0[(size_t)s * A.q + i] =
-        x0b + part + sqrt(s2 + t2 - vv) * A.z[(size_t)i * A.S + s];
+    for (int j = 0; j < A.p; j++) x0b += R.X0[(size_t)j * R.q + i] * bs[j];
+    R.y0[(size_t)s * R.q + i] =
+        x0b + part + sqrt(s2 + t2 - vv) * pred_z(R, A.S, s);
   }
 }
 

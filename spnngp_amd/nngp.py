@@ -352,3 +352,85 @@ class Model:
         return {"p.beta.samples": beta.reshape(nSamples, p).T.copy(),
                 "p.theta.samples": theta.reshape(nSamples, nTheta).T.copy(),
                 "accept": acc.value}
+
+
+# ---------------------------------------------------------------------------
+# kriging server
+# ---------------------------------------------------------------------------
+class Server:
+    """Resident kriging service (nngp_server_*): the fitted rows, their
+    neighbor-search index and the posterior samples stay in HBM; requests
+    are batched continuously as they arrive.  Thread-safe: submit and wait
+    may be called from different threads."""
+
+    def __init__(self, X, y, coords, m, betaSamples, thetaSamples, covModel,
+                 device_bytes=0, host_bytes=0, batch_sites=0, depth=0,
+                 device=0):
+        X = np.asarray(X, dtype=np.float64)
+        self.n, self.p = X.shape
+        self.m = m
+        self.cov = cov_index(covModel)
+        bs = np.asarray(betaSamples, dtype=np.float64)
+        self.S = bs.shape[1]
+        o = _lib.NngpServerOpts(device_bytes, host_bytes, batch_sites, depth,
+                                device, (C.c_int * 8)())
+        self._h = C.c_void_p()
+        check(load().nngp_server_create(
+            C.byref(self._h), _cm(X), _f64(y), _cm(coords), self.n, self.p,
+            m, _cm(bs), _cm(thetaSamples), self.S, self.cov, C.byref(o)))
+
+    def close(self):
+        if self._h:
+            load().nngp_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, X0, coords0, seed) -> int:
+        X0 = np.asarray(X0, dtype=np.float64)
+        t = C.c_longlong()
+        check(load().nngp_server_submit(self._h, X0.shape[0], _cm(X0),
+                                        _cm(coords0), int(seed), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket=-1, timeout=-1.0, copy=True):
+        """Wait for `ticket` (or any finished request when -1).  Returns
+        (ticket, info dict); info['y0'] is the q x S result (a copy when
+        copy=True, else None) and the ticket still has to be released."""
+        t = C.c_longlong(ticket)
+        info = _lib.NngpRequestInfo()
+        check(load().nngp_server_wait(self._h, C.byref(t), float(timeout),
+                                      C.byref(info)))
+        res = {"q": info.q, "status": info.status, "t_submit": info.t_submit,
+               "t_admit": info.t_admit, "t_done": info.t_done, "y0": None}
+        if copy and info.status == 0:
+            a = np.ctypeslib.as_array(info.y0, shape=(info.nSamples, info.q))
+            res["y0"] = a.T.copy()
+        return t.value, res
+
+    def release(self, ticket):
+        check(load().nngp_server_release(self._h, int(ticket)))
+
+    def predict(self, X0, coords0, seed):
+        """Synchronous request: {'p.y.0': q x S}."""
+        X0 = np.asarray(X0, dtype=np.float64)
+        q = X0.shape[0]
+        out = np.zeros(q * self.S)
+        check(load().nngp_server_predict(self._h, q, _cm(X0), _cm(coords0),
+                                         int(seed), out))
+        return {"p.y.0": out.reshape(self.S, q).T.copy()}
+
+    def stats(self):
+        s = _lib.NngpServerStats()
+        check(load().nngp_server_get_stats(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in s._fields_}

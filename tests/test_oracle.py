@@ -289,3 +289,35 @@ def test_nngp_draw_has_nngp_covariance(oracle):
     w = oracle.nngp_draw(B, F, nn["nnIndx"], nn["nnIndxLU"], z)
     np.testing.assert_allclose(w, np.linalg.solve(IB, np.sqrt(F) * z),
                                rtol=1e-10, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# the kriging server's counter-based normals
+# ---------------------------------------------------------------------------
+def test_philox_known_answers(oracle):
+    # Random123 kat_vectors, philox4x32 with 10 rounds
+    M = 0xFFFFFFFF
+    cases = [
+        ((0, 0, 0, 0), (0, 0),
+         (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+        ((M, M, M, M), (M, M),
+         (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+        ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344),
+         (0xa4093822, 0x299f31d0),
+         (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+    ]
+    for ctr, key, want in cases:
+        assert tuple(oracle.philox4x32_10(ctr, key)) == want
+
+
+def test_philox_normals_are_standard_normal(oracle):
+    import scipy.stats as ss
+    z = oracle.philox_normals(0xDEADBEEF12345678, 200, 100)
+    assert ss.kstest(z, "norm").pvalue > 1e-3
+    assert abs(z.mean()) < 4 / np.sqrt(z.size)
+    assert abs(z.var() - 1) < 6 * np.sqrt(2 / z.size)
+    # neighboring counters and seeds are uncorrelated
+    zz = z.reshape(200, 100)
+    assert abs(np.corrcoef(zz[:, 0], zz[:, 1])[0, 1]) < 0.3
+    z2 = oracle.philox_normals(0xDEADBEEF12345679, 200, 100)
+    assert abs(np.corrcoef(z, z2)[0, 1]) < 0.05
