@@ -197,6 +197,85 @@ def kriging_serving(c, X, y, m, seconds, rate, S, seed):
                     "memory; latency = result on host - scheduled arrival"}
 
 
+def conj_grid_bench(n, m, k, seed, cpu_points):
+    """spConjNNGP's k-fold grid search at scale: n sites ~ U[0,1]^2, X = 3
+    N(0,1) columns (no intercept), beta = (1, 2, -1), w from the NNGP prior
+    (matern nu = 1.5, sigma.sq = 1, phi = 8), tau.sq = 0.1; grid phi in 25
+    log-spaced points of [1, 40] x alpha in 20 log-spaced points of
+    [0.01, 1] (nu = 1.5); k folds of floor(n/k) held-out sites each."""
+    import spnngp_amd as nn
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(size=(n, 2))
+    c = c[np.argsort(c[:, 0], kind="stable")]
+    X = rng.normal(size=(n, 3))
+    idx = nn.mkNNIndxCB(c, m)
+    B, F, _ = nn.updateBF(c, idx["nnIndx"], idx["nnIndxLU"], m,
+                          [1.0, 0.0, 8.0, 1.5], "matern")
+    w = nn.prior_draw(B, F, idx["nnIndx"], idx["nnIndxLU"],
+                      rng.normal(size=n))
+    y = X @ [1.0, 2.0, -1.0] + w + rng.normal(scale=np.sqrt(0.1), size=n)
+    nho = n // k
+    ho = np.array([np.sort(rng.choice(n, nho, replace=False))
+                   for _ in range(k)], dtype=np.int32)
+    phis = np.exp(np.linspace(np.log(1.0), np.log(40.0), 25))
+    alphas = np.exp(np.linspace(np.log(0.01), np.log(1.0), 20))
+    ta = np.array([[ph, al, 1.5] for ph in phis for al in alphas])
+    g = ta.shape[0]
+    t0 = time.perf_counter()
+    cv = nn.CvGrid(y, X, c, m, ho, [2.0, 1.0], "matern")
+    setup_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    first = cv.run(ta)  # autotune + graph capture + run
+    first_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    r = cv.run(ta)
+    wall_s = time.perf_counter() - t0
+    cv.close()
+    best = int(np.argmin(r["scores"][:, 1]))
+    nmod = n - nho
+    # flops: updateConjBF + cross-products per training site, kriging-like
+    # solve + p-vector variance per held-out site (exp and sqrt count 1)
+    f_site = f_alg(m) + 2 * m * 4 + 10
+    f_pred = f_alg(m) + 2 * 3 * m + 3 * 3 + 20
+    flops = k * g * (nmod * f_site + nho * f_pred)
+    res = {"n": n, "m": m, "k_fold": k, "grid": g, "problems": k * g,
+           "site_updates": k * g * nmod, "predictions": k * g * nho,
+           "graph_ms": r["graph_ms"], "run_wall_s": wall_s,
+           "first_run_s": first_s, "setup_s": setup_s,
+           "site_updates_per_s": k * g * nmod / (r["graph_ms"] * 1e-3),
+           "achieved_tflops": flops / (r["graph_ms"] * 1e-3) / 1e12,
+           "bf_config": list(r["bf_config"]),
+           "tune_ms": [[t[0], t[1], t[2]] for t in first["tune"]],
+           "best_point": {"phi": float(ta[best, 0]),
+                          "alpha": float(ta[best, 1]),
+                          "crps": float(r["scores"][best, 1])}}
+    if cpu_points > 0:
+        from oracle import oracle as O
+        keep = np.ones(n, bool)
+        keep[ho[0]] = False
+        cm, Xm, ym = c[keep], X[keep], y[keep]
+        c0, X0 = c[ho[0]], X[ho[0]]
+        nnm = nn.mkNNIndxCB(cm, m)
+        nn0, _ = nn.nn2(cm, c0, m)
+        sub = ta[:cpu_points]
+        nt = cores()
+        t0 = time.perf_counter()
+        O.cNNGP(ym, Xm.T.reshape(-1).copy(), 3, nmod,
+                cm.T.reshape(-1).copy(), sub.reshape(-1).copy(),
+                nnm["nnIndx"], nnm["nnIndxLU"], X0.T.reshape(-1).copy(),
+                c0.T.reshape(-1).copy(), nho, nn0.T.reshape(-1).copy(),
+                cpu_points, m, [2.0, 1.0], 2, nThreads=nt)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {
+            "value": cpu_points * nmod / dt, "unit": "site-updates/s",
+            "cores": nt, "kind": "port",
+            "sample": f"oracle cNNGP, fold 1 x first {cpu_points} grid "
+                      f"points ({nmod} training + {nho} held-out sites), "
+                      f"{dt:.1f} s; the full grid would take "
+                      f"{dt * k * g / cpu_points:.0f} s"}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,6 +287,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=40)
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--grid-n", type=int, default=500_000)
+    ap.add_argument("--grid-cpu-points", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is "
@@ -258,6 +339,10 @@ def main():
     krig = kriging_serving(c, X, y, m, args.serve_seconds, args.serve_rate,
                            500, 4) if args.serve_seconds > 0 else None
 
+    print("[bench] conjugate grid", file=sys.stderr, flush=True)
+    grid = conj_grid_bench(args.grid_n, 15, 5, 3, args.grid_cpu_points) \
+        if args.grid_n > 0 else None
+
     line = {
         "metric": METRIC, "value": iters_s, "unit": "iter/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
@@ -288,6 +373,7 @@ def main():
                              - (tm["bf_ms"] + tm["sum_ms"]) / args.steps},
         "cpu_baseline": cpu,
         "kriging": krig,
+        "conj_grid": grid,
         "mixed": None,
         "knn_s": knn_s,
     }

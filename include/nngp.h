@@ -341,6 +341,52 @@ int nngp_server_predict(nngp_server *srv, int q, const double *X0,
                         double *pY0);
 int nngp_server_get_stats(nngp_server *srv, nngp_server_stats *st);
 
+/* ------------------------------------------------------------------------
+ * Conjugate-model k-fold grid search: the loop of R/spConjNNGP.R:287-357
+ * (one cNNGP call per fold, src/cNNGP.cpp:73-358, then the CRPS / RMSPE
+ * scores of R/util.R:247-255) as one batched, graph-captured run over all
+ * k folds x g grid points.
+ *
+ * nngp_cvgrid_create fixes the folds: hoIndx is nHo x kFold column-major
+ * (fold f's held-out rows at hoIndx[f*nHo ...], 0-based, the R code's
+ * `ho.id - 1`); each fold trains on the other n - nHo rows in their given
+ * (NNGP) order with its own neighbor index (the mkNNIndxCB of
+ * R/spConjNNGP.R:296-300) and predicts its held-out rows from their
+ * n.neighbors nearest training rows (the nn2 of R/spConjNNGP.R:311).
+ *
+ * nngp_cvgrid_run scores g grid points (thetaAlpha nTheta x g, the
+ * {phi, alpha[, nu]} columns cNNGP takes).  scores is g x 2 column-major,
+ * columns rmspe and crps: the k.fold.scores of R/spConjNNGP.R:350 (each
+ * the mean over folds of the per-fold score).  The launch shape of the
+ * updateConjBF pass is chosen by timing the candidates on the first run
+ * (sites per wavefront 64 or 1, block / LDS tile size); a different choice
+ * changes only the order of floating-point sums.
+ * --------------------------------------------------------------------- */
+
+typedef struct nngp_cvgrid nngp_cvgrid;
+
+typedef struct nngp_cvgrid_detail {
+  /* optional outputs (NULL: not returned), problem j = fold * g + point: */
+  double *betaHat;            /* p x (k g)                                  */
+  double *ab;                 /* 2 x (k g): a, b                            */
+  double *bBInv;              /* p*p x (k g): (X'V^-1X)^-1 of the fold      */
+  double *y0Hat, *y0HatVar;   /* nHo x (k g)                                */
+  /* filled by every run: */
+  double graph_ms;            /* device time of the run's graph             */
+  int bf_family, bf_threads;  /* launch shape in use (0: site per lane)    */
+  int tune_n;                 /* candidates timed at the last tuning        */
+  int tune_family[8], tune_threads[8];
+  double tune_ms[8];
+} nngp_cvgrid_detail;
+
+int nngp_cvgrid_create(nngp_cvgrid **out, const double *y, const double *X,
+                       int p, int n, const double *coords, int m, int kFold,
+                       const int *hoIndx, int nHo, const double *sigmaSqIG,
+                       int covModel, const nngp_opts *opts);
+int nngp_cvgrid_run(nngp_cvgrid *cv, const double *thetaAlpha, int g,
+                    double *scores, nngp_cvgrid_detail *detail);
+void nngp_cvgrid_destroy(nngp_cvgrid *cv);
+
 #ifdef __cplusplus
 }
 #endif

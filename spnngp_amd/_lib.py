@@ -38,6 +38,7 @@ EXPORTS = (
     "nngp_server_create", "nngp_server_destroy", "nngp_server_submit",
     "nngp_server_wait", "nngp_server_release", "nngp_server_predict",
     "nngp_server_get_stats",
+    "nngp_cvgrid_create", "nngp_cvgrid_run", "nngp_cvgrid_destroy",
 )
 
 _RNG_FN = C.CFUNCTYPE(C.c_double, C.c_void_p)
@@ -81,6 +82,15 @@ class NngpServerStats(C.Structure):
                 ("device_bytes_peak", C.c_ulonglong),
                 ("host_bytes_used", C.c_ulonglong),
                 ("host_bytes_peak", C.c_ulonglong)]
+
+
+class NngpCvGridDetail(C.Structure):
+    _fields_ = [("betaHat", C.c_void_p), ("ab", C.c_void_p),
+                ("bBInv", C.c_void_p), ("y0Hat", C.c_void_p),
+                ("y0HatVar", C.c_void_p), ("graph_ms", C.c_double),
+                ("bf_family", C.c_int), ("bf_threads", C.c_int),
+                ("tune_n", C.c_int), ("tune_family", C.c_int * 8),
+                ("tune_threads", C.c_int * 8), ("tune_ms", C.c_double * 8)]
 
 
 class NngpError(RuntimeError):
@@ -162,6 +172,13 @@ def load():
     L.nngp_server_release.argtypes = [_vp, C.c_longlong]
     L.nngp_server_predict.argtypes = [_vp, _i, _dp, _dp, C.c_ulonglong, _dp]
     L.nngp_server_get_stats.argtypes = [_vp, C.POINTER(NngpServerStats)]
+    L.nngp_cvgrid_create.argtypes = [C.POINTER(_vp), _dp, _dp, _i, _i, _dp,
+                                     _i, _i, _ip, _i, _dp, _i,
+                                     C.POINTER(NngpOpts)]
+    L.nngp_cvgrid_run.argtypes = [_vp, _dp, _i, _dp,
+                                  C.POINTER(NngpCvGridDetail)]
+    L.nngp_cvgrid_destroy.argtypes = [_vp]
+    L.nngp_cvgrid_destroy.restype = None
     _lib = L
     return _lib
 

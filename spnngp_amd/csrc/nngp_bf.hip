@@ -1,40 +1,54 @@
 // nngp_bf.hip — neighbor-system kernels: updateBF / updateConjBF
 // (src/rNNGP.cpp:46-101, src/cNNGP.cpp:24-69), the quadratic form Q
-// (src/util.cpp:261-280) and the site sums the samplers need.
+// (src/util.cpp:261-280) and the site sums the samplers and the conjugate
+// grid need.
 //
-// Two updateBF kernels:
-//  * k_bf_lane<M, COV>: one site per lane, the whole M x M system in
-//    registers, fully unrolled (M <= 16, closed-form covariances).  Sites
-//    with fewer than M neighbors (the first m sites of the NNGP order) are
-//    padded with far-away points: their rows/columns of C are exactly zero
-//    off the diagonal, so the real block factors bit-for-bit as if unpadded
-//    and the padded coefficients come out exactly 0.  The kernel also adds
-//    up its block's site terms (log F_i and e_i e_i^T / F_i, with e_i the
-//    residual vector of X and y at site i), so one launch gives everything
-//    a Metropolis step needs.
-//  * k_bf_coop: one site per wavefront, the matrix in LDS, one row per lane
-//    (any m <= 64, any covariance including the Bessel-K matern); it stores
-//    the residual vectors and k_stats adds them up.
+// Two updateBF kernels, both adding up their sites' terms in the same
+// launch (log F_i and e_i e_i^T / F_i, with e_i the residual vector of X
+// and y at site i), so one launch gives everything a Metropolis step or a
+// conjugate grid point needs:
+//  * k_bf_lane<M, COV, BATCH>: one site per lane (64 sites per wavefront),
+//    the whole M x M system in registers, fully unrolled (M <= 16,
+//    closed-form covariances).  Sites with fewer than M neighbors (the
+//    first m sites of the NNGP order) are padded with far-away points:
+//    their rows/columns of C are exactly zero off the diagonal, so the real
+//    block factors bit-for-bit as if unpadded and the padded coefficients
+//    come out exactly 0.  Block size 64, 128 or 256 threads.
+//  * k_bf_coop<BATCH>: one site per wavefront, the matrix in LDS, one row
+//    per lane (any m <= 64, any covariance including the Bessel-K matern),
+//    1, 2 or 4 wavefronts per block walking a contiguous run of sites.
 // Both compute C = s2 R(N,N) + t2 I, c = s2 R(N,i), the Cholesky factor of C
 // (pivots kept as reciprocal square roots), v = L^-1 c, B = L^-T v and
 // F = s2 + t2 - v.v, i.e. the reference's potrf -> potri -> symv result
 // without forming the inverse.
 //
-// Sums are deterministic: a fixed grid, every thread adds a fixed set of
-// sites in a fixed order, each block reduces with a fixed butterfly per
-// wavefront and its wavefronts in order into one partial row, and
-// k_rows_sum adds the rows of each column in a fixed order.
+// BATCH = true runs many problems in one launch: gridDim.y problems, problem
+// j is fold j / ng at grid point j % ng (the conjugate model's k-fold grid
+// search); fold arrays sit at fixed strides and each problem has its own
+// covariance parameters, partial rows and failure flag.
+//
+// Sums are deterministic for a given launch configuration: every thread
+// adds a fixed set of sites in a fixed order, each block reduces with a
+// fixed butterfly per wavefront and its wavefronts in order into one
+// partial row, and k_rows_sum adds the rows of each column in a fixed order.
+#include <hip/hip_runtime.h>
+
 #include <climits>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
 
 #include "nngp_kernels.hpp"
 
 namespace nngp {
 
-constexpr int kLaneBlock = 256;
-constexpr int kCoopWaves = 4;
+constexpr int kLaneMaxThreads = 256;
+constexpr int kCoopMaxWaves = 4;
 constexpr int kCoopMaxM = 64;
-constexpr int kRedThreads = 256;  // threads per block of the sum kernels
-constexpr int kRedBlocks = 1024;  // grid of the sum kernels (at most)
+constexpr int kCoopBlocks = 1024;  // logical blocks of the wavefront kernel
+constexpr int kRedThreads = 256;   // threads per block of the sum kernels
+constexpr int kRedBlocks = 1024;   // grid of the sum kernels (at most)
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -78,22 +92,50 @@ __device__ __forceinline__ double wave_sum_to63(double x) {
 }
 
 int rows_capacity(int n) {
-  const int lane_rows = (n + kLaneBlock - 1) / kLaneBlock;
-  return lane_rows > kRedBlocks ? lane_rows : kRedBlocks;
+  const int lane_rows = (n + 63) / 64;  // smallest lane block is 64 threads
+  return lane_rows > kCoopBlocks ? lane_rows : kCoopBlocks;
+}
+
+// ---------------------------------------------------------------------------
+// problem view
+// ---------------------------------------------------------------------------
+struct BfProblem {
+  const double2 *xy;
+  const int *nnIndx;
+  const int *lu;
+  const double *Z;
+  double *rows;
+  int *err;
+};
+
+template <bool BATCH>
+__device__ __forceinline__ BfProblem bf_problem(const BfArgs &A) {
+  BfProblem P{A.xy, A.nnIndx, A.lu, A.Z, A.rows, A.err};
+  if constexpr (BATCH) {
+    const int j = (int)blockIdx.y;
+    const int f = j / A.ng;
+    P.xy += (size_t)f * A.fs_xy;
+    P.nnIndx += (size_t)f * A.fs_nn;
+    P.lu += (size_t)f * A.fs_lu;
+    if (P.Z) P.Z += (size_t)f * A.fs_Z;
+    if (P.rows) P.rows += (size_t)j * A.ps_rows;
+    P.err += j;
+  }
+  return P;
 }
 
 // ---------------------------------------------------------------------------
 // one site per lane
 // ---------------------------------------------------------------------------
 
-// The block's partial row of one pass: column 0 sum log F_i, column 1 zero
-// (quadratic forms are formed from the cross-products), then the lower
-// triangle of e e^T / F row by row.  Called by every thread of the block.
-template <int NT>
-__device__ inline void block_site_terms(const BfArgs &A, bool valid,
-                                        double F, const double (&e)[kMaxNZ],
-                                        int row) {
-  __shared__ double red[NT / 64][kStatsMax];
+// The block's partial row: column 0 sum log F_i, column 1 zero (quadratic
+// forms are formed from the cross-products), then the lower triangle of
+// e e^T / F row by row.  Called by every thread of the block.
+__device__ inline void block_site_terms(int nz, double *rows, int rows_cap,
+                                        bool valid, double F,
+                                        const double (&e)[kMaxNZ], int row) {
+  __shared__ double red[kLaneMaxThreads / 64][kStatsMax];
+  const int nw = (int)blockDim.x >> 6;
   const int w = (int)threadIdx.x >> 6, t = (int)threadIdx.x & 63;
   const double rf = valid ? 1.0 / F : 0.0;
   double x = wave_sum_to63(valid ? log(F) : 0.0);
@@ -101,7 +143,7 @@ __device__ inline void block_site_terms(const BfArgs &A, bool valid,
   int g = 2;
 #pragma unroll
   for (int p = 0; p < kMaxNZ; p++) {
-    if (p < A.nz) {
+    if (p < nz) {
 #pragma unroll
       for (int q = 0; q <= p; q++) {
         x = wave_sum_to63(e[p] * e[q] * rf);
@@ -111,44 +153,45 @@ __device__ inline void block_site_terms(const BfArgs &A, bool valid,
     g += p + 1;
   }
   __syncthreads();
-  const int nv = 2 + A.nz * (A.nz + 1) / 2;
+  const int nv = 2 + nz * (nz + 1) / 2;
   const int k = (int)threadIdx.x;
   if (k < nv) {
     double s = 0.0;
     if (k != 1)
-#pragma unroll
-      for (int ww = 0; ww < NT / 64; ww++) s += red[ww][k];
-    A.rows[(size_t)k * A.rows_cap + row] = s;
+      for (int ww = 0; ww < nw; ww++) s += red[ww][k];
+    rows[(size_t)k * rows_cap + row] = s;
   }
 }
 
-template <int M, int COV>
-__global__ __launch_bounds__(kLaneBlock) void k_bf_lane(BfArgs A) {
-  const int b = xcd_block((A.n + kLaneBlock - 1) / kLaneBlock);
+template <int M, int COV, bool BATCH>
+__global__ __launch_bounds__(kLaneMaxThreads) void k_bf_lane(BfArgs A) {
+  const int nt = (int)blockDim.x;
+  const int b = xcd_block((A.n + nt - 1) / nt);
   if (b < 0) return;  // whole padding block
-  const int i = b * kLaneBlock + (int)threadIdx.x;
+  const BfProblem P = bf_problem<BATCH>(A);
+  const CovArgs c = BATCH ? A.covs[(int)blockIdx.y % A.ng] : A.c;
+  const int i = b * nt + (int)threadIdx.x;
   const bool valid = i < A.n;
-  const CovArgs &c = A.c;
   double Fi = 1.0;
   double e[kMaxNZ];
 #pragma unroll
   for (int j = 0; j < kMaxNZ; j++) e[j] = 0.0;
 
   if (valid) {
-    const int cnt = A.lu[A.n + i];
-    const int off = A.lu[i];
+    const int cnt = P.lu[A.n + i];
+    const int off = P.lu[i];
     if (cnt == 0) {  // site 0: F = s2 + t2, residual = the data
       Fi = c.diag;
 #pragma unroll
       for (int j = 0; j < kMaxNZ; j++)
-        if (j < A.nz) e[j] = A.Z[(size_t)i * A.zs + j];
+        if (j < A.nz) e[j] = P.Z[(size_t)i * A.zs + j];
     } else {
       // The index array carries 64 padding entries, so off + k is in
       // bounds; entries past this site's count are not its neighbors and
       // are replaced by index 0 before any use.
       int nb[M];
 #pragma unroll
-      for (int k = 0; k < M; k++) nb[k] = A.nnIndx[off + k];
+      for (int k = 0; k < M; k++) nb[k] = P.nnIndx[off + k];
       if (cnt < M) {
 #pragma unroll
         for (int k = 0; k < M; k++)
@@ -157,7 +200,7 @@ __global__ __launch_bounds__(kLaneBlock) void k_bf_lane(BfArgs A) {
       double px[M], py[M];
 #pragma unroll
       for (int k = 0; k < M; k++) {
-        const double2 q = A.xy[nb[k]];
+        const double2 q = P.xy[nb[k]];
         px[k] = q.x;
         py[k] = q.y;
       }
@@ -170,7 +213,7 @@ __global__ __launch_bounds__(kLaneBlock) void k_bf_lane(BfArgs A) {
           }
         }
       }
-      const double2 s = A.xy[i];
+      const double2 s = P.xy[i];
 
       constexpr int T = M * (M + 1) / 2;
       double a[T];  // packed lower triangle
@@ -185,7 +228,7 @@ __global__ __launch_bounds__(kLaneBlock) void k_bf_lane(BfArgs A) {
       }
       double vv;
       const bool ok = chol_solve<M>(a, v, vv);  // v <- B_i = C^-1 c
-      if (A.B) {
+      if (!BATCH && A.B) {
 #pragma unroll
         for (int k = 0; k < M; k++)
           if (k < cnt) A.B[off + k] = v[k];
@@ -195,23 +238,18 @@ __global__ __launch_bounds__(kLaneBlock) void k_bf_lane(BfArgs A) {
 #pragma unroll
       for (int j = 0; j < kMaxNZ; j++) {
         if (j < A.nz) {
-          double ej = A.Z[(size_t)i * A.zs + j];
+          double ej = P.Z[(size_t)i * A.zs + j];
 #pragma unroll
           for (int k = 0; k < M; k++)
-            ej = fma(-v[k], A.Z[(size_t)nb[k] * A.zs + j], ej);
+            ej = fma(-v[k], P.Z[(size_t)nb[k] * A.zs + j], ej);
           e[j] = ej;
         }
       }
-      if (!ok) atomicMin(A.err, i);
+      if (!ok) atomicMin(P.err, i);
     }
-    A.F[i] = Fi;
-    if (A.E) {
-#pragma unroll
-      for (int j = 0; j < kMaxNZ; j++)
-        if (j < A.nz) A.E[(size_t)j * A.n + i] = e[j];
-    }
+    if (!BATCH && A.F) A.F[i] = Fi;
   }
-  if (A.rows) block_site_terms<kLaneBlock>(A, valid, Fi, e, b);
+  if (P.rows) block_site_terms(A.nz, P.rows, A.rows_cap, valid, Fi, e, b);
 }
 
 // ---------------------------------------------------------------------------
@@ -229,47 +267,36 @@ __device__ __forceinline__ int tri_row(int p) {
 __host__ __device__ inline int coop_wave_doubles(int m) {
   return m * (m + 1) + 3 * m;
 }
-// Waves per block so that the block's LDS stays within 64 KB.
-static int coop_waves(int m) {
-  const int per = coop_wave_doubles(m) * (int)sizeof(double);
-  int w = 65536 / per;
-  return w < 1 ? 1 : (w > kCoopWaves ? kCoopWaves : w);
-}
 
-__global__ __launch_bounds__(64 * kCoopWaves) void k_bf_coop(BfArgs A) {
-  extern __shared__ double smem[];
-  const int nw = (int)blockDim.x >> 6;
-  const int ld = A.m + 1;
-  const int per_wave = coop_wave_doubles(A.m);
-  const int w = (int)threadIdx.x >> 6;
+// Solve site i's system in the wave's LDS area Am; every lane returns F and
+// the residual vector e (identical in all lanes); lane t < cnt returns its
+// coefficient B_t in bcoef.  False when a pivot is not positive.
+__device__ inline bool coop_site(const BfArgs &A, const BfProblem &P,
+                                 const CovArgs &c, double *Am, int i,
+                                 double &F, double (&e)[kMaxNZ],
+                                 double &bcoef, int &cnt_out, int &off_out) {
   const int t = (int)threadIdx.x & 63;
-  double *Am = smem + (size_t)w * per_wave;  // row-major, Am[k*ld + l]
+  const int ld = A.m + 1;
   double *cv = Am + A.m * ld;
   double *px = cv + A.m;
   double *py = px + A.m;
-
-  const int nlog = (A.n + nw - 1) / nw;
-  const int b = xcd_block(nlog);
-  if (b < 0) return;
-  const int i = b * nw + w;
-  if (i >= A.n) return;  // whole wave leaves; no block-level barrier below
-  const CovArgs &c = A.c;
-  const int cnt = A.lu[A.n + i];
+  const int cnt = P.lu[A.n + i];
+  const int off = P.lu[i];
+  cnt_out = cnt;
+  off_out = off;
+  bcoef = 0.0;
   if (cnt == 0) {
-    if (t == 0) {
-      A.F[i] = c.diag;
-      if (A.E)
-        for (int j = 0; j < A.nz; j++)
-          A.E[(size_t)j * A.n + i] = A.Z[(size_t)i * A.zs + j];
-    }
-    return;
+    F = c.diag;
+#pragma unroll
+    for (int j = 0; j < kMaxNZ; j++)
+      e[j] = j < A.nz ? P.Z[(size_t)i * A.zs + j] : 0.0;
+    return true;
   }
-  const int off = A.lu[i];
-  const double2 s = A.xy[i];
+  const double2 s = P.xy[i];
   int myid = 0;
   if (t < cnt) {
-    myid = A.nnIndx[off + t];
-    const double2 q = A.xy[myid];
+    myid = P.nnIndx[off + t];
+    const double2 q = P.xy[myid];
     px[t] = q.x;
     py[t] = q.y;
   }
@@ -313,13 +340,11 @@ __global__ __launch_bounds__(64 * kCoopWaves) void k_bf_coop(BfArgs A) {
     wave_sync();
   }
   if (!ok) {
-    if (t == 0) {
-      atomicMin(A.err, i);
-      A.F[i] = 1.0;  // keeps the sums finite; the pass reports the failure
-      if (A.E)
-        for (int j = 0; j < A.nz; j++) A.E[(size_t)j * A.n + i] = 0.0;
-    }
-    return;
+    F = 1.0;  // keeps the sums finite; the pass reports the failure
+#pragma unroll
+    for (int j = 0; j < kMaxNZ; j++) e[j] = 0.0;
+    wave_sync();
+    return false;
   }
   // forward solve: lane t holds x_t
   double x = t < cnt ? cv[t] : 0.0;
@@ -335,32 +360,94 @@ __global__ __launch_bounds__(64 * kCoopWaves) void k_bf_coop(BfArgs A) {
     if (t == k) x = bk;
     if (t < k) x = fma(-Am[k * ld + t], bk, x);
   }
-  if (A.B && t < cnt) A.B[off + t] = x;
-  if (t == 0) A.F[i] = c.diag - vv;
-  if (A.E) {
-    for (int j = 0; j < A.nz; j++) {
+  bcoef = t < cnt ? x : 0.0;
+  F = c.diag - vv;
+#pragma unroll
+  for (int j = 0; j < kMaxNZ; j++) {
+    e[j] = 0.0;
+    if (j < A.nz) {
       const double part =
-          wave_sum(t < cnt ? x * A.Z[(size_t)myid * A.zs + j] : 0.0);
-      if (t == 0)
-        A.E[(size_t)j * A.n + i] = A.Z[(size_t)i * A.zs + j] - part;
+          wave_sum(t < cnt ? x * P.Z[(size_t)myid * A.zs + j] : 0.0);
+      e[j] = P.Z[(size_t)i * A.zs + j] - part;
     }
+  }
+  wave_sync();  // this LDS area is reused by the wave's next site
+  return true;
+}
+
+// Logical block b covers sites [b*chunk, (b+1)*chunk); its waves take every
+// nw-th site of that run.  A.coop_blocks logical blocks per problem.
+template <bool BATCH>
+__global__ __launch_bounds__(64 * kCoopMaxWaves) void k_bf_coop(BfArgs A) {
+  extern __shared__ double smem[];
+  __shared__ double red[kCoopMaxWaves][kStatsMax];
+  const int nw = (int)blockDim.x >> 6;
+  const int w = (int)threadIdx.x >> 6;
+  const int t = (int)threadIdx.x & 63;
+  const int b = xcd_block(A.coop_blocks);
+  if (b < 0) return;
+  const BfProblem P = bf_problem<BATCH>(A);
+  const CovArgs c = BATCH ? A.covs[(int)blockIdx.y % A.ng] : A.c;
+  double *Am = smem + (size_t)w * coop_wave_doubles(A.m);
+  const int chunk = (A.n + A.coop_blocks - 1) / A.coop_blocks;
+  const int i0 = b * chunk;
+  const int i1 = min(A.n, i0 + chunk);
+  double acc[kStatsMax];
+#pragma unroll
+  for (int k = 0; k < kStatsMax; k++) acc[k] = 0.0;
+  for (int i = i0 + w; i < i1; i += nw) {
+    double F, e[kMaxNZ], bcoef;
+    int cnt, off;
+    const bool ok = coop_site(A, P, c, Am, i, F, e, bcoef, cnt, off);
+    if (!ok && t == 0) atomicMin(P.err, i);
+    if (!BATCH) {
+      if (A.B && ok && t < cnt) A.B[off + t] = bcoef;
+      if (A.F && t == 0) A.F[i] = F;
+    }
+    const double rf = 1.0 / F;
+    acc[0] += log(F);
+    int g = 2;
+#pragma unroll
+    for (int p = 0; p < kMaxNZ; p++) {
+      if (p < A.nz) {
+#pragma unroll
+        for (int q = 0; q <= p; q++) acc[g + q] += e[p] * e[q] * rf;
+      }
+      g += p + 1;
+    }
+  }
+  if (!P.rows) return;
+  const int nv = 2 + A.nz * (A.nz + 1) / 2;
+  if (t == 0) {
+#pragma unroll
+    for (int k = 0; k < kStatsMax; k++)
+      if (k < nv) red[w][k] = acc[k];
+  }
+  __syncthreads();
+  const int k = (int)threadIdx.x;
+  if (k < nv) {
+    double s = 0.0;
+    if (k != 1)
+      for (int ww = 0; ww < nw; ww++) s += red[ww][k];
+    P.rows[(size_t)k * A.rows_cap + b] = s;
   }
 }
 
 // ---------------------------------------------------------------------------
 // sums
 // ---------------------------------------------------------------------------
-// out[k] = sum over rows r < nrows of rows[k*cap + r] (one block per column
-// k, fixed per-thread order, fixed butterfly); block 0 also hands over the
-// updateBF failure flag: *err_out = *err, *err = INT_MAX.
+// Block (k, j): out[j*out_stride + k] = sum over rows r < nrows of column k
+// of problem j's rows (fixed per-thread order, fixed butterfly).  Block
+// (0, j) also hands over problem j's failure flag:
+// err_out[j] = err[j], err[j] = INT_MAX.
 __global__ __launch_bounds__(kRedThreads) void k_rows_sum(
-    const double *rows, int cap, int nrows, double *out, int *err,
-    double *err_out) {
+    const double *rows, int cap, int nrows, long long ps_rows, double *out,
+    int out_stride, int *err, double *err_out) {
   __shared__ double red[kRedThreads / 64];
-  const int k = (int)blockIdx.x;
+  const int k = (int)blockIdx.x, j = (int)blockIdx.y;
   const int w = (int)threadIdx.x >> 6, t = (int)threadIdx.x & 63;
   double s = 0.0;
-  const double *col = rows + (size_t)k * cap;
+  const double *col = rows + (size_t)j * ps_rows + (size_t)k * cap;
   for (int r = (int)threadIdx.x; r < nrows; r += kRedThreads) s += col[r];
   s = wave_sum(s);
   if (t == 0) red[w] = s;
@@ -369,19 +456,20 @@ __global__ __launch_bounds__(kRedThreads) void k_rows_sum(
     double tot = 0.0;
 #pragma unroll
     for (int ww = 0; ww < kRedThreads / 64; ww++) tot += red[ww];
-    out[k] = tot;
+    out[(size_t)j * out_stride + k] = tot;
     if (k == 0 && err) {
-      *err_out = (double)*err;
-      *err = INT_MAX;
+      err_out[j] = (double)err[j];
+      err[j] = INT_MAX;
     }
   }
 }
 
-static int rows_sum_launch(const double *rows, int cap, int nrows, int nv,
-                           double *out, int *err, double *err_out,
-                           hipStream_t st) {
-  hipLaunchKernelGGL(k_rows_sum, dim3(nv), dim3(kRedThreads), 0, st, rows,
-                     cap, nrows, out, err, err_out);
+int rows_sum_launch(const double *rows, int cap, int nrows, long long ps_rows,
+                    int nv, int nprob, double *out, int out_stride, int *err,
+                    double *err_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_rows_sum, dim3(nv, nprob), dim3(kRedThreads), 0, st,
+                     rows, cap, nrows, ps_rows, out, out_stride, err,
+                     err_out);
   return (int)hipGetLastError();
 }
 
@@ -390,112 +478,180 @@ static int red_grid(int n) {
   return b < 1 ? 1 : (b > kRedBlocks ? kRedBlocks : b);
 }
 
-template <int NZ>
-__global__ __launch_bounds__(kRedThreads) void k_stats(StatsArgs A) {
-  constexpr int NV = 2 + NZ * (NZ + 1) / 2;
-  __shared__ double red[kRedThreads / 64][NV];
-  double acc[NV];
-#pragma unroll
-  for (int v = 0; v < NV; v++) acc[v] = 0.0;
-  const int stride = (int)gridDim.x * kRedThreads;
-  for (int i = blockIdx.x * kRedThreads + threadIdx.x; i < A.n; i += stride) {
-    const double f = A.F[i];
-    acc[0] += log(f);
-    if constexpr (NZ > 0) {
-      const double rf = 1.0 / f;
-      double e[NZ];
-#pragma unroll
-      for (int j = 0; j < NZ; j++) e[j] = A.E[(size_t)j * A.n + i];
-      int g = 2;
-#pragma unroll
-      for (int p = 0; p < NZ; p++)
-#pragma unroll
-        for (int q = 0; q <= p; q++) acc[g++] += e[p] * e[q] * rf;
-    }
-  }
-  const int w = (int)threadIdx.x >> 6, t = (int)threadIdx.x & 63;
-#pragma unroll
-  for (int v = 0; v < NV; v++) {
-    const double x = wave_sum(acc[v]);
-    if (t == 0) red[w][v] = x;
-  }
-  __syncthreads();
-  if ((int)threadIdx.x < NV) {
-    double s = 0.0;
-#pragma unroll
-    for (int ww = 0; ww < kRedThreads / 64; ww++) s += red[ww][threadIdx.x];
-    A.rows[(size_t)threadIdx.x * A.rows_cap + blockIdx.x] = s;
-  }
-}
-
-int stats_launch(const StatsArgs &a, hipStream_t st) {
-  const int nb = red_grid(a.n);
-  const int nz = a.E ? a.nz : 0;
-  switch (nz) {
-#define NNGP_STATS_CASE(K)                                                  \
-  case K:                                                                  \
-    hipLaunchKernelGGL(k_stats<K>, dim3(nb), dim3(kRedThreads), 0, st, a); \
-    break;
-    NNGP_STATS_CASE(0)
-    NNGP_STATS_CASE(2)
-    NNGP_STATS_CASE(3)
-    NNGP_STATS_CASE(4)
-    NNGP_STATS_CASE(5)
-    NNGP_STATS_CASE(6)
-    NNGP_STATS_CASE(7)
-    NNGP_STATS_CASE(8)
-    NNGP_STATS_CASE(9)
-#undef NNGP_STATS_CASE
-    default: return -1;
-  }
-  const int nv = 2 + nz * (nz + 1) / 2;
-  return rows_sum_launch(a.rows, a.rows_cap, nb, nv, a.out, a.err,
-                         a.err_out, st);
-}
-
 // ---------------------------------------------------------------------------
-// dispatch of one pass
+// dispatch
 // ---------------------------------------------------------------------------
-template <int M>
-static void launch_lane_m(const BfArgs &a, int cov, hipStream_t st) {
-  const int nlog = (a.n + kLaneBlock - 1) / kLaneBlock;
-  const dim3 grid(8 * ((nlog + 7) / 8));
-  switch (cov) {
-    case 0: hipLaunchKernelGGL((k_bf_lane<M, 0>), grid, dim3(kLaneBlock), 0, st, a); break;
-    case 1: hipLaunchKernelGGL((k_bf_lane<M, 1>), grid, dim3(kLaneBlock), 0, st, a); break;
-    default: hipLaunchKernelGGL((k_bf_lane<M, 3>), grid, dim3(kLaneBlock), 0, st, a); break;
-  }
+bool bf_lane_ok(int m, int variant) { return m <= 16 && variant >= 0; }
+
+// LDS per block of the wavefront kernel with `waves` waves.
+static size_t coop_shm(int m, int waves) {
+  return (size_t)waves * coop_wave_doubles(m) * sizeof(double);
 }
 
-bool bf_fuses_sums(int m, int cov) { return cov != 2 && m <= 16; }
+bool bf_config_ok(const BfConfig &cfg, int m, int variant) {
+  if (cfg.family == 0)
+    return bf_lane_ok(m, variant) &&
+           (cfg.threads == 64 || cfg.threads == 128 || cfg.threads == 256);
+  const int waves = cfg.threads / 64;
+  return m <= kCoopMaxM && cfg.threads % 64 == 0 && waves >= 1 &&
+         waves <= kCoopMaxWaves && coop_shm(m, waves) <= 65536;
+}
 
-int bf_launch(const BfArgs &a, hipStream_t st) {
-  if (a.n <= 0) return 0;
-  const int cov = a.c.cov;
-  if (bf_fuses_sums(a.m, cov)) {
-    if (a.m <= 4) launch_lane_m<4>(a, cov, st);
-    else if (a.m <= 8) launch_lane_m<8>(a, cov, st);
-    else if (a.m <= 10) launch_lane_m<10>(a, cov, st);
-    else if (a.m <= 12) launch_lane_m<12>(a, cov, st);
-    else if (a.m <= 15) launch_lane_m<15>(a, cov, st);
-    else launch_lane_m<16>(a, cov, st);
+BfConfig bf_default(int m, int variant) {
+  BfConfig c;
+  if (bf_lane_ok(m, variant)) {
+    c.family = 0;
+    c.threads = 256;
   } else {
-    if (a.m > kCoopMaxM) return -1;
-    const int nw = coop_waves(a.m);
-    const int nlog = (a.n + nw - 1) / nw;
-    const dim3 grid(8 * ((nlog + 7) / 8));
-    const size_t shm = (size_t)nw * coop_wave_doubles(a.m) * sizeof(double);
-    hipLaunchKernelGGL(k_bf_coop, grid, dim3(64 * nw), shm, st, a);
+    c.family = 1;
+    int w = kCoopMaxWaves;
+    while (w > 1 && coop_shm(m, w) > 65536) w--;
+    c.threads = 64 * w;
+  }
+  return c;
+}
+
+int bf_rows(const BfArgs &a, const BfConfig &cfg) {
+  if (cfg.family == 0) return (a.n + cfg.threads - 1) / cfg.threads;
+  const int waves = cfg.threads / 64;
+  const int need = (a.n + waves - 1) / waves;
+  return need < kCoopBlocks ? need : kCoopBlocks;
+}
+
+template <int M, bool BATCH>
+static void launch_lane_m(const BfArgs &a, int variant, const dim3 &grid,
+                          const dim3 &blk, hipStream_t st) {
+  switch (variant) {
+    case 0: hipLaunchKernelGGL((k_bf_lane<M, 0, BATCH>), grid, blk, 0, st, a); break;
+    case 1: hipLaunchKernelGGL((k_bf_lane<M, 1, BATCH>), grid, blk, 0, st, a); break;
+    case kCovMatern15: hipLaunchKernelGGL((k_bf_lane<M, kCovMatern15, BATCH>), grid, blk, 0, st, a); break;
+    case kCovMatern25: hipLaunchKernelGGL((k_bf_lane<M, kCovMatern25, BATCH>), grid, blk, 0, st, a); break;
+    default: hipLaunchKernelGGL((k_bf_lane<M, 3, BATCH>), grid, blk, 0, st, a); break;
+  }
+}
+
+template <bool BATCH>
+static void launch_lane(const BfArgs &a, int variant, const dim3 &grid,
+                        const dim3 &blk, hipStream_t st) {
+  if (a.m <= 4) launch_lane_m<4, BATCH>(a, variant, grid, blk, st);
+  else if (a.m <= 8) launch_lane_m<8, BATCH>(a, variant, grid, blk, st);
+  else if (a.m <= 10) launch_lane_m<10, BATCH>(a, variant, grid, blk, st);
+  else if (a.m <= 12) launch_lane_m<12, BATCH>(a, variant, grid, blk, st);
+  else if (a.m <= 15) launch_lane_m<15, BATCH>(a, variant, grid, blk, st);
+  else launch_lane_m<16, BATCH>(a, variant, grid, blk, st);
+}
+
+int bf_launch(const BfArgs &a0, const BfConfig &cfg, int nprob,
+              hipStream_t st) {
+  if (a0.n <= 0 || nprob <= 0) return 0;
+  const bool batch = a0.ng > 0;
+  if (!batch && nprob != 1) return -2;
+  const int variant = a0.variant;
+  if (!bf_config_ok(cfg, a0.m, variant)) return -1;
+  BfArgs a = a0;
+  const int nrows = bf_rows(a, cfg);
+  if (a.rows && nrows > a.rows_cap) return -2;
+  const dim3 grid(8 * ((nrows + 7) / 8), nprob);
+  const dim3 blk(cfg.threads);
+  if (cfg.family == 0) {
+    if (batch) launch_lane<true>(a, variant, grid, blk, st);
+    else launch_lane<false>(a, variant, grid, blk, st);
+  } else {
+    a.coop_blocks = nrows;
+    const size_t shm = coop_shm(a.m, cfg.threads / 64);
+    if (batch)
+      hipLaunchKernelGGL(k_bf_coop<true>, grid, blk, shm, st, a);
+    else
+      hipLaunchKernelGGL(k_bf_coop<false>, grid, blk, shm, st, a);
   }
   return (int)hipGetLastError();
 }
 
-int bf_sums_launch(const BfArgs &a, double *out, double *err_out,
+int bf_sums_launch(const BfArgs &a, const BfConfig &cfg, int nprob,
+                   double *out, int out_stride, double *err_out,
                    hipStream_t st) {
   const int nv = 2 + a.nz * (a.nz + 1) / 2;
-  const int nrows = (a.n + kLaneBlock - 1) / kLaneBlock;
-  return rows_sum_launch(a.rows, a.rows_cap, nrows, nv, out, a.err, err_out,
-                         st);
+  return rows_sum_launch(a.rows, a.rows_cap, bf_rows(a, cfg), a.ps_rows, nv,
+                         nprob, out, out_stride, a.err, err_out, st);
+}
+
+// ---------------------------------------------------------------------------
+// autotuner
+// ---------------------------------------------------------------------------
+// Candidates: sites per wavefront (64: the per-lane kernel at 64, 128 or
+// 256 threads per block; 1: the wavefront kernel with 1, 2 or 4 waves per
+// block, i.e. LDS tiles of 1, 2 or 4 systems).  Each is timed with HIP
+// events on the caller's stream (one warm-up, then `reps` launches) and the
+// fastest kept.  Results are cached per (m, variant, nz, size class).
+namespace {
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int>, BfConfig> g_tune_cache;
+}  // namespace
+
+int bf_autotune(const BfArgs &a, int nprob, hipStream_t st, BfConfig *best,
+                BfTuneReport *rep) {
+  const int variant = a.variant;
+  const long long work = (long long)a.n * nprob;
+  int cls = 0;
+  while ((1LL << (cls + 1)) <= work) cls++;
+  const auto key = std::make_tuple(a.m, variant, a.nz, cls);
+  if (!rep) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      *best = it->second;
+      return 0;
+    }
+  }
+  std::vector<BfConfig> cand;
+  for (int th : {64, 128, 256}) cand.push_back(BfConfig{0, th});
+  for (int w : {1, 2, 4}) cand.push_back(BfConfig{1, 64 * w});
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return -1;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return -1;
+  }
+  const int reps = 3;
+  double best_ms = 1e300;
+  BfConfig pick = bf_default(a.m, variant);
+  int nc = 0;
+  int rc = 0;
+  for (const BfConfig &cfg : cand) {
+    if (!bf_config_ok(cfg, a.m, variant)) continue;
+    if (a.rows && bf_rows(a, cfg) > a.rows_cap) continue;
+    if ((rc = bf_launch(a, cfg, nprob, st)) != 0) break;
+    if ((rc = (int)hipEventRecord(e0, st)) != 0) break;
+    for (int r = 0; r < reps && rc == 0; r++) rc = bf_launch(a, cfg, nprob, st);
+    if (rc) break;
+    if ((rc = (int)hipEventRecord(e1, st)) != 0) break;
+    if ((rc = (int)hipEventSynchronize(e1)) != 0) break;
+    float ms = 0.0f;
+    if ((rc = (int)hipEventElapsedTime(&ms, e0, e1)) != 0) break;
+    const double per = ms / reps;
+    if (rep && nc < 8) {
+      rep->family[nc] = cfg.family;
+      rep->threads[nc] = cfg.threads;
+      rep->ms[nc] = per;
+    }
+    nc++;
+    if (per < best_ms) {
+      best_ms = per;
+      pick = cfg;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if (rep) {
+    rep->n = nc < 8 ? nc : 8;
+    rep->best = pick;
+    rep->best_ms = best_ms;
+  }
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_cache[key] = pick;
+  *best = pick;
+  return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -531,8 +687,8 @@ __global__ __launch_bounds__(kRedThreads) void k_quadform(QArgs A) {
 int q_launch(const QArgs &a, hipStream_t st) {
   const int nb = red_grid(a.n);
   hipLaunchKernelGGL(k_quadform, dim3(nb), dim3(kRedThreads), 0, st, a);
-  return rows_sum_launch(a.rows, kRedBlocks, nb, 1, a.out, nullptr, nullptr,
-                         st);
+  return rows_sum_launch(a.rows, kRedBlocks, nb, 0, 1, 1, a.out, 1, nullptr,
+                         nullptr, st);
 }
 
 __global__ void k_interleave(const double *cm, int n, double2 *xy) {

@@ -150,7 +150,24 @@ __device__ inline double matern_s2(double d, const CovArgs &a) {
   return a.mnorm * pow(x, a.nu) * bessel_k(x, a);
 }
 
-// s2 * rho(D) from the squared distance d2 (COV is the covModel).
+// Covariance variants of the templated kernels: the four covModels, plus
+// the matern at nu = 1.5 and nu = 2.5 in closed form, (1 + x) e^-x and
+// (1 + x + x^2/3) e^-x with x = phi D (the Bessel-K expression of
+// src/util.cpp:228-237 equals these identically; nu = 0.5 is exp(-x)).
+constexpr int kCovMatern15 = 5;
+constexpr int kCovMatern25 = 6;
+
+// The templated variant for covModel cov at smoothness nu, or -1 when only
+// the runtime (Bessel-K) evaluation applies.
+__host__ __device__ inline int cov_variant(int cov, double nu) {
+  if (cov != 2) return cov;
+  if (nu == 0.5) return 0;
+  if (nu == 1.5) return kCovMatern15;
+  if (nu == 2.5) return kCovMatern25;
+  return -1;
+}
+
+// s2 * rho(D) from the squared distance d2 (COV is a variant above).
 template <int COV>
 __device__ __forceinline__ double s2_corr(double d2, const CovArgs &a) {
   if constexpr (COV == 0) {
@@ -162,6 +179,12 @@ __device__ __forceinline__ double s2_corr(double d2, const CovArgs &a) {
     return (D > 0.0 && D <= a.inv_phi) ? v : (D >= a.inv_phi ? 0.0 : a.s2);
   } else if constexpr (COV == 2) {
     return matern_s2(nn_sqrt(d2), a);
+  } else if constexpr (COV == kCovMatern15) {
+    const double x = nn_sqrt(d2) * a.phi;
+    return s2_exp(-x, a) * (1.0 + x);
+  } else if constexpr (COV == kCovMatern25) {
+    const double x = nn_sqrt(d2) * a.phi;
+    return s2_exp(-x, a) * (1.0 + x + x * x * (1.0 / 3.0));
   } else {
     return s2_exp(-a.phi2 * d2, a);
   }

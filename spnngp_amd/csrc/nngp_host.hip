@@ -308,11 +308,7 @@ int Model::create(int n_, int m_, int p_, const double *coords,
                             hipMemcpyHostToDevice, st.s));
     NNGP_HIP(hipStreamSynchronize(st.s));
   }
-  fused = bf_fuses_sums(m, cov);
-  for (int s = 0; s < 2; s++) {
-    NNGP_TRY(F[s].alloc(sizeof(double) * (size_t)n));
-    if (nz && !fused) NNGP_TRY(E[s].alloc(sizeof(double) * (size_t)nz * n));
-  }
+  for (int s = 0; s < 2; s++) NNGP_TRY(F[s].alloc(sizeof(double) * (size_t)n));
   NNGP_TRY(rows.alloc(sizeof(double) * (size_t)rows_capacity(n) * kStatsMax));
   NNGP_TRY(sums.alloc(sizeof(double) * kSumsLen));
   NNGP_TRY(err.alloc(sizeof(int)));
@@ -337,14 +333,16 @@ int Model::bf(const CovArgs &c, int slot, bool storeB) {
   a.Z = nz ? Z.as<double>() : nullptr;
   a.nz = nz;
   a.zs = zs;
-  a.E = (nz && !fused) ? E[slot].as<double>() : nullptr;
   a.err = err.as<int>();  // INT_MAX here: every sums pass resets it
-  a.rows = fused ? rows.as<double>() : nullptr;
+  a.rows = rows.as<double>();
   a.rows_cap = rows_capacity(n);
   a.c = c;
+  a.variant = cov_variant(c.cov, c.nu);
+  const BfConfig cfg = bf_default(m, a.variant);
   last_bf = a;
+  last_cfg = cfg;
   const int ev = prof ? ev_begin(0) : -1;
-  const int rc = bf_launch(a, st.s);
+  const int rc = bf_launch(a, cfg, 1, st.s);
   if (ev >= 0) ev_end(ev);
   if (rc == -1)
     return set_err(NNGP_ERR_UNSUPPORTED, "n.neighbors = %d exceeds 64", m);
@@ -357,23 +355,9 @@ int Model::sums_launch(int slot, int off) {
   double *out = sums.as<double>() + off;
   double *err_out = sums.as<double>() + 2 * kStatsMax + (off ? 1 : 0);
   const int ev = prof ? ev_begin(1) : -1;
-  int rc;
-  if (fused) {
-    rc = bf_sums_launch(last_bf, out, err_out, st.s);
-  } else {
-    StatsArgs a;
-    std::memset(&a, 0, sizeof a);
-    a.E = nz ? E[slot].as<double>() : nullptr;
-    a.F = F[slot].as<double>();
-    a.n = n;
-    a.nz = nz;
-    a.rows = rows.as<double>();
-    a.rows_cap = rows_capacity(n);
-    a.out = out;
-    a.err = err.as<int>();
-    a.err_out = err_out;
-    rc = stats_launch(a, st.s);
-  }
+  (void)slot;
+  const int rc =
+      bf_sums_launch(last_bf, last_cfg, 1, out, kStatsMax, err_out, st.s);
   if (ev >= 0) ev_end(ev);
   if (rc) return set_err(NNGP_ERR_HIP, "sum kernel launch failed");
   return NNGP_OK;

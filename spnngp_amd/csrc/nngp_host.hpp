@@ -184,20 +184,19 @@ struct Model {
   long long nIndx = 0;
   Stream st;
   DevBuf xy, nnIndx, lu, Z;
-  DevBuf B[2], F[2], E[2];
+  DevBuf B[2], F[2];
   // sums: [0, kStatsMax) pass at off 0, [kStatsMax, 2 kStatsMax) pass at
   // off kStatsMax, then the updateBF failure flag of each (as doubles)
   DevBuf rows, sums, err;
   PinnedBuf hsums;  // host mirror of sums
-  bool fused = true;   // the updateBF kernel adds the site terms itself
   BfArgs last_bf;      // arguments of the last bf() launch
+  BfConfig last_cfg;   // and its launch shape
   PassOut slot_out[2]; // sums of the last completed pass into each slot
 
   int create(int n, int m, int p, const double *coords, const int *nnIndx,
              const int *nnIndxLU, const double *X, const double *y, int cov,
              const nngp_opts *opts);
-  // B/F (and E when the sums are not fused) of theta into slot; B stored
-  // only when storeB
+  // B, F and the block sums of theta into slot; B stored only when storeB
   int bf(const CovArgs &c, int slot, bool storeB);
   // sums of the last bf() into sums[off .. off + kStatsMax)
   int sums_launch(int slot, int off);

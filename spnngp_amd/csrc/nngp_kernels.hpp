@@ -10,40 +10,51 @@ constexpr int kMaxP = 8;    // covariates supported by the fused passes
 constexpr int kMaxNZ = kMaxP + 1;
 constexpr int kStatsMax = 2 + kMaxNZ * (kMaxNZ + 1) / 2;  // logdet, Q, gram
 
-// One neighbor-system pass over sites [0, n): B, F of one theta and,
-// when E != nullptr, the residual vectors e_j(i) = Z(i,j) - B_i . Z(N(i),j).
+// One neighbor-system pass over sites [0, n): B, F of one theta, and the
+// per-site terms summed per block: column 0 sum_i log F_i, column 1 zero,
+// then the lower triangle (row by row) of sum_i e_i e_i^T / F_i where e_i is
+// the residual vector (X row, y) - B_i (X, y)_N(i).  Quadratic forms of any
+// r = X beta - y follow from these cross-products.
 struct BfArgs {
   const double2 *xy;     // n interleaved coordinates
-  const int *nnIndx;     // flat neighbor index
+  const int *nnIndx;     // flat neighbor index (+64 readable padding)
   const int *lu;         // nnIndxLU, 2n
   int n;
   int m;                 // n.neighbors (max count of any site)
   double *B;             // nIndx, or nullptr: not stored
-  double *F;             // n
+  double *F;             // n, or nullptr: not stored
   const double *Z;       // n x zs row-major: X row then y (nz = p + 1 used)
   int nz;
   int zs;
-  double *E;             // nz x n (E[j*n + i]) or nullptr
   int *err;              // smallest site whose Cholesky failed (INT_MAX = ok)
-  double *rows;          // partial-row buffer (kStatsMax columns x rows_cap)
-  int rows_cap;          // or nullptr when the kernel does not add sums
+  double *rows;          // partial rows, column k at rows[k*rows_cap + r],
+  int rows_cap;          // or nullptr when no sums are wanted
   CovArgs c;
+  int variant;           // cov_variant(c.cov, c.nu): templated kernel, or -1
+  // Batched passes (ng > 0): gridDim.y problems, problem j = fold j / ng at
+  // grid point j % ng, whose covariance parameters are covs[j % ng]; fold f
+  // reads xy + f*fs_xy, nnIndx + f*fs_nn, lu + f*fs_lu, Z + f*fs_Z; problem
+  // j writes rows + j*ps_rows and err[j].  B and F are not stored.
+  int ng = 0;
+  const CovArgs *covs = nullptr;
+  long long fs_xy = 0, fs_nn = 0, fs_lu = 0, fs_Z = 0, ps_rows = 0;
+  int coop_blocks = 0;   // set by bf_launch
 };
 
-// Per-site terms of a pass, summed: column 0 sum_i log F_i, column 1 zero,
-// then the lower triangle (row by row) of sum_i e_i e_i^T / F_i where e_i is
-// the residual vector (X row, y) - B_i (X, y)_N(i).  Quadratic forms of any
-// r = X beta - y follow from these cross-products.
-struct StatsArgs {
-  const double *E;       // nz x n, or nullptr (then only sum log F)
-  const double *F;
-  int n;
-  int nz;                // p + 1
-  double *rows;          // partial rows, kStatsMax x rows_cap
-  int rows_cap;
-  double *out;           // sums (device)
-  int *err;              // optional: the updateBF failure flag; its value is
-  double *err_out;       // copied to *err_out and the flag reset to INT_MAX
+// Launch shape of a pass: family 0 = one site per lane (threads 64, 128 or
+// 256 per block), family 1 = one site per wavefront (threads = 64 x waves,
+// waves 1, 2 or 4, each wave an LDS tile of one system).
+struct BfConfig {
+  int family = 0;
+  int threads = 256;
+};
+
+struct BfTuneReport {
+  int n = 0;             // candidates timed
+  int family[8], threads[8];
+  double ms[8];          // mean launch time of each
+  BfConfig best;
+  double best_ms = 0.0;
 };
 
 // Plain Q(u, v) of src/util.cpp:261-280 over stored B and F.
@@ -55,16 +66,31 @@ struct QArgs {
   double *out;
 };
 
-// Rows a partial-row buffer needs for n sites (times kStatsMax columns).
+// Rows a partial-row buffer needs for n sites under any launch shape.
 int rows_capacity(int n);
-// True when bf_launch adds the site terms itself (per-lane kernel); else the
-// pass stores E and stats_launch adds them.
-bool bf_fuses_sums(int m, int cov);
-int bf_launch(const BfArgs &a, hipStream_t st);
-// Sums of a fused bf_launch (same BfArgs) into out; hands over the flag.
-int bf_sums_launch(const BfArgs &a, double *out, double *err_out,
+bool bf_lane_ok(int m, int variant);
+bool bf_config_ok(const BfConfig &cfg, int m, int variant);
+// Per-lane kernel at 256 threads when it applies, else the widest LDS tile.
+BfConfig bf_default(int m, int variant);
+// Partial rows one problem of a pass writes under cfg.
+int bf_rows(const BfArgs &a, const BfConfig &cfg);
+// nprob problems (1 unless a.ng > 0).  -1: cfg does not apply to (m,
+// variant); -2: bad batch or rows_cap too small; else a hipError_t.
+int bf_launch(const BfArgs &a, const BfConfig &cfg, int nprob,
+              hipStream_t st);
+// Sums of the bf_launch with the same arguments: problem j's column k to
+// out[j*out_stride + k]; its failure flag to err_out[j] (and reset).
+int bf_sums_launch(const BfArgs &a, const BfConfig &cfg, int nprob,
+                   double *out, int out_stride, double *err_out,
                    hipStream_t st);
-int stats_launch(const StatsArgs &a, hipStream_t st);
+// Column sums of nprob blocks of partial rows (see k_rows_sum).
+int rows_sum_launch(const double *rows, int cap, int nrows, long long ps_rows,
+                    int nv, int nprob, double *out, int out_stride, int *err,
+                    double *err_out, hipStream_t st);
+// Time every applicable launch shape of this pass and keep the fastest
+// (cached per (m, variant, nz, size class) unless rep asks for timings).
+int bf_autotune(const BfArgs &a, int nprob, hipStream_t st, BfConfig *best,
+                BfTuneReport *rep);
 int q_launch(const QArgs &a, hipStream_t st);
 
 // *p = v on the stream (no memset node, safe under graph capture).
@@ -94,7 +120,7 @@ struct KnnArgs {
   int n, m;
   int sorted;            // 1: NNGP order equals the x order
   int *nnIndx;           // outputs in the reference layout
-  double *nnDist;
+  double *nnDist;        // or nullptr: not stored
   int *lu;
 };
 int knn_launch(const KnnArgs &a, hipStream_t st);

@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kKnnBlock) void k_knn_pred(KnnArgs A) {
     if (k >= M - cnt) {
       const int o = off + k - (M - cnt);
       A.nnIndx[o] = top.id[k];
-      A.nnDist[o] = sqrt(top.d[k]);
+      if (A.nnDist) A.nnDist[o] = sqrt(top.d[k]);
     }
   }
 }

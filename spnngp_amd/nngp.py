@@ -434,3 +434,87 @@ class Server:
         s = _lib.NngpServerStats()
         check(load().nngp_server_get_stats(self._h, C.byref(s)))
         return {f: getattr(s, f) for f, _ in s._fields_}
+
+
+# ---------------------------------------------------------------------------
+# conjugate k-fold grid search
+# ---------------------------------------------------------------------------
+class CvGrid:
+    """spConjNNGP's k-fold scoring of a theta grid (nngp_cvgrid_*): folds
+    fixed at construction (ho: k x nHo held-out rows, 0-based), every run
+    scores all folds x grid points in one graph launch."""
+
+    def __init__(self, y, X, coords, m, ho, sigmaSqIG, covModel):
+        X = np.asarray(X, dtype=np.float64)
+        self.n, self.p = X.shape
+        ho = np.asarray(ho)
+        self.k, self.nho = ho.shape
+        self.m = m
+        self.cov = cov_index(covModel)
+        self._h = C.c_void_p()
+        opts = _opts()
+        check(load().nngp_cvgrid_create(
+            C.byref(self._h), _f64(y), _cm(X), self.p, self.n, _cm(coords),
+            m, self.k, _i32(ho.reshape(-1)), self.nho, _f64(sigmaSqIG),
+            self.cov, C.byref(opts)))
+
+    def close(self):
+        if self._h:
+            load().nngp_cvgrid_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, thetaAlpha, details=False, predictions=False):
+        """thetaAlpha: g x nTheta (columns phi, alpha[, nu]).  Returns
+        {'scores': g x 2 (rmspe, crps), 'graph_ms', 'bf_config', 'tune'}
+        and, on request, per (fold, point) beta.hat, ab, bB.inv and the
+        held-out predictions (arrays indexed [fold, point, ...])."""
+        ta = np.asarray(thetaAlpha, dtype=np.float64)
+        g = ta.shape[0]
+        k, p, nho = self.k, self.p, self.nho
+        scores = np.zeros(2 * g)
+        d = _lib.NngpCvGridDetail()
+        keep = []
+
+        def buf(size):
+            a = np.zeros(size)
+            keep.append(a)
+            return a.ctypes.data_as(C.c_void_p)
+
+        if details:
+            d.betaHat = buf(p * g * k)
+            d.ab = buf(2 * g * k)
+            d.bBInv = buf(p * p * g * k)
+        if predictions:
+            d.y0Hat = buf(nho * g * k)
+            d.y0HatVar = buf(nho * g * k)
+        check(load().nngp_cvgrid_run(self._h, _f64(ta.reshape(-1)), g, scores,
+                                     C.byref(d)))
+        out = {"scores": scores.reshape(2, g).T.copy(),
+               "graph_ms": d.graph_ms,
+               "bf_config": ("lane" if d.bf_family == 0 else "wave",
+                             d.bf_threads),
+               "tune": [("lane" if d.tune_family[t] == 0 else "wave",
+                         d.tune_threads[t], d.tune_ms[t])
+                        for t in range(d.tune_n)]}
+        i = 0
+        if details:
+            out["beta.hat"] = keep[0].reshape(k, g, p)
+            out["ab"] = keep[1].reshape(k, g, 2)
+            out["bB.inv"] = keep[2].reshape(k, g, p, p).transpose(0, 1, 3, 2)
+            i = 3
+        if predictions:
+            out["y.0.hat"] = keep[i].reshape(k, g, nho)
+            out["y.0.hat.var"] = keep[i + 1].reshape(k, g, nho)
+        return out
