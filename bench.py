@@ -57,6 +57,59 @@ def make_data(n, m, seed):
     return c, X, y, idx, knn_s
 
 
+def mcmc_bench(c, X, y, idx, m, steps, warmup, start, precision):
+    """Warm up, autotune the updateBF launch shape at the chain's current
+    theta, then time `steps` MH iterations with the data resident."""
+    import spnngp_amd as nn
+    md = nn.Model(c, idx["nnIndx"], idx["nnIndxLU"], m, "exponential", X, y,
+                  precision=precision)
+    w = md.rNNGP(nSamples=warmup, seed=11, **start)
+    th_last = w["p.theta.samples"][:, -1]
+    start_t = dict(start, sigmaSqStarting=th_last[0], tauSqStarting=th_last[1],
+                   phiStarting=th_last[2],
+                   betaStarting=w["p.beta.samples"][:, -1])
+    cand, chosen = md.autotune(th_last)
+    md.profile(True)
+    md.timing()  # drains the stream: nothing in flight when the clock starts
+    t0 = time.perf_counter()
+    out = md.rNNGP(nSamples=steps, seed=12, **start_t)
+    tm = md.timing()  # synchronizes the model's stream
+    dt = time.perf_counter() - t0
+    md.close()
+    return {"iters_s": steps / dt, "dt": dt, "tm": tm, "out": out,
+            "beta_mean": out["p.beta.samples"].mean(1),
+            "theta_mean": out["p.theta.samples"].mean(1),
+            "tune": {"candidates": [list(t) for t in cand],
+                     "chosen": list(chosen)}}
+
+
+def mixed_errors(c, X, y, idx, m, beta):
+    """Mixed-precision pass against the oracle's FP64 updateBF and Q at the
+    generating theta: max |dB|, max |dF| and the log-likelihood error."""
+    import spnngp_amd as nn
+    from oracle import oracle as O
+    n = len(y)
+    theta = [2.0, 0.2, 12.0]
+    B0, F0, ld0 = O.updateBF(c.T.reshape(-1).copy(), idx["nnIndx"],
+                             idx["nnIndxLU"], n, m, theta, 0,
+                             nThreads=cores())
+    r = X @ beta - y
+    q0 = O.Q(B0, F0, r, r, idx["nnIndx"], idx["nnIndxLU"])
+    ll0 = -0.5 * ld0 - 0.5 * q0
+    res = {}
+    for name, prec in (("mixed", 1), ("fp64", 0)):
+        with nn.Model(c, idx["nnIndx"], idx["nnIndxLU"], m, "exponential", X,
+                      y, precision=prec) as md:
+            o = md.update(theta, slot=0, beta=beta)
+            B, F = md.get_bf(0)
+        ll = -0.5 * o["logDet"] - 0.5 * o["qResid"]
+        res[name] = {"max_abs_dB": float(np.max(np.abs(B - B0))),
+                     "max_abs_dF": float(np.max(np.abs(F - F0))),
+                     "max_rel_dF": float(np.max(np.abs(F - F0) / F0)),
+                     "loglik_rel_err": float(abs(ll - ll0) / abs(ll0))}
+    return res
+
+
 def cores():
     try:
         avail = len(os.sched_getaffinity(0))
@@ -310,27 +363,27 @@ def main():
                  tauSqStarting=0.2, phiStarting=12.0, nuStarting=0.0,
                  sigmaSqTuning=0.02, tauSqTuning=0.02, phiTuning=0.02,
                  nuTuning=0.0)
-    md = nn.Model(c, idx["nnIndx"], idx["nnIndxLU"], m, "exponential", X, y)
-    print(f"[bench] warmup {args.warmup}", file=sys.stderr, flush=True)
-    w = md.rNNGP(nSamples=args.warmup, seed=11, **start)
-    th_last = w["p.theta.samples"][:, -1]
-    start_t = dict(start, sigmaSqStarting=th_last[0], tauSqStarting=th_last[1],
-                   phiStarting=th_last[2],
-                   betaStarting=w["p.beta.samples"][:, -1])
-    md.profile(True)
-    md.timing()  # drains the stream: nothing in flight when the clock starts
-    print(f"[bench] timing {args.steps} MH iterations", file=sys.stderr,
+    print(f"[bench] FP64 chain, warmup {args.warmup}", file=sys.stderr,
           flush=True)
-    t0 = time.perf_counter()
-    out = md.rNNGP(nSamples=args.steps, seed=12, **start_t)
-    tm = md.timing()  # synchronizes the model's stream
-    dt = time.perf_counter() - t0
-    iters_s = args.steps / dt
+    f64 = mcmc_bench(c, X, y, idx, m, args.steps, args.warmup, start, 0)
+    print("[bench] mixed chain", file=sys.stderr, flush=True)
+    mix = mcmc_bench(c, X, y, idx, m, args.steps, args.warmup, start, 1)
+    mix_err = mixed_errors(c, X, y, idx, m, f64["beta_mean"])
+    iters_s, dt, tm, out = f64["iters_s"], f64["dt"], f64["tm"], f64["out"]
     bf_ms = tm["bf_ms"] / max(tm["bf_launches"], 1)
     flops = sum(f_alg(min(i, m)) for i in range(1, min(n, m))) \
         + (n - m) * f_alg(m)
     achieved = flops / (bf_ms * 1e-3) / 1e12
-    md.close()
+    mixed = {"iters_s": mix["iters_s"],
+             "speedup_vs_fp64": mix["iters_s"] / iters_s,
+             "updateBF_ms": mix["tm"]["bf_ms"] / max(mix["tm"]["bf_launches"],
+                                                      1),
+             "acceptance": mix["out"]["accept"] / args.steps,
+             "posterior_mean_diff": {
+                 "beta": (mix["beta_mean"] - f64["beta_mean"]).tolist(),
+                 "theta": (mix["theta_mean"] - f64["theta_mean"]).tolist()},
+             "errors_vs_oracle": mix_err,
+             "bf_tune": mix["tune"]}
 
     print("[bench] cpu baseline", file=sys.stderr, flush=True)
     cpu = cpu_baseline(c, X, y, idx, m, args.cpu_iters, start) \
@@ -359,7 +412,8 @@ def main():
                      "bound_note": "FP64 compute: the kernel runs on the "
                                    "vector FMA units, whose FP64 peak equals "
                                    "the FP64 matrix peak (78.6 TFLOP/s)",
-                     "kernel": "updateBF (k_bf_lane<15,exp>)",
+                     "kernel": "updateBF (k_bf_lane<15,exp>, %s %d threads)"
+                               % tuple(f64["tune"]["chosen"]),
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
                      "traffic": None,
@@ -374,7 +428,8 @@ def main():
         "cpu_baseline": cpu,
         "kriging": krig,
         "conj_grid": grid,
-        "mixed": None,
+        "mixed": mixed,
+        "bf_tune": f64["tune"],
         "knn_s": knn_s,
     }
     print(json.dumps(line), flush=True)

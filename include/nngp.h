@@ -259,6 +259,26 @@ int nngp_model_rNNGP(nngp_model *model, const double *sigmaSqIG,
 int nngp_model_profile(nngp_model *model, int enable);
 int nngp_model_timing(nngp_model *model, double *t);
 
+/* Launch shapes of the updateBF pass timed by the autotuner. */
+typedef struct nngp_tune_report {
+  int n;                      /* candidates timed                         */
+  int family[8];              /* 0: one site per lane, 1: per wavefront   */
+  int threads[8];             /* threads per block                        */
+  int mode[8];                /* NNGP_MIXED models: 1 = covariance copy
+                                 kept in LDS for the refinement residual  */
+  double ms[8];               /* mean launch time                         */
+  int best_family, best_threads, best_mode;
+} nngp_tune_report;
+
+/* Time every launch shape of this model's updateBF pass at theta (sites per
+ * wavefront 64 or 1; 64/128/256-thread blocks or 1/2/4-system LDS tiles)
+ * and use the fastest for every later pass.  Opt-in: the shape fixes the
+ * order of the per-block sums, so a chain is bit-reproducible for a fixed
+ * shape; without this call the shape is fixed (256-thread blocks where the
+ * per-lane kernel applies). */
+int nngp_model_autotune(nngp_model *model, const double *theta,
+                        nngp_tune_report *report);
+
 /* ------------------------------------------------------------------------
  * Kriging server: rNNGPPredict (src/rNNGPPredict.cpp:25-222) together with
  * its nn2 neighbor query (R/generics.R:642) as a resident service.  The

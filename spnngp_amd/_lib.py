@@ -39,6 +39,7 @@ EXPORTS = (
     "nngp_server_wait", "nngp_server_release", "nngp_server_predict",
     "nngp_server_get_stats",
     "nngp_cvgrid_create", "nngp_cvgrid_run", "nngp_cvgrid_destroy",
+    "nngp_model_autotune",
 )
 
 _RNG_FN = C.CFUNCTYPE(C.c_double, C.c_void_p)
@@ -91,6 +92,13 @@ class NngpCvGridDetail(C.Structure):
                 ("bf_family", C.c_int), ("bf_threads", C.c_int),
                 ("tune_n", C.c_int), ("tune_family", C.c_int * 8),
                 ("tune_threads", C.c_int * 8), ("tune_ms", C.c_double * 8)]
+
+
+class NngpTuneReport(C.Structure):
+    _fields_ = [("n", C.c_int), ("family", C.c_int * 8),
+                ("threads", C.c_int * 8), ("mode", C.c_int * 8),
+                ("ms", C.c_double * 8), ("best_family", C.c_int),
+                ("best_threads", C.c_int), ("best_mode", C.c_int)]
 
 
 class NngpError(RuntimeError):
@@ -178,6 +186,7 @@ def load():
     L.nngp_cvgrid_run.argtypes = [_vp, _dp, _i, _dp,
                                   C.POINTER(NngpCvGridDetail)]
     L.nngp_cvgrid_destroy.argtypes = [_vp]
+    L.nngp_model_autotune.argtypes = [_vp, _dp, C.POINTER(NngpTuneReport)]
     L.nngp_cvgrid_destroy.restype = None
     _lib = L
     return _lib

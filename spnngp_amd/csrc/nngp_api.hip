@@ -902,6 +902,30 @@ int nngp_model_timing(nngp_model *model, double *t) {
   return NNGP_OK;
 }
 
+int nngp_model_autotune(nngp_model *model, const double *theta,
+                        nngp_tune_report *report) {
+  clear_err();
+  if (!model || !theta) return set_err(NNGP_ERR_ARG, "bad arguments");
+  Model &md = model->md;
+  const CovArgs c = make_cov(md.cov, theta[0], theta[1], theta[2],
+                             md.cov == 2 ? theta[3] : 0.0);
+  BfTuneReport rep;
+  NNGP_TRY(md.autotune(c, &rep));
+  if (report) {
+    report->n = rep.n;
+    for (int k = 0; k < 8; k++) {
+      report->family[k] = k < rep.n ? rep.family[k] : 0;
+      report->threads[k] = k < rep.n ? rep.threads[k] : 0;
+      report->mode[k] = k < rep.n ? rep.mode[k] : 0;
+      report->ms[k] = k < rep.n ? rep.ms[k] : 0.0;
+    }
+    report->best_family = rep.best.family;
+    report->best_threads = rep.best.threads;
+    report->best_mode = rep.best.mode;
+  }
+  return NNGP_OK;
+}
+
 int nngp_model_get_bf(nngp_model *model, int slot, double *B, double *F) {
   clear_err();
   if (!model || slot < 0 || slot > 1) return set_err(NNGP_ERR_ARG, "bad arguments");

@@ -253,6 +253,149 @@ __global__ __launch_bounds__(kLaneMaxThreads) void k_bf_lane(BfArgs A) {
 }
 
 // ---------------------------------------------------------------------------
+// mixed precision, one site per lane
+// ---------------------------------------------------------------------------
+// The covariance is built in FP32 from FP32 neighbor offsets (FP64
+// differences to the site, rounded), factored and solved in FP32 for B0.
+// One refinement step forms the residual r = c - C B0 of that FP32 system
+// in FP64 arithmetic (each FP32 x FP64 product is exact before the sum),
+// solves C d = r with the FP32 factor and sets B = B0 + d, which removes
+// the factorization's error.  F = s2 + t2 - c.B and every sum stay in FP64.
+// What remains is the FP32 rounding of the covariance entries themselves.
+// COPY = true keeps C in LDS (lane-interleaved) for the residual; false
+// evaluates the entries a second time (bit-identically) to save LDS.
+template <int COV>
+__device__ __forceinline__ float pair32(float dxk, float dyk, float dxl,
+                                        float dyl, const CovArgs32 &c) {
+  const float ex = dxk - dxl, ey = dyk - dyl;
+  return s2_corr32<COV>(fmaf(ex, ex, ey * ey), c);
+}
+
+template <int M, int COV, bool COPY>
+__global__ __launch_bounds__(kLaneMaxThreads, COPY ? 1 : 2) void k_bf_mixed(BfArgs A) {
+  extern __shared__ float cl[];  // COPY: entry t of lane L at cl[t*nt + L]
+  const int nt = (int)blockDim.x;
+  const int b = xcd_block((A.n + nt - 1) / nt);
+  if (b < 0) return;
+  const CovArgs &c = A.c;
+  const CovArgs32 c32 = cov32(c);
+  const int L = (int)threadIdx.x;
+  const int i = b * nt + L;
+  const bool valid = i < A.n;
+  double Fi = 1.0;
+  double e[kMaxNZ];
+#pragma unroll
+  for (int j = 0; j < kMaxNZ; j++) e[j] = 0.0;
+
+  if (valid) {
+    const int cnt = A.lu[A.n + i];
+    const int off = A.lu[i];
+    if (cnt == 0) {
+      Fi = c.diag;
+#pragma unroll
+      for (int j = 0; j < kMaxNZ; j++)
+        if (j < A.nz) e[j] = A.Z[(size_t)i * A.zs + j];
+    } else {
+      const double2 s = A.xy[i];
+      // as in k_bf_lane: padding entries of the index read as site 0, then
+      // replaced by far-away finite offsets (every product with one is +inf
+      // or 0 in FP32, so its correlations are exactly 0)
+      int nb[M];
+#pragma unroll
+      for (int k = 0; k < M; k++) nb[k] = A.nnIndx[off + k];
+      if (cnt < M) {
+#pragma unroll
+        for (int k = 0; k < M; k++)
+          if (k >= cnt) nb[k] = 0;
+      }
+      float dx[M], dy[M];
+#pragma unroll
+      for (int k = 0; k < M; k++) {
+        const double2 q = A.xy[nb[k]];
+        dx[k] = (float)(q.x - s.x);
+        dy[k] = (float)(q.y - s.y);
+      }
+      if (cnt < M) {
+#pragma unroll
+        for (int k = 0; k < M; k++) {
+          if (k >= cnt) {
+            dx[k] = 1e30f * (k + 1);
+            dy[k] = -1e30f * (k + 1);
+          }
+        }
+      }
+      constexpr int T = M * (M + 1) / 2;
+      float a[T], v[M];
+#pragma unroll
+      for (int k = 0; k < M; k++) {
+        v[k] = s2_corr32<COV>(fmaf(dx[k], dx[k], dy[k] * dy[k]), c32);
+        if (COPY) cl[(T + k) * nt + L] = v[k];
+#pragma unroll
+        for (int l = 0; l < k; l++) {
+          a[tri(k, l)] = pair32<COV>(dx[k], dy[k], dx[l], dy[l], c32);
+          if (COPY) cl[tri(k, l) * nt + L] = a[tri(k, l)];
+        }
+        a[tri(k, k)] = c32.diag;
+      }
+      const bool ok = chol_factor32<M>(a);
+      chol_apply32<M>(a, v);  // v <- B0
+      double bd[M], r[M];
+      const double dg = (double)c32.diag;
+#pragma unroll
+      for (int k = 0; k < M; k++) {
+        bd[k] = (double)v[k];
+        const float ck =
+            COPY ? cl[(T + k) * nt + L]
+                 : s2_corr32<COV>(fmaf(dx[k], dx[k], dy[k] * dy[k]), c32);
+        r[k] = fma(-dg, bd[k], (double)ck);
+      }
+#pragma unroll
+      for (int k = 1; k < M; k++) {
+#pragma unroll
+        for (int l = 0; l < k; l++) {
+          const double ckl =
+              COPY ? (double)cl[tri(k, l) * nt + L]
+                   : (double)pair32<COV>(dx[k], dy[k], dx[l], dy[l], c32);
+          r[k] = fma(-ckl, bd[l], r[k]);
+          r[l] = fma(-ckl, bd[k], r[l]);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < M; k++) v[k] = (float)r[k];
+      chol_apply32<M>(a, v);  // v <- d
+      double cb = 0.0;
+#pragma unroll
+      for (int k = 0; k < M; k++) {
+        bd[k] += (double)v[k];
+        const float ck =
+            COPY ? cl[(T + k) * nt + L]
+                 : s2_corr32<COV>(fmaf(dx[k], dx[k], dy[k] * dy[k]), c32);
+        cb = fma((double)ck, bd[k], cb);
+      }
+      if (A.B) {
+#pragma unroll
+        for (int k = 0; k < M; k++)
+          if (k < cnt) A.B[off + k] = bd[k];
+      }
+      Fi = c.diag - cb;
+#pragma unroll
+      for (int j = 0; j < kMaxNZ; j++) {
+        if (j < A.nz) {
+          double ej = A.Z[(size_t)i * A.zs + j];
+#pragma unroll
+          for (int k = 0; k < M; k++)
+            ej = fma(-bd[k], A.Z[(size_t)nb[k] * A.zs + j], ej);
+          e[j] = ej;
+        }
+      }
+      if (!ok || !(Fi > 0.0)) atomicMin(A.err, i);
+    }
+    if (A.F) A.F[i] = Fi;
+  }
+  if (A.rows) block_site_terms(A.nz, A.rows, A.rows_cap, valid, Fi, e, b);
+}
+
+// ---------------------------------------------------------------------------
 // one site per wavefront (generic m, generic covariance)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int tri_row(int p) {
@@ -483,6 +626,9 @@ static int red_grid(int n) {
 // ---------------------------------------------------------------------------
 bool bf_lane_ok(int m, int variant) { return m <= 16 && variant >= 0; }
 
+static int lane_M(int m);
+static size_t mixed_copy_shm(int m, int threads);
+
 // LDS per block of the wavefront kernel with `waves` waves.
 static size_t coop_shm(int m, int waves) {
   return (size_t)waves * coop_wave_doubles(m) * sizeof(double);
@@ -491,7 +637,9 @@ static size_t coop_shm(int m, int waves) {
 bool bf_config_ok(const BfConfig &cfg, int m, int variant) {
   if (cfg.family == 0)
     return bf_lane_ok(m, variant) &&
-           (cfg.threads == 64 || cfg.threads == 128 || cfg.threads == 256);
+           (cfg.threads == 64 || cfg.threads == 128 || cfg.threads == 256) &&
+           (cfg.mode == 0 ||
+            (cfg.mode == 1 && mixed_copy_shm(m, cfg.threads) <= 65536));
   const int waves = cfg.threads / 64;
   return m <= kCoopMaxM && cfg.threads % 64 == 0 && waves >= 1 &&
          waves <= kCoopMaxWaves && coop_shm(m, waves) <= 65536;
@@ -530,6 +678,52 @@ static void launch_lane_m(const BfArgs &a, int variant, const dim3 &grid,
   }
 }
 
+// Per-lane kernel size for m neighbors.
+static int lane_M(int m) {
+  return m <= 4 ? 4 : m <= 8 ? 8 : m <= 10 ? 10 : m <= 12 ? 12 : m <= 15 ? 15
+                                                                          : 16;
+}
+
+// LDS of the mixed kernel's covariance copy: (M(M+1)/2 + M) floats a lane.
+static size_t mixed_copy_shm(int m, int threads) {
+  const int M = lane_M(m);
+  return (size_t)threads * (M * (M + 1) / 2 + M) * sizeof(float);
+}
+
+template <int M, bool COPY>
+static void launch_mixed_mc(const BfArgs &a, int variant, const dim3 &grid,
+                            const dim3 &blk, size_t shm, hipStream_t st) {
+  switch (variant) {
+    case 0: hipLaunchKernelGGL((k_bf_mixed<M, 0, COPY>), grid, blk, shm, st, a); break;
+    case 1: hipLaunchKernelGGL((k_bf_mixed<M, 1, COPY>), grid, blk, shm, st, a); break;
+    case kCovMatern15: hipLaunchKernelGGL((k_bf_mixed<M, kCovMatern15, COPY>), grid, blk, shm, st, a); break;
+    case kCovMatern25: hipLaunchKernelGGL((k_bf_mixed<M, kCovMatern25, COPY>), grid, blk, shm, st, a); break;
+    default: hipLaunchKernelGGL((k_bf_mixed<M, 3, COPY>), grid, blk, shm, st, a); break;
+  }
+}
+
+template <int M>
+static void launch_mixed_m(const BfArgs &a, int variant, bool copy,
+                           const dim3 &grid, const dim3 &blk, hipStream_t st) {
+  if (copy)
+    launch_mixed_mc<M, true>(a, variant, grid, blk,
+                             mixed_copy_shm(a.m, (int)blk.x), st);
+  else
+    launch_mixed_mc<M, false>(a, variant, grid, blk, 0, st);
+}
+
+static void launch_mixed(const BfArgs &a, int variant, bool copy,
+                         const dim3 &grid, const dim3 &blk, hipStream_t st) {
+  switch (lane_M(a.m)) {
+    case 4: launch_mixed_m<4>(a, variant, copy, grid, blk, st); break;
+    case 8: launch_mixed_m<8>(a, variant, copy, grid, blk, st); break;
+    case 10: launch_mixed_m<10>(a, variant, copy, grid, blk, st); break;
+    case 12: launch_mixed_m<12>(a, variant, copy, grid, blk, st); break;
+    case 15: launch_mixed_m<15>(a, variant, copy, grid, blk, st); break;
+    default: launch_mixed_m<16>(a, variant, copy, grid, blk, st); break;
+  }
+}
+
 template <bool BATCH>
 static void launch_lane(const BfArgs &a, int variant, const dim3 &grid,
                         const dim3 &blk, hipStream_t st) {
@@ -555,6 +749,7 @@ int bf_launch(const BfArgs &a0, const BfConfig &cfg, int nprob,
   const dim3 blk(cfg.threads);
   if (cfg.family == 0) {
     if (batch) launch_lane<true>(a, variant, grid, blk, st);
+    else if (a.mixed) launch_mixed(a, variant, cfg.mode == 1, grid, blk, st);
     else launch_lane<false>(a, variant, grid, blk, st);
   } else {
     a.coop_blocks = nrows;
@@ -594,7 +789,7 @@ int bf_autotune(const BfArgs &a, int nprob, hipStream_t st, BfConfig *best,
   const long long work = (long long)a.n * nprob;
   int cls = 0;
   while ((1LL << (cls + 1)) <= work) cls++;
-  const auto key = std::make_tuple(a.m, variant, a.nz, cls);
+  const auto key = std::make_tuple(a.m, variant * 4 + a.mixed, a.nz, cls);
   if (!rep) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune_cache.find(key);
@@ -604,8 +799,13 @@ int bf_autotune(const BfArgs &a, int nprob, hipStream_t st, BfConfig *best,
     }
   }
   std::vector<BfConfig> cand;
-  for (int th : {64, 128, 256}) cand.push_back(BfConfig{0, th});
-  for (int w : {1, 2, 4}) cand.push_back(BfConfig{1, 64 * w});
+  if (a.mixed) {  // the mixed kernels: block size x covariance copy in LDS
+    for (int th : {64, 128, 256})
+      for (int mode : {0, 1}) cand.push_back(BfConfig{0, th, mode});
+  } else {
+    for (int th : {64, 128, 256}) cand.push_back(BfConfig{0, th, 0});
+    for (int w : {1, 2, 4}) cand.push_back(BfConfig{1, 64 * w, 0});
+  }
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess) return -1;
   if (hipEventCreate(&e1) != hipSuccess) {
@@ -632,6 +832,7 @@ int bf_autotune(const BfArgs &a, int nprob, hipStream_t st, BfConfig *best,
     if (rep && nc < 8) {
       rep->family[nc] = cfg.family;
       rep->threads[nc] = cfg.threads;
+      rep->mode[nc] = cfg.mode;
       rep->ms[nc] = per;
     }
     nc++;

@@ -255,6 +255,94 @@ __device__ __forceinline__ bool chol_solve(double (&a)[M * (M + 1) / 2],
   return ok;
 }
 
+// ---------------------------------------------------------------------------
+// FP32 pieces of the mixed-precision updateBF
+// ---------------------------------------------------------------------------
+// Covariance parameters rounded to FP32 once per launch.
+struct CovArgs32 {
+  float s2, phi, inv_phi, phi2, diag;
+};
+
+__device__ __forceinline__ CovArgs32 cov32(const CovArgs &a) {
+  CovArgs32 c;
+  c.s2 = (float)a.s2;
+  c.phi = (float)a.phi;
+  c.inv_phi = (float)a.inv_phi;
+  c.phi2 = (float)a.phi2;
+  c.diag = (float)a.diag;
+  return c;
+}
+
+// s2 * rho in FP32 from an FP32 squared distance (hardware exp2 and sqrt,
+// about 1 ulp each).
+template <int COV>
+__device__ __forceinline__ float s2_corr32(float d2, const CovArgs32 &a) {
+  constexpr float kLog2e = 1.4426950408889634f;
+  if constexpr (COV == 0) {
+    return a.s2 * __builtin_amdgcn_exp2f(-a.phi * __builtin_amdgcn_sqrtf(d2) * kLog2e);
+  } else if constexpr (COV == 1) {
+    const float D = __builtin_amdgcn_sqrtf(d2);
+    const float t = a.phi * D;
+    const float v = a.s2 * (1.0f - 1.5f * t + 0.5f * (t * t * t));
+    return (D > 0.0f && D <= a.inv_phi) ? v : (D >= a.inv_phi ? 0.0f : a.s2);
+  } else if constexpr (COV == kCovMatern15) {
+    const float x = __builtin_amdgcn_sqrtf(d2) * a.phi;
+    if (!(x < 100.0f)) return 0.0f;  // e^-x underflows FP32; keeps inf out
+    return a.s2 * __builtin_amdgcn_exp2f(-x * kLog2e) * (1.0f + x);
+  } else if constexpr (COV == kCovMatern25) {
+    const float x = __builtin_amdgcn_sqrtf(d2) * a.phi;
+    if (!(x < 100.0f)) return 0.0f;
+    return a.s2 * __builtin_amdgcn_exp2f(-x * kLog2e) *
+           (1.0f + x + x * x * (1.0f / 3.0f));
+  } else {
+    return a.s2 * __builtin_amdgcn_exp2f(-a.phi2 * d2 * kLog2e);
+  }
+}
+
+// FP32 Cholesky of a packed lower triangle (diagonal kept as 1/L_jj);
+// false when a pivot is not positive.
+template <int M>
+__device__ __forceinline__ bool chol_factor32(float (&a)[M * (M + 1) / 2]) {
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < M; j++) {
+    const float piv = a[tri(j, j)];
+    ok = ok && (piv > 0.0f);
+    const float r = __builtin_amdgcn_rsqf(piv);
+    a[tri(j, j)] = r;
+#pragma unroll
+    for (int k = j + 1; k < M; k++) a[tri(k, j)] *= r;
+#pragma unroll
+    for (int k = j + 1; k < M; k++) {
+      const float lkj = a[tri(k, j)];
+#pragma unroll
+      for (int l = j + 1; l <= k; l++)
+        a[tri(k, l)] = fmaf(-lkj, a[tri(l, j)], a[tri(k, l)]);
+    }
+  }
+  return ok;
+}
+
+// v <- C^-1 v with the factor of chol_factor32.
+template <int M>
+__device__ __forceinline__ void chol_apply32(const float (&a)[M * (M + 1) / 2],
+                                             float (&v)[M]) {
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    float t = v[k];
+#pragma unroll
+    for (int l = 0; l < k; l++) t = fmaf(-a[tri(k, l)], v[l], t);
+    v[k] = t * a[tri(k, k)];
+  }
+#pragma unroll
+  for (int k = M - 1; k >= 0; k--) {
+    const float t = v[k] * a[tri(k, k)];
+    v[k] = t;
+#pragma unroll
+    for (int l = 0; l < k; l++) v[l] = fmaf(-a[tri(k, l)], t, v[l]);
+  }
+}
+
 // Philox4x32-10 (Salmon, Moraes, Dror & Shaw, "Parallel random numbers: as
 // easy as 1, 2, 3", SC'11): ten rounds of the two 32x32->64 multiplies with
 // the published multipliers and Weyl key increments.

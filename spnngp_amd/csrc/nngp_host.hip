@@ -338,7 +338,13 @@ int Model::bf(const CovArgs &c, int slot, bool storeB) {
   a.rows_cap = rows_capacity(n);
   a.c = c;
   a.variant = cov_variant(c.cov, c.nu);
-  const BfConfig cfg = bf_default(m, a.variant);
+  a.mixed = precision == NNGP_MIXED ? 1 : 0;
+  BfConfig cfg = bf_default(m, a.variant);
+  if (a.mixed) {  // untuned mixed passes keep the covariance copy in LDS
+    const BfConfig copy{0, 64, 1};
+    if (bf_config_ok(copy, m, a.variant)) cfg = copy;
+  }
+  if (have_tuned && bf_config_ok(tuned_cfg, m, a.variant)) cfg = tuned_cfg;
   last_bf = a;
   last_cfg = cfg;
   const int ev = prof ? ev_begin(0) : -1;
@@ -348,6 +354,36 @@ int Model::bf(const CovArgs &c, int slot, bool storeB) {
     return set_err(NNGP_ERR_UNSUPPORTED, "n.neighbors = %d exceeds 64", m);
   if (rc) return set_err(NNGP_ERR_HIP, "updateBF launch failed: %s",
                          hipGetErrorString((hipError_t)rc));
+  return NNGP_OK;
+}
+
+int Model::autotune(const CovArgs &c, BfTuneReport *rep) {
+  BfArgs a;
+  a.xy = xy.as<double2>();
+  a.nnIndx = nnIndx.as<int>();
+  a.lu = lu.as<int>();
+  a.n = n;
+  a.m = m;
+  a.B = nullptr;
+  a.F = F[0].as<double>();
+  a.Z = nz ? Z.as<double>() : nullptr;
+  a.nz = nz;
+  a.zs = zs;
+  a.err = err.as<int>();
+  a.rows = rows.as<double>();
+  a.rows_cap = rows_capacity(n);
+  a.c = c;
+  a.variant = cov_variant(c.cov, c.nu);
+  a.mixed = precision == NNGP_MIXED ? 1 : 0;
+  BfConfig best;
+  const int rc = bf_autotune(a, 1, st.s, &best, rep);
+  if (rc) return set_err(NNGP_ERR_HIP, "autotune launch failed (%d)", rc);
+  // the timing passes may have flagged a failure: start clean
+  if (fill_int_launch(err.as<int>(), INT_MAX, st.s))
+    return set_err(NNGP_ERR_HIP, "flag reset kernel failed");
+  NNGP_HIP(hipStreamSynchronize(st.s));
+  tuned_cfg = best;
+  have_tuned = true;
   return NNGP_OK;
 }
 

@@ -191,7 +191,12 @@ struct Model {
   PinnedBuf hsums;  // host mirror of sums
   BfArgs last_bf;      // arguments of the last bf() launch
   BfConfig last_cfg;   // and its launch shape
+  BfConfig tuned_cfg;  // launch shape chosen by autotune(), when have_tuned
+  bool have_tuned = false;
   PassOut slot_out[2]; // sums of the last completed pass into each slot
+
+  // Time the launch shapes of the pass at c and use the fastest from now on.
+  int autotune(const CovArgs &c, BfTuneReport *rep);
 
   int create(int n, int m, int p, const double *coords, const int *nnIndx,
              const int *nnIndxLU, const double *X, const double *y, int cov,

@@ -31,6 +31,8 @@ struct BfArgs {
   int rows_cap;          // or nullptr when no sums are wanted
   CovArgs c;
   int variant;           // cov_variant(c.cov, c.nu): templated kernel, or -1
+  int mixed = 0;         // 1: FP32 factor/solve + one FP64 refinement step
+                         // (per-lane shape, single pass only; else FP64)
   // Batched passes (ng > 0): gridDim.y problems, problem j = fold j / ng at
   // grid point j % ng, whose covariance parameters are covs[j % ng]; fold f
   // reads xy + f*fs_xy, nnIndx + f*fs_nn, lu + f*fs_lu, Z + f*fs_Z; problem
@@ -47,11 +49,12 @@ struct BfArgs {
 struct BfConfig {
   int family = 0;
   int threads = 256;
+  int mode = 0;          // mixed-precision passes: 1 = covariance copy in LDS
 };
 
 struct BfTuneReport {
   int n = 0;             // candidates timed
-  int family[8], threads[8];
+  int family[8], threads[8], mode[8];
   double ms[8];          // mean launch time of each
   BfConfig best;
   double best_ms = 0.0;

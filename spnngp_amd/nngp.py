@@ -318,6 +318,19 @@ class Model:
         return {"bf_ms": t[0], "bf_launches": int(t[1]), "sum_ms": t[2],
                 "sum_launches": int(t[3])}
 
+    def autotune(self, theta):
+        """Time the updateBF launch shapes at theta and keep the fastest.
+        Returns [(kind, threads, ms), ...] and the chosen (kind, threads)."""
+        th = np.zeros(4)
+        th[:len(theta)] = theta
+        r = _lib.NngpTuneReport()
+        check(load().nngp_model_autotune(self._h, th, C.byref(r)))
+        kind = ("lane", "wave")
+        tag = ("", "+lds-copy")
+        return ([(kind[r.family[t]] + tag[r.mode[t]], r.threads[t], r.ms[t])
+                 for t in range(r.n)],
+                (kind[r.best_family] + tag[r.best_mode], r.best_threads))
+
     def qresid(self, beta, slot=0):
         q = C.c_double()
         check(load().nngp_model_qresid(self._h, slot, _f64(beta), C.byref(q)))

@@ -334,3 +334,68 @@ def test_bad_cov_model_is_reported(nngp, oracle):
         nngp.updateBF(c, nn["nnIndx"], nn["nnIndxLU"], 5, [1.0, 0.1, 2.0], 9)
     assert e.value.status == nngp.NNGP_ERR_COV_MODEL
     assert "cov.model is not correctly specified" in str(e.value)
+
+
+# ---------------------------------------------------------------------------
+# mixed precision (FP32 build/factor/solve + one FP64 refinement step)
+# ---------------------------------------------------------------------------
+# The mixed form refines against its FP32-built covariance, so what remains
+# against the FP64 oracle is the FP32 rounding of the covariance entries,
+# amplified by the conditioning of the neighbor systems: the bounds below
+# are those of that analysis (entries to ~1e-7, kappa ~ 1e2).
+@pytest.mark.gpu
+@pytest.mark.parametrize("cov,m", [(0, 15), (0, 8), (3, 10), (1, 12)])
+def test_mixed_updateBF_close_to_oracle(oracle, nngp, cov, m):
+    print(f"mixed updateBF cov={cov} m={m}", flush=True)
+    n, p = 4000, 2
+    c = uniform_coords(n, 40 + m)
+    rng = np.random.default_rng(m)
+    X = np.column_stack([np.ones(n), rng.normal(size=n)])
+    y = X @ [1.0, -1.0] + rng.normal(size=n)
+    nn = oracle.mkNNIndxCB(c, m)
+    theta = [2.0, 0.2, 12.0 if cov != 1 else 3.0]
+    B0, F0, ld0 = oracle.updateBF(c.T.reshape(-1).copy(), nn["nnIndx"],
+                                  nn["nnIndxLU"], n, m, theta, cov)
+    beta = np.array([1.0, -1.0])
+    r = X @ beta - y
+    q0 = oracle.Q(B0, F0, r, r, nn["nnIndx"], nn["nnIndxLU"])
+    with nngp.Model(c, nn["nnIndx"], nn["nnIndxLU"], m, cov, X, y,
+                    precision=1) as md:
+        out = md.update(theta, slot=0, beta=beta)
+        B, F = md.get_bf(0)
+        # the same pass after tuning (any launch shape) agrees to rounding
+        md.autotune(theta)
+        out2 = md.update(theta, slot=1, beta=beta)
+    ll0 = -0.5 * ld0 - 0.5 * q0
+    ll = -0.5 * out["logDet"] - 0.5 * out["qResid"]
+    assert abs(ll - ll0) / abs(ll0) < 1e-6
+    assert np.max(np.abs(F - F0) / F0) < 1e-5
+    assert np.max(np.abs(B - B0)) / np.max(np.abs(B0)) < 1e-4
+    assert abs(out2["logDet"] - out["logDet"]) < 1e-9 * abs(out["logDet"])
+
+
+@pytest.mark.gpu
+def test_mixed_chain_matches_fp64_chain(oracle, nngp):
+    """Same RNG tape: the mixed chain's posterior summaries equal the FP64
+    chain's far inside Monte Carlo error."""
+    print("mixed chain", flush=True)
+    n, m, S = 3000, 10, 300
+    c = uniform_coords(n, 77)
+    rng = np.random.default_rng(77)
+    X = np.column_stack([np.ones(n), rng.normal(size=n)])
+    y = X @ [1.0, 5.0] + np.sin(5 * c[:, 0]) + rng.normal(size=n)
+    nn = oracle.mkNNIndxCB(c, m)
+    args = dict(sigmaSqIG=[2, 1], tauSqIG=[2, 1], phiUnif=[3, 30],
+                nuUnif=None, betaStarting=[1.0, 5.0], sigmaSqStarting=1.0,
+                tauSqStarting=1.0, phiStarting=6.0, nuStarting=0.0,
+                sigmaSqTuning=0.2, tauSqTuning=0.2, phiTuning=0.2,
+                nuTuning=0.0, nSamples=S, seed=5)
+    res = {}
+    for prec in (0, 1):
+        with nngp.Model(c, nn["nnIndx"], nn["nnIndxLU"], m, 0, X, y,
+                        precision=prec) as md:
+            res[prec] = md.rNNGP(**args)
+    for key in ("p.beta.samples", "p.theta.samples"):
+        a, b = res[0][key], res[1][key]
+        se = a.std(1) / np.sqrt(S)
+        assert np.all(np.abs(a.mean(1) - b.mean(1)) < 0.01 * se + 1e-9)
