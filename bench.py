@@ -28,6 +28,11 @@ sys.path.insert(0, ROOT)
 METRIC = "MCMC iter/s @ n=1e6,m=15 (FP64 vs mixed) + kriging p50/p99 latency"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per launch of the FP64 updateBF pass at n=1e6, m=15: two
+# rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this script, each scaled
+# by what the counter reads for a copy kernel of known size (committed
+# summary: profiles/updateBF_rNNGP_n1e6_m15_pmc_traffic.md)
+BF_TRAFFIC_BYTES = 131_891_335
 
 
 def f_alg(m):
@@ -416,7 +421,12 @@ def main():
                                % tuple(f64["tune"]["chosen"]),
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": None,
+                     "traffic": BF_TRAFFIC_BYTES if (n, m) == (1_000_000, 15)
+                                else None,
+                     "traffic_note": "HBM bytes per launch from rocprofv3 "
+                                     "FETCH_SIZE + WRITE_SIZE, calibrated "
+                                     "on a copy kernel; algorithmic bytes "
+                                     "84 per site (84 MB per launch)",
                      "flops_per_launch": flops,
                      "launch_ms": bf_ms,
                      "launches": tm["bf_launches"]},

@@ -10,6 +10,8 @@ reference's native routines:
     Model                              device-resident fitted data
     Server                             resident kriging service
     CvGrid                             conjugate k-fold grid search
+    spNNGP, spConjNNGP, predict,       the R-level front ends
+    crps, rmspe
 
 There is no CPU fallback: without the built library every call raises.
 """
@@ -20,12 +22,13 @@ from ._lib import (NNGP_ERR_ARG, NNGP_ERR_COV_MODEL, NNGP_ERR_HIP,
 from .nngp import (COV_MODELS, CvGrid, Model, Q, Server, cNNGP, cov_index, device_count,
                    mkNNIndx, mkNNIndxCB, n_indx, nn2, prior_draw, rNNGP,
                    rNNGPPredict, updateBF, updateConjBF)
+from .spnngp import crps, predict, rmspe, spConjNNGP, spNNGP
 
 __all__ = [
     "COV_MODELS", "CvGrid", "Model", "NngpError", "Q", "Server", "cNNGP", "cov_index",
     "device_count", "exported_symbols", "load", "mkNNIndx", "mkNNIndxCB",
     "n_indx", "nn2", "prior_draw", "rNNGP", "rNNGPPredict", "updateBF",
-    "updateConjBF",
+    "updateConjBF", "spNNGP", "spConjNNGP", "predict", "crps", "rmspe",
     "NNGP_OK", "NNGP_ERR_ARG", "NNGP_ERR_NOT_PD", "NNGP_ERR_COV_MODEL",
     "NNGP_ERR_HIP", "NNGP_ERR_NOMEM", "NNGP_ERR_INTERRUPT",
     "NNGP_ERR_UNSUPPORTED", "NNGP_FP64", "NNGP_MIXED",

@@ -1,0 +1,143 @@
+"""The R-level front ends end to end (spNNGP, spConjNNGP, predict), each
+against the oracle run on the same ordered data with the same draws."""
+import numpy as np
+import pytest
+
+RTOL = 1e-10
+
+
+def max_rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def gp_data(n, seed):
+    """The CPU-runnable response case: U[0,1]^2, X = [1, N(0,1)],
+    beta = (1, 5), w an exact exponential GP (sigma.sq 5, phi 6), tau.sq 1."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(size=(n, 2))
+    X = np.column_stack([np.ones(n), rng.normal(size=n)])
+    D = np.sqrt(((c[:, None] - c[None]) ** 2).sum(-1))
+    L = np.linalg.cholesky(5 * np.exp(-6 * D) + 1e-8 * np.eye(n))
+    y = X @ [1.0, 5.0] + L @ rng.normal(size=n) + rng.normal(size=n)
+    return c, X, y
+
+
+@pytest.mark.gpu
+def test_spNNGP_matches_oracle_chain(oracle, nngp):
+    print("spNNGP front end", flush=True)
+    n, m, S = 600, 10, 150
+    c, X, y = gp_data(n, 1)
+    rng = np.random.default_rng(2)
+    zs, us = rng.normal(size=S * 8), rng.uniform(size=S)
+    priors = {"sigma.sq.IG": [2, 5], "tau.sq.IG": [2, 1],
+              "phi.Unif": [3, 300]}
+    starting = {"sigma.sq": 5, "tau.sq": 1, "phi": 6}
+    tuning = {"sigma.sq": 0.5, "tau.sq": 0.5, "phi": 0.5}
+    fit = nngp.spNNGP(y, X, c, starting=starting, tuning=tuning,
+                      priors=priors, n_samples=S, n_neighbors=m,
+                      rng=oracle.TapeRng(zs, us), return_neighbor_info=True)
+    # the oracle on the x-ordered data with the OLS start
+    o = np.argsort(c[:, 0], kind="stable")
+    co, Xo, yo = c[o], X[o], y[o]
+    nn = oracle.mkNNIndxCB(co, m)
+    assert np.array_equal(fit["neighbor.info"]["nn.indx"], nn["nnIndx"])
+    b0 = np.linalg.lstsq(Xo, yo, rcond=None)[0]
+    ref = oracle.rNNGP(yo, Xo, 2, n, m, co.T.reshape(-1).copy(), 0,
+                       nn["nnIndx"], nn["nnIndxLU"], sigmaSqIG=[2, 5],
+                       tauSqIG=[2, 1], phiUnif=[3, 300], nuUnif=None,
+                       betaStarting=b0, sigmaSqStarting=5.0,
+                       tauSqStarting=1.0, phiStarting=6.0, nuStarting=0.0,
+                       sigmaSqTuning=0.5, tauSqTuning=0.5, phiTuning=0.5,
+                       nuTuning=0.0, nSamples=S,
+                       rng=oracle.TapeRng(zs, us))
+    assert max_rel(fit["p.beta.samples"], ref["p.beta.samples"].T) < RTOL
+    assert max_rel(fit["p.theta.samples"], ref["p.theta.samples"].T) < RTOL
+    # results come back in the caller's order
+    assert np.array_equal(fit["coords"], c) and np.array_equal(fit["y"], y)
+
+
+@pytest.mark.gpu
+def test_predict_response_matches_oracle(oracle, nngp):
+    print("predict front end", flush=True)
+    n, m, S, q = 500, 8, 60, 40
+    c, X, y = gp_data(n, 3)
+    rng = np.random.default_rng(4)
+    fit = nngp.spNNGP(y, X, c, starting={"sigma.sq": 5, "tau.sq": 1,
+                                         "phi": 6},
+                      tuning={"sigma.sq": 0.3, "tau.sq": 0.3, "phi": 0.3},
+                      priors={"sigma.sq.IG": [2, 5], "tau.sq.IG": [2, 1],
+                              "phi.Unif": [3, 300]},
+                      n_samples=S, n_neighbors=m, seed=9)
+    c0 = rng.uniform(size=(q, 2))
+    X0 = np.column_stack([np.ones(q), rng.normal(size=q)])
+    zs = rng.normal(size=q * S)
+    got = nngp.predict(fit, X0, c0, sub_sample={"start": 11, "thin": 2},
+                       rng=oracle.TapeRng(zs, np.zeros(1)))["p.y.0"]
+    s_indx = np.arange(11, S + 1, 2) - 1
+    beta = fit["p.beta.samples"][s_indx].T
+    th = fit["p.theta.samples"][s_indx].T
+    Sq = len(s_indx)
+    nn0, _ = oracle.nn2(c, c0, m)
+    ref = oracle.rNNGPPredict(X.T.reshape(-1).copy(), y,
+                              c.T.reshape(-1).copy(), n, 2, m,
+                              X0.T.reshape(-1).copy(),
+                              c0.T.reshape(-1).copy(), q,
+                              nn0.T.reshape(-1).copy(),
+                              beta.T.reshape(-1).copy(),
+                              th.T.reshape(-1).copy(), Sq, 0, zs[:q * Sq])
+    assert got.shape == (q, Sq)
+    assert max_rel(got, ref) < RTOL
+
+
+@pytest.mark.gpu
+def test_spConjNNGP_kfold_and_final_fit(oracle, nngp):
+    print("spConjNNGP front end", flush=True)
+    n, m, k = 900, 10, 3
+    c, X, y = gp_data(n, 5)
+    rng = np.random.default_rng(6)
+    nho = n // k
+    ho = np.array([np.sort(rng.choice(n, nho, replace=False))
+                   for _ in range(k)], dtype=np.int32)
+    grid = {"phi": [3.0, 3.0, 9.0, 9.0], "alpha": [0.1, 0.5, 0.1, 0.5]}
+    c0 = rng.uniform(size=(25, 2))
+    X0 = np.column_stack([np.ones(25), rng.normal(size=25)])
+    fit = nngp.spConjNNGP(y, X, c, theta_alpha=grid, sigma_sq_IG=[2, 5],
+                          n_neighbors=m, k_fold=k, X_0=X0, coords_0=c0,
+                          ho_indx=ho, n_samples=50)
+    o = np.argsort(c[:, 0], kind="stable")
+    co, Xo, yo = c[o], X[o], y[o]
+    ta = np.column_stack([grid["phi"], grid["alpha"]])
+    want = np.zeros((4, 2))
+    for f in range(k):
+        keep = np.ones(n, bool)
+        keep[ho[f]] = False
+        nn = oracle.mkNNIndxCB(co[keep], m)
+        nn0, _ = oracle.nn2(co[keep], co[ho[f]], m)
+        r = oracle.cNNGP(yo[keep], Xo[keep].T.reshape(-1).copy(), 2,
+                         int(keep.sum()), co[keep].T.reshape(-1).copy(),
+                         ta.reshape(-1).copy(), nn["nnIndx"], nn["nnIndxLU"],
+                         Xo[ho[f]].T.reshape(-1).copy(),
+                         co[ho[f]].T.reshape(-1).copy(), nho,
+                         nn0.T.reshape(-1).copy(), 4, m, [2, 5], 0)
+        for j in range(4):
+            want[j, 0] += nngp.rmspe(yo[ho[f]], r["y.0.hat"][:, j]) / k
+            want[j, 1] += nngp.crps(yo[ho[f]], r["y.0.hat"][:, j],
+                                    r["y.0.hat.var"][:, j]) / k
+    assert max_rel(fit["k.fold.scores"][:, 2:], want) < RTOL
+    best = ta[np.argmin(want[:, 1])]
+    assert fit["theta.alpha"]["phi"] == best[0]
+    assert fit["theta.alpha"]["alpha"] == best[1]
+    nn = oracle.mkNNIndxCB(co, m)
+    nn0, _ = oracle.nn2(co, c0, m)
+    r = oracle.cNNGP(yo, Xo.T.reshape(-1).copy(), 2, n,
+                     co.T.reshape(-1).copy(), best.copy(), nn["nnIndx"],
+                     nn["nnIndxLU"], X0.T.reshape(-1).copy(),
+                     c0.T.reshape(-1).copy(), 25, nn0.T.reshape(-1).copy(),
+                     1, m, [2, 5], 0)
+    assert max_rel(fit["beta.hat"], r["beta.hat"][:, 0]) < RTOL
+    assert max_rel(fit["y.0.hat"], r["y.0.hat"][:, 0]) < RTOL
+    assert max_rel(fit["y.0.hat.var"], r["y.0.hat.var"][:, 0]) < RTOL
+    a, b = r["ab"][:, 0]
+    assert fit["sigma.sq.hat"] == pytest.approx(b / (a - 1), rel=RTOL)
+    assert fit["p.beta.theta.samples"].shape == (50, 4)
