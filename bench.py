@@ -334,6 +334,65 @@ def conj_grid_bench(n, m, k, seed, cpu_points):
     return res
 
 
+def stress_bench(n, m, iters, seed):
+    """The stress case: n sites, 90% from 50 Gaussian clusters (centres
+    U[0,1]^2, sd 0.02) and 10% U[0,1]^2, then x stretched by 3; X = [1,
+    N(0,1)], beta = (1, 5); w from the NNGP prior (matern nu = 2.5, phi =
+    20, sigma.sq = 1, plus a 1e-6 jitter), tau.sq = 0.05; m neighbors.  nu is sampled, so every
+    pass evaluates the general Bessel-K matern (one site per wavefront).
+    Times `iters` MH iterations after one warm-up iteration."""
+    import spnngp_amd as nn
+    rng = np.random.default_rng(seed)
+    nc = int(0.9 * n)
+    centres = rng.uniform(size=(50, 2))
+    c = np.vstack([centres[rng.integers(0, 50, nc)]
+                   + rng.normal(scale=0.02, size=(nc, 2)),
+                   rng.uniform(size=(n - nc, 2))])
+    c[:, 0] *= 3.0
+    c = c[np.argsort(c[:, 0], kind="stable")]
+    X = np.column_stack([np.ones(n), rng.normal(size=n)])
+    t0 = time.perf_counter()
+    idx = nn.mkNNIndxCB(c, m)
+    knn_s = time.perf_counter() - t0
+    # the draw carries a 1e-6 nugget: with sites ~1e-4 apart inside a
+    # cluster, the nugget-free nu = 2.5 neighbor covariances are singular to
+    # working precision (dpotrf fails on them, as the reference's would)
+    B, F, _ = nn.updateBF(c, idx["nnIndx"], idx["nnIndxLU"], m,
+                          [1.0, 1e-6, 20.0, 2.5], "matern")
+    w = nn.prior_draw(B, F, idx["nnIndx"], idx["nnIndxLU"],
+                      rng.normal(size=n))
+    del B, F
+    y = X @ [1.0, 5.0] + w + rng.normal(scale=np.sqrt(0.05), size=n)
+    start = dict(sigmaSqIG=[2.0, 1.0], tauSqIG=[2.0, 0.05],
+                 phiUnif=[3.0, 100.0], nuUnif=[0.5, 4.0],
+                 betaStarting=np.linalg.lstsq(X, y, rcond=None)[0],
+                 sigmaSqStarting=1.0, tauSqStarting=0.05, phiStarting=20.0,
+                 nuStarting=2.5, sigmaSqTuning=0.01, tauSqTuning=0.01,
+                 phiTuning=0.01, nuTuning=0.01)
+    md = nn.Model(c, idx["nnIndx"], idx["nnIndxLU"], m, "matern", X, y)
+    md.rNNGP(nSamples=1, seed=21, **start)
+    cand, chosen = md.autotune([1.0, 0.05, 20.0, 2.4])
+    md.profile(True)
+    md.timing()
+    t0 = time.perf_counter()
+    out = md.rNNGP(nSamples=iters, seed=22, **start)
+    tm = md.timing()
+    dt = time.perf_counter() - t0
+    md.close()
+    bf_ms = tm["bf_ms"] / max(tm["bf_launches"], 1)
+    pairs = m * (m + 1) // 2
+    return {"n": n, "m": m, "iters": iters, "iters_s": iters / dt,
+            "ms_per_iter": dt / iters * 1e3, "updateBF_ms": bf_ms,
+            "bessel_evals_per_s": n * pairs / (bf_ms * 1e-3),
+            "flops_per_s": n * f_alg(m) / (bf_ms * 1e-3),
+            "bf_tune": {"candidates": [list(t) for t in cand],
+                        "chosen": list(chosen)},
+            "accept": out["accept"], "knn_s": knn_s,
+            "precision": "fp64 (the mixed form covers the per-lane kernels, "
+                         "m <= 16, closed-form covariances; this case runs "
+                         "FP64 only)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -347,6 +406,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--grid-n", type=int, default=500_000)
     ap.add_argument("--grid-cpu-points", type=int, default=4)
+    ap.add_argument("--stress-n", type=int, default=4_000_000)
+    ap.add_argument("--stress-iters", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is "
@@ -401,6 +462,10 @@ def main():
     grid = conj_grid_bench(args.grid_n, 15, 5, 3, args.grid_cpu_points) \
         if args.grid_n > 0 else None
 
+    print("[bench] stress", file=sys.stderr, flush=True)
+    stress = stress_bench(args.stress_n, 32, args.stress_iters, 5) \
+        if args.stress_n > 0 else None
+
     line = {
         "metric": METRIC, "value": iters_s, "unit": "iter/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
@@ -438,6 +503,7 @@ def main():
         "cpu_baseline": cpu,
         "kriging": krig,
         "conj_grid": grid,
+        "stress": stress,
         "mixed": mixed,
         "bf_tune": f64["tune"],
         "knn_s": knn_s,
