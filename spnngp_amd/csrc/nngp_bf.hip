@@ -48,6 +48,7 @@ constexpr int kCoopMaxWaves = 4;
 constexpr int kCoopMaxM = 64;
 constexpr int kCoopBlocks = 1024;  // logical blocks of the wavefront kernel
 constexpr int kRedThreads = 256;   // threads per block of the sum kernels
+constexpr int kRowsThreads = 1024; // threads per column of k_rows_sum
 constexpr int kRedBlocks = 1024;   // grid of the sum kernels (at most)
 
 __device__ __forceinline__ void wave_sync() {
@@ -583,22 +584,22 @@ __global__ __launch_bounds__(64 * kCoopMaxWaves) void k_bf_coop(BfArgs A) {
 // of problem j's rows (fixed per-thread order, fixed butterfly).  Block
 // (0, j) also hands over problem j's failure flag:
 // err_out[j] = err[j], err[j] = INT_MAX.
-__global__ __launch_bounds__(kRedThreads) void k_rows_sum(
+__global__ __launch_bounds__(kRowsThreads) void k_rows_sum(
     const double *rows, int cap, int nrows, long long ps_rows, double *out,
     int out_stride, int *err, double *err_out) {
-  __shared__ double red[kRedThreads / 64];
+  __shared__ double red[kRowsThreads / 64];
   const int k = (int)blockIdx.x, j = (int)blockIdx.y;
   const int w = (int)threadIdx.x >> 6, t = (int)threadIdx.x & 63;
   double s = 0.0;
   const double *col = rows + (size_t)j * ps_rows + (size_t)k * cap;
-  for (int r = (int)threadIdx.x; r < nrows; r += kRedThreads) s += col[r];
+  for (int r = (int)threadIdx.x; r < nrows; r += kRowsThreads) s += col[r];
   s = wave_sum(s);
   if (t == 0) red[w] = s;
   __syncthreads();
   if (threadIdx.x == 0) {
     double tot = 0.0;
 #pragma unroll
-    for (int ww = 0; ww < kRedThreads / 64; ww++) tot += red[ww];
+    for (int ww = 0; ww < kRowsThreads / 64; ww++) tot += red[ww];
     out[(size_t)j * out_stride + k] = tot;
     if (k == 0 && err) {
       err_out[j] = (double)err[j];
@@ -610,7 +611,7 @@ __global__ __launch_bounds__(kRedThreads) void k_rows_sum(
 int rows_sum_launch(const double *rows, int cap, int nrows, long long ps_rows,
                     int nv, int nprob, double *out, int out_stride, int *err,
                     double *err_out, hipStream_t st) {
-  hipLaunchKernelGGL(k_rows_sum, dim3(nv, nprob), dim3(kRedThreads), 0, st,
+  hipLaunchKernelGGL(k_rows_sum, dim3(nv, nprob), dim3(kRowsThreads), 0, st,
                      rows, cap, nrows, ps_rows, out, out_stride, err,
                      err_out);
   return (int)hipGetLastError();

@@ -315,6 +315,13 @@ int Model::create(int n_, int m_, int p_, const double *coords,
   if (fill_int_launch(err.as<int>(), INT_MAX, st.s))
     return set_err(NNGP_ERR_HIP, "flag reset kernel failed");
   NNGP_TRY(hsums.alloc(sizeof(double) * kSumsLen));
+  {
+    void *dp = nullptr;
+    if (hipHostGetDevicePointer(&dp, hsums.p, 0) == hipSuccess && dp)
+      sums_mapped = static_cast<double *>(dp);
+    else
+      (void)hipGetLastError();  // fall back to a copy per pass
+  }
   NNGP_HIP(hipStreamSynchronize(st.s));
   return NNGP_OK;
 }
@@ -388,8 +395,9 @@ int Model::autotune(const CovArgs &c, BfTuneReport *rep) {
 }
 
 int Model::sums_launch(int slot, int off) {
-  double *out = sums.as<double>() + off;
-  double *err_out = sums.as<double>() + 2 * kStatsMax + (off ? 1 : 0);
+  double *base = sums_mapped ? sums_mapped : sums.as<double>();
+  double *out = base + off;
+  double *err_out = base + 2 * kStatsMax + (off ? 1 : 0);
   const int ev = prof ? ev_begin(1) : -1;
   (void)slot;
   const int rc =
@@ -416,8 +424,9 @@ double quad_resid(const PassOut &po, const double *beta, int p) {
 }
 
 int Model::fetch_sums() {
-  NNGP_HIP(hipMemcpyAsync(hsums.p, sums.p, sizeof(double) * kSumsLen,
-                          hipMemcpyDeviceToHost, st.s));
+  if (!sums_mapped)
+    NNGP_HIP(hipMemcpyAsync(hsums.p, sums.p, sizeof(double) * kSumsLen,
+                            hipMemcpyDeviceToHost, st.s));
   NNGP_HIP(hipStreamSynchronize(st.s));
   if (prof) NNGP_TRY(collect());
   return NNGP_OK;

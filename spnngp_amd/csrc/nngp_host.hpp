@@ -189,6 +189,8 @@ struct Model {
   // off kStatsMax, then the updateBF failure flag of each (as doubles)
   DevBuf rows, sums, err;
   PinnedBuf hsums;  // host mirror of sums
+  double *sums_mapped = nullptr;  // device address of hsums: the sum kernel
+                                  // writes there directly (no copy)
   BfArgs last_bf;      // arguments of the last bf() launch
   BfConfig last_cfg;   // and its launch shape
   BfConfig tuned_cfg;  // launch shape chosen by autotune(), when have_tuned
