@@ -46,7 +46,7 @@ namespace nngp {
 constexpr int kLaneMaxThreads = 256;
 constexpr int kCoopMaxWaves = 4;
 constexpr int kCoopMaxM = 64;
-constexpr int kCoopBlocks = 1024;  // logical blocks of the wavefront kernel
+constexpr int kCoopBlocks = 2048;  // logical blocks of the wavefront kernel
 constexpr int kRedThreads = 256;   // threads per block of the sum kernels
 constexpr int kRowsThreads = 1024; // threads per column of k_rows_sum
 constexpr int kRedBlocks = 1024;   // grid of the sum kernels (at most)
@@ -416,9 +416,10 @@ __host__ __device__ inline int coop_wave_doubles(int m) {
 // the residual vector e (identical in all lanes); lane t < cnt returns its
 // coefficient B_t in bcoef.  False when a pivot is not positive.
 __device__ inline bool coop_site(const BfArgs &A, const BfProblem &P,
-                                 const CovArgs &c, double *Am, int i,
-                                 double &F, double (&e)[kMaxNZ],
-                                 double &bcoef, int &cnt_out, int &off_out) {
+                                 const CovArgs &c, const BesselTab &bt,
+                                 double *Am, int i, double &F,
+                                 double (&e)[kMaxNZ], double &bcoef,
+                                 int &cnt_out, int &off_out) {
   const int t = (int)threadIdx.x & 63;
   const int ld = A.m + 1;
   double *cv = Am + A.m * ld;
@@ -455,7 +456,7 @@ __device__ inline bool coop_site(const BfArgs &A, const BfProblem &P,
     } else {
       const double ax = k < cnt ? px[k] : s.x;
       const double ay = k < cnt ? py[k] : s.y;
-      const double val = s2_corr_rt(sqdist(ax, ay, px[l], py[l]), c);
+      const double val = s2_corr_rt(sqdist(ax, ay, px[l], py[l]), c, bt);
       if (k < cnt)
         Am[k * ld + l] = val;
       else
@@ -525,6 +526,7 @@ template <bool BATCH>
 __global__ __launch_bounds__(64 * kCoopMaxWaves) void k_bf_coop(BfArgs A) {
   extern __shared__ double smem[];
   __shared__ double red[kCoopMaxWaves][kStatsMax];
+  __shared__ BesselTab bt;
   const int nw = (int)blockDim.x >> 6;
   const int w = (int)threadIdx.x >> 6;
   const int t = (int)threadIdx.x & 63;
@@ -532,41 +534,43 @@ __global__ __launch_bounds__(64 * kCoopMaxWaves) void k_bf_coop(BfArgs A) {
   if (b < 0) return;
   const BfProblem P = bf_problem<BATCH>(A);
   const CovArgs c = BATCH ? A.covs[(int)blockIdx.y % A.ng] : A.c;
+  if (c.cov == 2) {  // the matern's loop divisors for this block's mu
+    bessel_tab_fill(bt, c);
+    __syncthreads();
+  }
   double *Am = smem + (size_t)w * coop_wave_doubles(A.m);
   const int chunk = (A.n + A.coop_blocks - 1) / A.coop_blocks;
   const int i0 = b * chunk;
   const int i1 = min(A.n, i0 + chunk);
-  double acc[kStatsMax];
-#pragma unroll
-  for (int k = 0; k < kStatsMax; k++) acc[k] = 0.0;
+  // the wave's running sums live in LDS (lane 0 updates them), which keeps
+  // this kernel's registers low enough for several waves per SIMD
+  const int nv = 2 + A.nz * (A.nz + 1) / 2;
+  for (int k = t; k < nv; k += 64) red[w][k] = 0.0;
+  wave_sync();
   for (int i = i0 + w; i < i1; i += nw) {
     double F, e[kMaxNZ], bcoef;
     int cnt, off;
-    const bool ok = coop_site(A, P, c, Am, i, F, e, bcoef, cnt, off);
+    const bool ok = coop_site(A, P, c, bt, Am, i, F, e, bcoef, cnt, off);
     if (!ok && t == 0) atomicMin(P.err, i);
     if (!BATCH) {
       if (A.B && ok && t < cnt) A.B[off + t] = bcoef;
       if (A.F && t == 0) A.F[i] = F;
     }
-    const double rf = 1.0 / F;
-    acc[0] += log(F);
-    int g = 2;
+    if (t == 0) {
+      const double rf = 1.0 / F;
+      red[w][0] += log(F);
+      int g = 2;
 #pragma unroll
-    for (int p = 0; p < kMaxNZ; p++) {
-      if (p < A.nz) {
+      for (int p = 0; p < kMaxNZ; p++) {
+        if (p < A.nz) {
 #pragma unroll
-        for (int q = 0; q <= p; q++) acc[g + q] += e[p] * e[q] * rf;
+          for (int q = 0; q <= p; q++) red[w][g + q] += e[p] * e[q] * rf;
+        }
+        g += p + 1;
       }
-      g += p + 1;
     }
   }
   if (!P.rows) return;
-  const int nv = 2 + A.nz * (A.nz + 1) / 2;
-  if (t == 0) {
-#pragma unroll
-    for (int k = 0; k < kStatsMax; k++)
-      if (k < nv) red[w][k] = acc[k];
-  }
   __syncthreads();
   const int k = (int)threadIdx.x;
   if (k < nv) {

@@ -34,6 +34,8 @@ struct CovArgs {
   double g1, g2;   // Gamma_1(mu), Gamma_2(mu)
   double rgp;      // 1 / Gamma(1 + mu)
   double rgm;      // 1 / Gamma(1 - mu)
+  double gp, gm;   // Gamma(1 + mu), Gamma(1 - mu)
+  double fa;       // pi mu / sin(pi mu) (1 at mu = 0)
   int nl;          // round(nu)
   int cov;         // covModel
 };
@@ -142,12 +144,155 @@ __device__ inline double bessel_k(double x, const CovArgs &a) {
   return kmu;
 }
 
+// Reciprocals the Bessel-K loops divide by, which depend only on the loop
+// index and mu: filled once per block (bessel_tab_fill) so the series and
+// continued fraction multiply instead of divide.
+constexpr int kBesTab = 128;
+struct BesselTab {
+  double ik[kBesTab];    // 1 / k
+  double ik2[kBesTab];   // 1 / (k^2 - mu^2)
+  double ikm[kBesTab];   // 1 / (k - mu)
+  double ikp[kBesTab];   // 1 / (k + mu)
+  double iaa[kBesTab];   // 1 / (mu^2 - 1/4 - k (k - 1)), Steed's a_k
+};
+
+// Every thread of the block calls it (then a barrier before use).
+__device__ inline void bessel_tab_fill(BesselTab &t, const CovArgs &a) {
+  const double mu = a.mu;
+  for (int k = (int)threadIdx.x; k < kBesTab; k += (int)blockDim.x) {
+    const double dk = (double)(k > 0 ? k : 1);
+    t.ik[k] = 1.0 / dk;
+    t.ik2[k] = 1.0 / (dk * dk - mu * mu);
+    t.ikm[k] = 1.0 / (dk - mu);
+    t.ikp[k] = 1.0 / (dk + mu);
+    t.iaa[k] = 1.0 / (mu * mu - 0.25 - dk * (dk - 1.0));
+  }
+}
+
+// 1/y to ~1 ulp: hardware reciprocal and two Newton steps.
+__device__ __forceinline__ double nn_rcp(double y) {
+  double r = __builtin_amdgcn_rcp(y);
+  double e = fma(-y, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-y, r, 1.0);
+  return fma(r, e, r);
+}
+
+// x^nu K_nu(x) for 0 < x <= 700: the method of bessel_k above (Temme's
+// series for x <= 2, Steed's continued fraction above, upward recurrence in
+// nu) with the loop divisors taken from t, the powers of x folded into one
+// exp, and sinh / cosh of the series built from one exp.
+__device__ inline double xnu_bessel_k(double x, const CovArgs &a,
+                                      const BesselTab &t) {
+  const double eps = 1e-17;
+  const double mu = a.mu;
+  const double tx = nn_rcp(x) * 2.0;  // 2 / x
+  double kmu, kmu1, scale;
+  if (x <= 2.0) {
+    const double lg = -log(0.5 * x);
+    const double sg = mu * lg;
+    const double es = exp(sg);
+    const double ies = nn_rcp(es);
+    double fs;  // sinh(sg) / sg
+    if (fabs(sg) < 0.1) {
+      const double s2 = sg * sg;
+      fs = 1.0 + s2 * (1.0 / 6.0 + s2 * (1.0 / 120.0 + s2 * (1.0 / 5040.0 +
+           s2 * (1.0 / 362880.0 + s2 * (1.0 / 39916800.0)))));
+    } else {
+      fs = 0.5 * (es - ies) / sg;
+    }
+    double f = a.fa * (a.g1 * (0.5 * (es + ies)) + a.g2 * fs * lg);
+    double pk = 0.5 * es * a.gp;
+    double qk = 0.5 * ies * a.gm;
+    double ck = 1.0;
+    const double d = 0.25 * x * x;
+    double s0 = f, s1 = pk;
+    for (int k = 1; k < 200; k++) {
+      double i2, i1, im, ip;
+      if (k < kBesTab) {
+        i2 = t.ik2[k];
+        i1 = t.ik[k];
+        im = t.ikm[k];
+        ip = t.ikp[k];
+      } else {
+        i2 = 1.0 / ((double)k * k - mu * mu);
+        i1 = 1.0 / k;
+        im = 1.0 / (k - mu);
+        ip = 1.0 / (k + mu);
+      }
+      f = (k * f + pk + qk) * i2;
+      ck *= d * i1;
+      pk *= im;
+      qk *= ip;
+      const double t0 = ck * f;
+      s0 += t0;
+      s1 += ck * (pk - k * f);
+      if (fabs(t0) < fabs(s0) * eps) break;
+    }
+    kmu = s0;
+    kmu1 = s1 * tx;
+    scale = exp(a.nu * (0.6931471805599453 - lg));  // x^nu
+  } else {
+    double b = 2.0 * (1.0 + x);
+    double dd = nn_rcp(b);
+    double h = dd, dh = dd;
+    double q1 = 0.0, q2 = 1.0;
+    const double a1 = 0.25 - mu * mu;
+    double q = a1, c = a1;
+    double aa = -a1;
+    double s = 1.0 + q * dh;
+    for (int i = 2; i < 2000; i++) {
+      aa -= 2.0 * (i - 1);
+      double inv_i, inv_aa;
+      if (i < kBesTab) {
+        inv_i = t.ik[i];
+        inv_aa = t.iaa[i];
+      } else {
+        inv_i = 1.0 / i;
+        inv_aa = 1.0 / aa;
+      }
+      c = -aa * c * inv_i;
+      const double qn = (q1 - b * q2) * inv_aa;
+      q1 = q2;
+      q2 = qn;
+      q += c * qn;
+      b += 2.0;
+      dd = nn_rcp(b + aa * dd);
+      dh = (b * dd - 1.0) * dh;
+      h += dh;
+      const double ds = q * dh;
+      s += ds;
+      if (fabs(ds) < fabs(s) * eps) break;
+    }
+    h = a1 * h;
+    // K_mu without its e^-x: sqrt(pi / (2x)) / s; e^-x x^nu folded below
+    kmu = 1.2533141373155003 * nn_rsqrt(x) * nn_rcp(s);
+    kmu1 = kmu * (mu + x + 0.5 - h) * (0.5 * tx);
+    scale = exp(a.nu * log(x) - x);
+  }
+  for (int i = 1; i <= a.nl; i++) {
+    const double kn = fma((mu + i) * tx, kmu1, kmu);
+    kmu = kmu1;
+    kmu1 = kn;
+  }
+  return kmu * scale;
+}
+
 // s2 * matern correlation at distance d (src/util.cpp:228-237).
 __device__ inline double matern_s2(double d, const CovArgs &a) {
   double x = d * a.phi;
   if (!(x > 0.0)) return a.s2;
   if (x > 700.0) return 0.0;  // K_nu(x) x^nu underflows long before this
   return a.mnorm * pow(x, a.nu) * bessel_k(x, a);
+}
+
+// The same with the loop divisors from a block's table.
+__device__ inline double matern_s2_tab(double d, const CovArgs &a,
+                                       const BesselTab &t) {
+  const double x = d * a.phi;
+  if (!(x > 0.0)) return a.s2;
+  if (x > 700.0) return 0.0;
+  return a.mnorm * xnu_bessel_k(x, a, t);
 }
 
 // Covariance variants of the templated kernels: the four covModels, plus
@@ -198,6 +343,13 @@ __device__ inline double s2_corr_rt(double d2, const CovArgs &a) {
     case 2: return s2_corr<2>(d2, a);
     default: return s2_corr<3>(d2, a);
   }
+}
+
+// The same, with the matern's Bessel-K loop divisors from a block's table.
+__device__ inline double s2_corr_rt(double d2, const CovArgs &a,
+                                    const BesselTab &t) {
+  if (a.cov == 2) return matern_s2_tab(nn_sqrt(d2), a, t);
+  return s2_corr_rt(d2, a);
 }
 
 // Squared distance of two points.

@@ -84,6 +84,10 @@ CovArgs make_cov(int cov, double s2, double t2, double phi, double nu) {
     const double rm = 1.0 / std::tgamma(1.0 - mu);
     c.rgp = rp;
     c.rgm = rm;
+    c.gp = std::tgamma(1.0 + mu);
+    c.gm = std::tgamma(1.0 - mu);
+    const double pm = 3.14159265358979323846 * mu;
+    c.fa = std::fabs(pm) < 1e-300 ? 1.0 : pm / std::sin(pm);
     c.g2 = 0.5 * (rm + rp);
     if (std::fabs(mu) < 0.1) {
       const double u = mu * mu;
