@@ -412,9 +412,69 @@ __host__ __device__ inline int coop_wave_doubles(int m) {
   return m * (m + 1) + 3 * m;
 }
 
+// Cholesky of the cnt x cnt lower triangle in LDS (row-major, ld; row t
+// read by lane t) and the two triangular solves, for cnt <= MM <= 64.
+// Lane t keeps row t of the factor in registers; the entries of column j
+// reach the other rows by readlane, so the right-looking update is register
+// arithmetic (no LDS round trips in the O(m^3) part).  The factor goes back
+// to LDS for the back substitution, which needs its columns.  ct is lane
+// t's right-hand side; on return x = (C^-1 c)_t and vv = c^T C^-1 c.
+// False (uniformly) when a pivot is not positive.
+template <int MM>
+__device__ inline bool wave_chol_rows(double *Am, int ld, int cnt, int t,
+                                      double ct, double &x, double &vv) {
+  // Rows t >= cnt stay zero, so their L_tj is zero and the updates below
+  // need no bound on l.  Entries above lane t's diagonal collect values
+  // that no step reads: step j broadcasts the L_lj of lanes l > j only.
+  double row[MM];
+#pragma unroll
+  for (int l = 0; l < MM; l++)
+    row[l] = (t < cnt && l <= t) ? Am[t * ld + l] : 0.0;
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < MM; j++) {
+    // a guard, not a break: a loop exit would stop the full unrolling that
+    // keeps row[] in registers
+    if (j < cnt) {
+      const double piv = readlane_d(row[j], j);
+      ok = ok && (piv > 0.0);
+      const double r = nn_rsqrt(piv > 0.0 ? piv : 1.0);
+      const double ltj = row[j] * r;  // L_tj for t > j
+      row[j] = t == j ? r : ltj;      // the diagonal keeps 1 / L_jj
+#pragma unroll
+      for (int l = j + 1; l < MM; l++)
+        row[l] = fma(-ltj, readlane_d(ltj, l), row[l]);
+    }
+  }
+  if (!ok) return false;
+  // forward solve: lane t finishes x_t at step t
+  double xt = ct;
+#pragma unroll
+  for (int k = 0; k < MM; k++) {
+    if (k < cnt) {
+      const double xk = readlane_d(xt * row[k], k);
+      xt = t == k ? xk : (t > k ? fma(-row[k], xk, xt) : xt);
+    }
+  }
+  vv = wave_sum(t < cnt ? xt * xt : 0.0);
+#pragma unroll
+  for (int l = 0; l < MM; l++)
+    if (l < cnt && t < cnt && l <= t) Am[t * ld + l] = row[l];
+  wave_sync();
+  // back solve B = L^-T x: column k of the factor is row k's entries
+  for (int k = cnt - 1; k >= 0; k--) {
+    const double bk = readlane_d(xt, k) * Am[k * ld + k];
+    if (t == k) xt = bk;
+    if (t < k) xt = fma(-Am[k * ld + t], bk, xt);
+  }
+  x = xt;
+  return true;
+}
+
 // Solve site i's system in the wave's LDS area Am; every lane returns F and
 // the residual vector e (identical in all lanes); lane t < cnt returns its
 // coefficient B_t in bcoef.  False when a pivot is not positive.
+template <int MM>
 __device__ inline bool coop_site(const BfArgs &A, const BfProblem &P,
                                  const CovArgs &c, const BesselTab &bt,
                                  double *Am, int i, double &F,
@@ -464,46 +524,13 @@ __device__ inline bool coop_site(const BfArgs &A, const BfProblem &P,
     }
   }
   wave_sync();
-  // right-looking Cholesky, lane t owns row t; diagonal keeps 1/L_jj
-  bool ok = true;
-  for (int j = 0; j < cnt; j++) {
-    const double piv = Am[j * ld + j];
-    ok = ok && (piv > 0.0);
-    if (!ok) break;
-    const double r = nn_rsqrt(piv);
-    double ltj = 0.0;
-    if (t > j && t < cnt) {
-      ltj = Am[t * ld + j] * r;
-      Am[t * ld + j] = ltj;
-    }
-    wave_sync();
-    if (t == j) Am[j * ld + j] = r;
-    if (t > j && t < cnt) {
-      for (int l = j + 1; l <= t; l++)
-        Am[t * ld + l] = fma(-ltj, Am[l * ld + j], Am[t * ld + l]);
-    }
-    wave_sync();
-  }
-  if (!ok) {
+  double x, vv;
+  if (!wave_chol_rows<MM>(Am, ld, cnt, t, t < cnt ? cv[t] : 0.0, x, vv)) {
     F = 1.0;  // keeps the sums finite; the pass reports the failure
 #pragma unroll
     for (int j = 0; j < kMaxNZ; j++) e[j] = 0.0;
     wave_sync();
     return false;
-  }
-  // forward solve: lane t holds x_t
-  double x = t < cnt ? cv[t] : 0.0;
-  for (int k = 0; k < cnt; k++) {
-    const double vk = readlane_d(x, k) * Am[k * ld + k];
-    if (t == k) x = vk;
-    if (t > k && t < cnt) x = fma(-Am[t * ld + k], vk, x);
-  }
-  const double vv = wave_sum(t < cnt ? x * x : 0.0);
-  // back solve B = L^-T v
-  for (int k = cnt - 1; k >= 0; k--) {
-    const double bk = readlane_d(x, k) * Am[k * ld + k];
-    if (t == k) x = bk;
-    if (t < k) x = fma(-Am[k * ld + t], bk, x);
   }
   bcoef = t < cnt ? x : 0.0;
   F = c.diag - vv;
@@ -522,7 +549,7 @@ __device__ inline bool coop_site(const BfArgs &A, const BfProblem &P,
 
 // Logical block b covers sites [b*chunk, (b+1)*chunk); its waves take every
 // nw-th site of that run.  A.coop_blocks logical blocks per problem.
-template <bool BATCH>
+template <bool BATCH, int MM>
 __global__ __launch_bounds__(64 * kCoopMaxWaves) void k_bf_coop(BfArgs A) {
   extern __shared__ double smem[];
   __shared__ double red[kCoopMaxWaves][kStatsMax];
@@ -550,7 +577,8 @@ __global__ __launch_bounds__(64 * kCoopMaxWaves) void k_bf_coop(BfArgs A) {
   for (int i = i0 + w; i < i1; i += nw) {
     double F, e[kMaxNZ], bcoef;
     int cnt, off;
-    const bool ok = coop_site(A, P, c, bt, Am, i, F, e, bcoef, cnt, off);
+    const bool ok =
+        coop_site<MM>(A, P, c, bt, Am, i, F, e, bcoef, cnt, off);
     if (!ok && t == 0) atomicMin(P.err, i);
     if (!BATCH) {
       if (A.B && ok && t < cnt) A.B[off + t] = bcoef;
@@ -740,6 +768,19 @@ static void launch_lane(const BfArgs &a, int variant, const dim3 &grid,
   else launch_lane_m<16, BATCH>(a, variant, grid, blk, st);
 }
 
+// The wavefront kernel's register-resident factor is sized by the smallest
+// of 16, 32 and 64 rows that holds m.
+template <bool BATCH>
+static void launch_coop(const BfArgs &a, const dim3 &grid, const dim3 &blk,
+                        size_t shm, hipStream_t st) {
+  if (a.m <= 16)
+    hipLaunchKernelGGL((k_bf_coop<BATCH, 16>), grid, blk, shm, st, a);
+  else if (a.m <= 32)
+    hipLaunchKernelGGL((k_bf_coop<BATCH, 32>), grid, blk, shm, st, a);
+  else
+    hipLaunchKernelGGL((k_bf_coop<BATCH, 64>), grid, blk, shm, st, a);
+}
+
 int bf_launch(const BfArgs &a0, const BfConfig &cfg, int nprob,
               hipStream_t st) {
   if (a0.n <= 0 || nprob <= 0) return 0;
@@ -760,9 +801,9 @@ int bf_launch(const BfArgs &a0, const BfConfig &cfg, int nprob,
     a.coop_blocks = nrows;
     const size_t shm = coop_shm(a.m, cfg.threads / 64);
     if (batch)
-      hipLaunchKernelGGL(k_bf_coop<true>, grid, blk, shm, st, a);
+      launch_coop<true>(a, grid, blk, shm, st);
     else
-      hipLaunchKernelGGL(k_bf_coop<false>, grid, blk, shm, st, a);
+      launch_coop<false>(a, grid, blk, shm, st);
   }
   return (int)hipGetLastError();
 }
