@@ -14,8 +14,9 @@
 //    their rows/columns of C are exactly zero off the diagonal, so the real
 //    block factors bit-for-bit as if unpadded and the padded coefficients
 //    come out exactly 0.  Block size 64, 128 or 256 threads.
-//  * k_bf_coop<BATCH>: one site per wavefront, the matrix in LDS, one row
-//    per lane (any m <= 64, any covariance including the Bessel-K matern),
+//  * k_bf_coop<BATCH, MM>: one site per wavefront, the matrix built in LDS
+//    and factored with one row per lane in registers (any m <= MM <= 64,
+//    any covariance including the Bessel-K matern),
 //    1, 2 or 4 wavefronts per block walking a contiguous run of sites.
 // Both compute C = s2 R(N,N) + t2 I, c = s2 R(N,i), the Cholesky factor of C
 // (pivots kept as reciprocal square roots), v = L^-1 c, B = L^-T v and
@@ -40,6 +41,7 @@
 #include <vector>
 
 #include "nngp_kernels.hpp"
+#include "nngp_wave.hpp"
 
 namespace nngp {
 
@@ -50,24 +52,6 @@ constexpr int kCoopBlocks = 2048;  // logical blocks of the wavefront kernel
 constexpr int kRedThreads = 256;   // threads per block of the sum kernels
 constexpr int kRowsThreads = 1024; // threads per column of k_rows_sum
 constexpr int kRedBlocks = 1024;   // grid of the sum kernels (at most)
-
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ double readlane_d(double v, int lane) {
-  int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-  int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-  return __hiloint2double(hi, lo);
-}
-
-__device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-  return x;
-}
 
 // DPP lane move of a double (two 32-bit moves); lanes outside ROWMASK get 0.
 template <int CTRL, int ROWMASK>
@@ -412,65 +396,6 @@ __host__ __device__ inline int coop_wave_doubles(int m) {
   return m * (m + 1) + 3 * m;
 }
 
-// Cholesky of the cnt x cnt lower triangle in LDS (row-major, ld; row t
-// read by lane t) and the two triangular solves, for cnt <= MM <= 64.
-// Lane t keeps row t of the factor in registers; the entries of column j
-// reach the other rows by readlane, so the right-looking update is register
-// arithmetic (no LDS round trips in the O(m^3) part).  The factor goes back
-// to LDS for the back substitution, which needs its columns.  ct is lane
-// t's right-hand side; on return x = (C^-1 c)_t and vv = c^T C^-1 c.
-// False (uniformly) when a pivot is not positive.
-template <int MM>
-__device__ inline bool wave_chol_rows(double *Am, int ld, int cnt, int t,
-                                      double ct, double &x, double &vv) {
-  // Rows t >= cnt stay zero, so their L_tj is zero and the updates below
-  // need no bound on l.  Entries above lane t's diagonal collect values
-  // that no step reads: step j broadcasts the L_lj of lanes l > j only.
-  double row[MM];
-#pragma unroll
-  for (int l = 0; l < MM; l++)
-    row[l] = (t < cnt && l <= t) ? Am[t * ld + l] : 0.0;
-  bool ok = true;
-#pragma unroll
-  for (int j = 0; j < MM; j++) {
-    // a guard, not a break: a loop exit would stop the full unrolling that
-    // keeps row[] in registers
-    if (j < cnt) {
-      const double piv = readlane_d(row[j], j);
-      ok = ok && (piv > 0.0);
-      const double r = nn_rsqrt(piv > 0.0 ? piv : 1.0);
-      const double ltj = row[j] * r;  // L_tj for t > j
-      row[j] = t == j ? r : ltj;      // the diagonal keeps 1 / L_jj
-#pragma unroll
-      for (int l = j + 1; l < MM; l++)
-        row[l] = fma(-ltj, readlane_d(ltj, l), row[l]);
-    }
-  }
-  if (!ok) return false;
-  // forward solve: lane t finishes x_t at step t
-  double xt = ct;
-#pragma unroll
-  for (int k = 0; k < MM; k++) {
-    if (k < cnt) {
-      const double xk = readlane_d(xt * row[k], k);
-      xt = t == k ? xk : (t > k ? fma(-row[k], xk, xt) : xt);
-    }
-  }
-  vv = wave_sum(t < cnt ? xt * xt : 0.0);
-#pragma unroll
-  for (int l = 0; l < MM; l++)
-    if (l < cnt && t < cnt && l <= t) Am[t * ld + l] = row[l];
-  wave_sync();
-  // back solve B = L^-T x: column k of the factor is row k's entries
-  for (int k = cnt - 1; k >= 0; k--) {
-    const double bk = readlane_d(xt, k) * Am[k * ld + k];
-    if (t == k) xt = bk;
-    if (t < k) xt = fma(-Am[k * ld + t], bk, xt);
-  }
-  x = xt;
-  return true;
-}
-
 // Solve site i's system in the wave's LDS area Am; every lane returns F and
 // the residual vector e (identical in all lanes); lane t < cnt returns its
 // coefficient B_t in bcoef.  False when a pivot is not positive.
@@ -773,12 +698,16 @@ static void launch_lane(const BfArgs &a, int variant, const dim3 &grid,
 template <bool BATCH>
 static void launch_coop(const BfArgs &a, const dim3 &grid, const dim3 &blk,
                         size_t shm, hipStream_t st) {
-  if (a.m <= 16)
-    hipLaunchKernelGGL((k_bf_coop<BATCH, 16>), grid, blk, shm, st, a);
-  else if (a.m <= 32)
-    hipLaunchKernelGGL((k_bf_coop<BATCH, 32>), grid, blk, shm, st, a);
-  else
-    hipLaunchKernelGGL((k_bf_coop<BATCH, 64>), grid, blk, shm, st, a);
+  switch (wave_rows_for(a.m)) {
+    case 16:
+      hipLaunchKernelGGL((k_bf_coop<BATCH, 16>), grid, blk, shm, st, a);
+      break;
+    case 32:
+      hipLaunchKernelGGL((k_bf_coop<BATCH, 32>), grid, blk, shm, st, a);
+      break;
+    default:
+      hipLaunchKernelGGL((k_bf_coop<BATCH, 64>), grid, blk, shm, st, a);
+  }
 }
 
 int bf_launch(const BfArgs &a0, const BfConfig &cfg, int nprob,

@@ -24,6 +24,7 @@
 
 #include "nngp_host.hpp"
 #include "nngp_strips.hpp"
+#include "nngp_wave.hpp"
 
 namespace nngp {
 
@@ -278,26 +279,10 @@ __global__ __launch_bounds__(kCvPredThreads) void k_cv_pred_lane(
                  b);
 }
 
-__device__ __forceinline__ void wave_sync_cv() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ double readlane_cv(double v, int lane) {
-  int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-  int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-  return __hiloint2double(hi, lo);
-}
-
-__device__ __forceinline__ double wave_sum_cv(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-  return x;
-}
-
-// One held-out site per wavefront (any m <= 64, any covariance); logical
-// block b walks a contiguous run of sites, its waves every nw-th one.
+// One held-out site per wavefront (any m <= MM <= 64, any covariance);
+// logical block b walks a contiguous run of sites, its waves every nw-th
+// one.
+template <int MM>
 __global__ __launch_bounds__(64 * kCvCoopWaves) void k_cv_pred_coop(
     CvPredArgs A) {
   extern __shared__ double smem[];
@@ -328,7 +313,7 @@ __global__ __launch_bounds__(64 * kCvCoopWaves) void k_cv_pred_coop(
       px[t] = q.x;
       py[t] = q.y;
     }
-    wave_sync_cv();
+    wave_sync();
     const double2 s0 = A.xy0[(size_t)f * A.nho + i];
     const int np = m * (m + 1) / 2;
     for (int q = t; q < np; q += 64) {
@@ -341,41 +326,12 @@ __global__ __launch_bounds__(64 * kCvCoopWaves) void k_cv_pred_coop(
     }
     const double c0 = t < m ? s2_corr_rt(sqdist(px[t], py[t], s0.x, s0.y), c)
                             : 0.0;
-    wave_sync_cv();
-    bool ok = true;
-    for (int jj = 0; jj < m; jj++) {
-      const double piv = Am[jj * ld + jj];
-      ok = ok && piv > 0.0;
-      if (!ok) break;
-      const double r = nn_rsqrt(piv);
-      double ltj = 0.0;
-      if (t > jj && t < m) {
-        ltj = Am[t * ld + jj] * r;
-        Am[t * ld + jj] = ltj;
-      }
-      wave_sync_cv();
-      if (t == jj) Am[jj * ld + jj] = r;
-      if (t > jj && t < m)
-        for (int l = jj + 1; l <= t; l++)
-          Am[t * ld + l] = fma(-ltj, Am[l * ld + jj], Am[t * ld + l]);
-      wave_sync_cv();
-    }
-    if (!ok) {
+    wave_sync();
+    double x, vv;
+    if (!wave_chol_rows<MM>(Am, ld, m, t, c0, x, vv)) {
       if (t == 0) atomicMin(A.perr + j, i);
-      wave_sync_cv();
+      wave_sync();
       continue;
-    }
-    double x = c0;
-    for (int k = 0; k < m; k++) {
-      const double vk = readlane_cv(x, k) * Am[k * ld + k];
-      if (t == k) x = vk;
-      if (t > k && t < m) x = fma(-Am[t * ld + k], vk, x);
-    }
-    const double vv = wave_sum_cv(t < m ? x * x : 0.0);
-    for (int k = m - 1; k >= 0; k--) {
-      const double bk = readlane_cv(x, k) * Am[k * ld + k];
-      if (t == k) x = bk;
-      if (t < k) x = fma(-Am[k * ld + t], bk, x);
     }
     const double *zr = Z + (size_t)id * A.zs;
     double u[kMaxP];
@@ -384,10 +340,10 @@ __global__ __launch_bounds__(64 * kCvCoopWaves) void k_cv_pred_coop(
       double xb = 0.0;
       for (int r = 0; r < p; r++) {
         const double xr = t < m ? zr[r] : 0.0;
-        u[r] = wave_sum_cv(x * xr);
+        u[r] = wave_sum(x * xr);
         xb += xr * so[r];
       }
-      part = wave_sum_cv(t < m ? x * (zr[p] - xb) : 0.0);
+      part = wave_sum(t < m ? x * (zr[p] - xb) : 0.0);
     }
     double yhat = part;
     for (int r = 0; r < p; r++) {
@@ -408,7 +364,7 @@ __global__ __launch_bounds__(64 * kCvCoopWaves) void k_cv_pred_coop(
       crps_acc += cr;
       sq_acc += s2;
     }
-    wave_sync_cv();  // this LDS area is reused by the wave's next site
+    wave_sync();  // this LDS area is reused by the wave's next site
   }
   block_pair_sum(crps_acc, sq_acc, A.prow + (size_t)j * 2 * A.prow_cap,
                  A.prow_cap, b);
@@ -452,8 +408,17 @@ static int cv_pred_launch(CvPredArgs a, int variant, int nprob,
     while (waves > 1 && waves * per > 65536) waves--;
     // chunking assumed kCvCoopWaves waves per logical block; fewer waves
     // per block only lengthen each wave's walk over the same run
-    hipLaunchKernelGGL(k_cv_pred_coop, grid, dim3(64 * waves), waves * per,
-                       st, a);
+    const dim3 blk(64 * waves);
+    switch (wave_rows_for(a.m)) {
+      case 16:
+        hipLaunchKernelGGL(k_cv_pred_coop<16>, grid, blk, waves * per, st, a);
+        break;
+      case 32:
+        hipLaunchKernelGGL(k_cv_pred_coop<32>, grid, blk, waves * per, st, a);
+        break;
+      default:
+        hipLaunchKernelGGL(k_cv_pred_coop<64>, grid, blk, waves * per, st, a);
+    }
   }
   return (int)hipGetLastError();
 }
@@ -537,7 +502,7 @@ int CvGrid::create(const double *yv, const double *X, int p_, int n_,
                        f, hf[t]);
       held[hf[t]] = 1;
     }
-    int r = 0, t = 0;
+    int r = 0;
     for (int i = 0; i < n; i++) {
       if (held[i]) continue;
       cm[r] = coords[i];
@@ -552,7 +517,6 @@ int CvGrid::create(const double *yv, const double *X, int p_, int n_,
       c0[(size_t)nho + s] = coords[(size_t)n + i];
       for (int j = 0; j < p; j++) hx0[(size_t)j * nho + s] = X[(size_t)j * n + i];
       hy0[s] = yv[i];
-      t++;
     }
     NNGP_HIP(hipMemcpyAsync(Z.as<double>() + (size_t)f * nmod * zs, hz.data(),
                             sizeof(double) * hz.size(), hipMemcpyHostToDevice,

@@ -14,6 +14,7 @@
 //   y0 = x0.beta + w.(y_N - X_N beta),  u = x0 - X_N^T w,
 //   var = b (u^T Vb u + 1 + alpha - w.c0) / (a - 1).
 #include "nngp_kernels.hpp"
+#include "nngp_wave.hpp"
 
 namespace nngp {
 
@@ -176,55 +177,7 @@ static int pred_coop_waves(int m) {
   return w < 1 ? 1 : (w > kPredCoopWaves ? kPredCoopWaves : w);
 }
 
-__device__ __forceinline__ void wave_sync_p() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ double readlane_dp(double v, int lane) {
-  int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-  int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-  return __hiloint2double(hi, lo);
-}
-
-// Factor and solve the cnt x cnt system in Am (row-major, ld) with
-// right-hand side held one entry per lane in x; returns x = C^-1 c per lane
-// and vv = c^T C^-1 c.  All lanes must call it.
-__device__ inline bool wave_chol_solve(double *Am, int ld, int cnt, int t,
-                                       double &x, double &vv) {
-  for (int j = 0; j < cnt; j++) {
-    const double piv = Am[j * ld + j];
-    if (!(piv > 0.0)) return false;
-    const double r = nn_rsqrt(piv);
-    double ltj = 0.0;
-    if (t > j && t < cnt) {
-      ltj = Am[t * ld + j] * r;
-      Am[t * ld + j] = ltj;
-    }
-    wave_sync_p();
-    if (t == j) Am[j * ld + j] = r;
-    if (t > j && t < cnt)
-      for (int l = j + 1; l <= t; l++)
-        Am[t * ld + l] = fma(-ltj, Am[l * ld + j], Am[t * ld + l]);
-    wave_sync_p();
-  }
-  for (int k = 0; k < cnt; k++) {
-    const double vk = readlane_dp(x, k) * Am[k * ld + k];
-    if (t == k) x = vk;
-    if (t > k && t < cnt) x = fma(-Am[t * ld + k], vk, x);
-  }
-  double s = t < cnt ? x * x : 0.0;
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-  vv = s;
-  for (int k = cnt - 1; k >= 0; k--) {
-    const double bk = readlane_dp(x, k) * Am[k * ld + k];
-    if (t == k) x = bk;
-    if (t < k) x = fma(-Am[k * ld + t], bk, x);
-  }
-  return true;
-}
-
+template <int MM>
 __global__ __launch_bounds__(64 * kPredCoopWaves) void k_pred_coop(
     PredArgs A) {
   extern __shared__ double smem[];
@@ -247,7 +200,7 @@ __global__ __launch_bounds__(64 * kPredCoopWaves) void k_pred_coop(
     px[t] = pt.x;
     py[t] = pt.y;
   }
-  wave_sync_p();
+  wave_sync();
   const double2 s0 = R.xy0[i];
   const int np = m * (m + 1) / 2;
   for (int p = t; p < np; p += 64) {
@@ -258,11 +211,11 @@ __global__ __launch_bounds__(64 * kPredCoopWaves) void k_pred_coop(
     Am[k * ld + l] = k == l ? c.diag
                             : s2_corr_rt(sqdist(px[k], py[k], px[l], py[l]), c);
   }
-  double x = 0.0;
-  if (t < m) x = s2_corr_rt(sqdist(px[t], py[t], s0.x, s0.y), c);
-  wave_sync_p();
-  double vv;
-  const bool ok = wave_chol_solve(Am, ld, m, t, x, vv);
+  const double c0 =
+      t < m ? s2_corr_rt(sqdist(px[t], py[t], s0.x, s0.y), c) : 0.0;
+  wave_sync();
+  double x, vv;
+  const bool ok = wave_chol_rows<MM>(Am, ld, m, t, c0, x, vv);
   if (!ok) {
     if (t == 0) atomicMin(R.err, i);
     return;
@@ -274,7 +227,7 @@ __global__ __launch_bounds__(64 * kPredCoopWaves) void k_pred_coop(
     for (int j = 0; j < A.p; j++) xb += A.X[(size_t)j * A.n + id] * bs[j];
     part = x * (A.y[id] - xb);
   }
-  for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+  part = wave_sum(part);
   if (t == 0) {
     double x0b = 0.0;
     for (int j = 0; j < A.p; j++) x0b += R.X0[(size_t)j * R.q + i] * bs[j];
@@ -310,8 +263,16 @@ int pred_launch(const PredArgs &a, hipStream_t st) {
     const int nw = pred_coop_waves(a.m);
     const dim3 grid(a.nsite, (a.S + nw - 1) / nw);
     const size_t shm = (size_t)nw * (a.m * (a.m + 1) + 2 * a.m) * sizeof(double);
-    hipLaunchKernelGGL(k_pred_coop, grid, dim3(64 * nw), shm, st,
-                       a);
+    switch (wave_rows_for(a.m)) {
+      case 16:
+        hipLaunchKernelGGL(k_pred_coop<16>, grid, dim3(64 * nw), shm, st, a);
+        break;
+      case 32:
+        hipLaunchKernelGGL(k_pred_coop<32>, grid, dim3(64 * nw), shm, st, a);
+        break;
+      default:
+        hipLaunchKernelGGL(k_pred_coop<64>, grid, dim3(64 * nw), shm, st, a);
+    }
   }
   return (int)hipGetLastError();
 }
@@ -386,6 +347,7 @@ __global__ __launch_bounds__(kPredBlock) void k_conj_pred_lane(ConjPredArgs A) {
   if (!ok) atomicMin(A.err, i);
 }
 
+template <int MM>
 __global__ __launch_bounds__(64 * kPredCoopWaves) void k_conj_pred_coop(
     ConjPredArgs A) {
   extern __shared__ double smem[];
@@ -404,7 +366,7 @@ __global__ __launch_bounds__(64 * kPredCoopWaves) void k_conj_pred_coop(
     px[t] = pt.x;
     py[t] = pt.y;
   }
-  wave_sync_p();
+  wave_sync();
   const double2 s0 = A.xy0[i];
   const int np = m * (m + 1) / 2;
   for (int p = t; p < np; p += 64) {
@@ -415,12 +377,11 @@ __global__ __launch_bounds__(64 * kPredCoopWaves) void k_conj_pred_coop(
     Am[k * ld + l] = k == l ? c.diag
                             : s2_corr_rt(sqdist(px[k], py[k], px[l], py[l]), c);
   }
-  double c0 = 0.0;
-  if (t < m) c0 = s2_corr_rt(sqdist(px[t], py[t], s0.x, s0.y), c);
-  double x = c0;
-  wave_sync_p();
-  double vv;
-  if (!wave_chol_solve(Am, ld, m, t, x, vv)) {
+  const double c0 =
+      t < m ? s2_corr_rt(sqdist(px[t], py[t], s0.x, s0.y), c) : 0.0;
+  wave_sync();
+  double x, vv;
+  if (!wave_chol_rows<MM>(Am, ld, m, t, c0, x, vv)) {
     if (t == 0) atomicMin(A.err, i);
     return;
   }
@@ -429,16 +390,14 @@ __global__ __launch_bounds__(64 * kPredCoopWaves) void k_conj_pred_coop(
   double u[kPredMaxP];
   for (int j = 0; j < p; j++) {
     const double xj = t < m ? A.X[(size_t)j * A.n + id] : 0.0;
-    double uj = x * xj;
-    for (int o = 32; o > 0; o >>= 1) uj += __shfl_xor(uj, o);
-    u[j] = uj;
+    u[j] = wave_sum(x * xj);
   }
   if (t < m) {
     double xb = 0.0;
     for (int j = 0; j < p; j++) xb += A.X[(size_t)j * A.n + id] * A.beta[j];
     part = x * (A.y[id] - xb);
   }
-  for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+  part = wave_sum(part);
   if (t == 0) {
     double yhat = 0.0;
     for (int j = 0; j < p; j++) yhat += A.X0[(size_t)j * A.n0 + i] * A.beta[j];
@@ -483,8 +442,19 @@ int conj_pred_launch(const ConjPredArgs &a, hipStream_t st) {
     const int nw = pred_coop_waves(a.m);
     const dim3 grid((a.n0 + nw - 1) / nw);
     const size_t shm = (size_t)nw * (a.m * (a.m + 1) + 2 * a.m) * sizeof(double);
-    hipLaunchKernelGGL(k_conj_pred_coop, grid, dim3(64 * nw), shm,
-                       st, a);
+    switch (wave_rows_for(a.m)) {
+      case 16:
+        hipLaunchKernelGGL(k_conj_pred_coop<16>, grid, dim3(64 * nw), shm, st,
+                           a);
+        break;
+      case 32:
+        hipLaunchKernelGGL(k_conj_pred_coop<32>, grid, dim3(64 * nw), shm, st,
+                           a);
+        break;
+      default:
+        hipLaunchKernelGGL(k_conj_pred_coop<64>, grid, dim3(64 * nw), shm, st,
+                           a);
+    }
   }
   return (int)hipGetLastError();
 }

@@ -100,6 +100,8 @@ CASES = [
     (800, 10, 2, [1.0, 0.05, 20.0, 2.5]),
     (900, 32, 0, [1.0, 0.05, 10.0, 0.0]),
     (600, 32, 2, [1.0, 0.05, 20.0, 0.7]),
+    (500, 48, 0, [1.0, 0.1, 5.0, 0.0]),
+    (300, 64, 2, [1.0, 0.2, 4.0, 1.2]),
     (400, 3, 0, [1.0, 0.5, 2.0, 0.0]),
 ]
 
@@ -253,7 +255,7 @@ def test_rNNGP_fit_rep_matches_oracle(oracle, nngp):
 # kriging and the conjugate model
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("cov,m", [(0, 15), (0, 10), (3, 8), (1, 12),
-                                   (2, 15), (0, 20)])
+                                   (2, 15), (0, 20), (0, 40)])
 def test_rNNGPPredict_matches_oracle(oracle, nngp, cov, m):
     n, p, q, S = 3000, 3, 300, 70
     print(f"predict cov={cov} m={m}", flush=True)
