@@ -338,9 +338,11 @@ def stress_bench(n, m, iters, seed):
     """The stress case: n sites, 90% from 50 Gaussian clusters (centres
     U[0,1]^2, sd 0.02) and 10% U[0,1]^2, then x stretched by 3; X = [1,
     N(0,1)], beta = (1, 5); w from the NNGP prior (matern nu = 2.5, phi =
-    20, sigma.sq = 1, plus a 1e-6 jitter), tau.sq = 0.05; m neighbors.  nu is sampled, so every
-    pass evaluates the general Bessel-K matern (one site per wavefront).
-    Times `iters` MH iterations after one warm-up iteration."""
+    20, sigma.sq = 1, plus a 1e-6 jitter), tau.sq = 0.05; m neighbors.  nu
+    is sampled, so every pass evaluates the general Bessel-K matern (one
+    site per wavefront).  Times `iters` MH iterations after one warm-up
+    iteration, in FP64 and in the mixed form (each autotuned), and reports
+    the mixed pass's error against the FP64 pass at the generating theta."""
     import spnngp_amd as nn
     rng = np.random.default_rng(seed)
     nc = int(0.9 * n)
@@ -369,28 +371,55 @@ def stress_bench(n, m, iters, seed):
                  sigmaSqStarting=1.0, tauSqStarting=0.05, phiStarting=20.0,
                  nuStarting=2.5, sigmaSqTuning=0.01, tauSqTuning=0.01,
                  phiTuning=0.01, nuTuning=0.01)
-    md = nn.Model(c, idx["nnIndx"], idx["nnIndxLU"], m, "matern", X, y)
-    md.rNNGP(nSamples=1, seed=21, **start)
-    cand, chosen = md.autotune([1.0, 0.05, 20.0, 2.4])
-    md.profile(True)
-    md.timing()
-    t0 = time.perf_counter()
-    out = md.rNNGP(nSamples=iters, seed=22, **start)
-    tm = md.timing()
-    dt = time.perf_counter() - t0
-    md.close()
-    bf_ms = tm["bf_ms"] / max(tm["bf_launches"], 1)
-    pairs = m * (m + 1) // 2
-    return {"n": n, "m": m, "iters": iters, "iters_s": iters / dt,
-            "ms_per_iter": dt / iters * 1e3, "updateBF_ms": bf_ms,
-            "bessel_evals_per_s": n * pairs / (bf_ms * 1e-3),
-            "flops_per_s": n * f_alg(m) / (bf_ms * 1e-3),
-            "bf_tune": {"candidates": [list(t) for t in cand],
-                        "chosen": list(chosen)},
-            "accept": out["accept"], "knn_s": knn_s,
-            "precision": "fp64 (the mixed form covers the per-lane kernels, "
-                         "m <= 16, closed-form covariances; this case runs "
-                         "FP64 only)"}
+    th0 = [1.0, 0.05, 20.0, 2.5]
+    beta0 = start["betaStarting"]
+    legs = {}
+    ref = None
+    for prec, name in ((0, "fp64"), (1, "mixed")):
+        md = nn.Model(c, idx["nnIndx"], idx["nnIndxLU"], m, "matern", X, y,
+                      precision=prec)
+        md.rNNGP(nSamples=1, seed=21, **start)
+        cand, chosen = md.autotune([1.0, 0.05, 20.0, 2.4])
+        # one pass at the generating theta: the mixed form's error against
+        # the FP64 pass on the same inputs
+        po = md.update(th0, slot=0, beta=beta0)
+        ll = -0.5 * po["logDet"] - 0.5 * po["qResid"]
+        B, F = md.get_bf(0)
+        md.profile(True)
+        md.timing()
+        t0 = time.perf_counter()
+        out = md.rNNGP(nSamples=iters, seed=22, **start)
+        tm = md.timing()
+        dt = time.perf_counter() - t0
+        md.close()
+        bf_ms = tm["bf_ms"] / max(tm["bf_launches"], 1)
+        pairs = m * (m + 1) // 2
+        leg = {"iters_s": iters / dt, "ms_per_iter": dt / iters * 1e3,
+               "updateBF_ms": bf_ms,
+               "bessel_evals_per_s": n * pairs / (bf_ms * 1e-3),
+               "flops_per_s": n * f_alg(m) / (bf_ms * 1e-3),
+               "bf_tune": {"candidates": [list(t) for t in cand],
+                           "chosen": list(chosen)},
+               "accept": out["accept"]}
+        if ref is None:
+            ref = (ll, B, F)
+        else:
+            leg["vs_fp64"] = {
+                "loglik_rel_err": abs(ll - ref[0]) / abs(ref[0]),
+                "max_rel_dF": float(np.max(np.abs(F - ref[2]) / ref[2])),
+                "max_dB_over_max_B": float(np.max(np.abs(B - ref[1]))
+                                           / np.max(np.abs(ref[1])))}
+        del B, F
+        legs[name] = leg
+    del ref
+    fp, mx = legs["fp64"], legs["mixed"]
+    return {"n": n, "m": m, "iters": iters, "knn_s": knn_s,
+            # FP64 leg at the top level, the mixed leg beside it
+            **fp, "mixed": mx,
+            "mixed_speedup": mx["iters_s"] / fp["iters_s"],
+            "precision": "fp64 and mixed (FP32 build incl. Bessel K, FP32 "
+                         "factor and solves, one FP64 refinement), each "
+                         "autotuned"}
 
 
 def main():

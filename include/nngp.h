@@ -259,14 +259,17 @@ int nngp_model_rNNGP(nngp_model *model, const double *sigmaSqIG,
 int nngp_model_profile(nngp_model *model, int enable);
 int nngp_model_timing(nngp_model *model, double *t);
 
-/* Launch shapes of the updateBF pass timed by the autotuner. */
+/* Launch shapes of the updateBF pass timed by the autotuner (at most
+ * NNGP_TUNE_MAX reported). */
+#define NNGP_TUNE_MAX 12
 typedef struct nngp_tune_report {
-  int n;                      /* candidates timed                         */
-  int family[8];              /* 0: one site per lane, 1: per wavefront   */
-  int threads[8];             /* threads per block                        */
-  int mode[8];                /* NNGP_MIXED models: 1 = covariance copy
-                                 kept in LDS for the refinement residual  */
-  double ms[8];               /* mean launch time                         */
+  int n;                          /* candidates timed                     */
+  int family[NNGP_TUNE_MAX];      /* 0: one site per lane, 1: per wavefront */
+  int threads[NNGP_TUNE_MAX];     /* threads per block                    */
+  int mode[NNGP_TUNE_MAX];        /* NNGP_MIXED per-lane kernel: 1 = the
+                                     covariance copy kept in LDS for the
+                                     refinement residual                  */
+  double ms[NNGP_TUNE_MAX];       /* mean launch time                     */
   int best_family, best_threads, best_mode;
 } nngp_tune_report;
 
@@ -395,8 +398,8 @@ typedef struct nngp_cvgrid_detail {
   double graph_ms;            /* device time of the run's graph             */
   int bf_family, bf_threads;  /* launch shape in use (0: site per lane)    */
   int tune_n;                 /* candidates timed at the last tuning        */
-  int tune_family[8], tune_threads[8];
-  double tune_ms[8];
+  int tune_family[NNGP_TUNE_MAX], tune_threads[NNGP_TUNE_MAX];
+  double tune_ms[NNGP_TUNE_MAX];
 } nngp_cvgrid_detail;
 
 int nngp_cvgrid_create(nngp_cvgrid **out, const double *y, const double *X,

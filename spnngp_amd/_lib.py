@@ -25,6 +25,7 @@ NNGP_ERR_TIMEOUT = 8
 
 NNGP_FP64 = 0
 NNGP_MIXED = 1
+TUNE_MAX = 12  # NNGP_TUNE_MAX of include/nngp.h
 
 # Every symbol declared in include/nngp.h.
 EXPORTS = (
@@ -90,14 +91,15 @@ class NngpCvGridDetail(C.Structure):
                 ("bBInv", C.c_void_p), ("y0Hat", C.c_void_p),
                 ("y0HatVar", C.c_void_p), ("graph_ms", C.c_double),
                 ("bf_family", C.c_int), ("bf_threads", C.c_int),
-                ("tune_n", C.c_int), ("tune_family", C.c_int * 8),
-                ("tune_threads", C.c_int * 8), ("tune_ms", C.c_double * 8)]
+                ("tune_n", C.c_int), ("tune_family", C.c_int * TUNE_MAX),
+                ("tune_threads", C.c_int * TUNE_MAX),
+                ("tune_ms", C.c_double * TUNE_MAX)]
 
 
 class NngpTuneReport(C.Structure):
-    _fields_ = [("n", C.c_int), ("family", C.c_int * 8),
-                ("threads", C.c_int * 8), ("mode", C.c_int * 8),
-                ("ms", C.c_double * 8), ("best_family", C.c_int),
+    _fields_ = [("n", C.c_int), ("family", C.c_int * TUNE_MAX),
+                ("threads", C.c_int * TUNE_MAX), ("mode", C.c_int * TUNE_MAX),
+                ("ms", C.c_double * TUNE_MAX), ("best_family", C.c_int),
                 ("best_threads", C.c_int), ("best_mode", C.c_int)]
 
 

@@ -913,7 +913,7 @@ int nngp_model_autotune(nngp_model *model, const double *theta,
   NNGP_TRY(md.autotune(c, &rep));
   if (report) {
     report->n = rep.n;
-    for (int k = 0; k < 8; k++) {
+    for (int k = 0; k < NNGP_TUNE_MAX; k++) {
       report->family[k] = k < rep.n ? rep.family[k] : 0;
       report->threads[k] = k < rep.n ? rep.threads[k] : 0;
       report->mode[k] = k < rep.n ? rep.mode[k] : 0;

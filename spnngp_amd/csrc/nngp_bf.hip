@@ -405,13 +405,15 @@ __device__ inline bool coop_site(const BfArgs &A, const BfProblem &P,
                                  double *Am, int i, double &F,
                                  double (&e)[kMaxNZ], double &bcoef,
                                  int &cnt_out, int &off_out) {
-  const int t = (int)threadIdx.x & 63;
+  const int t = lane_opaque((int)threadIdx.x & 63);
   const int ld = A.m + 1;
   double *cv = Am + A.m * ld;
   double *px = cv + A.m;
   double *py = px + A.m;
-  const int cnt = P.lu[A.n + i];
-  const int off = P.lu[i];
+  // wave-uniform (one site per wavefront): scalar registers, so the
+  // guards on cnt below are scalar branches, not exec-mask changes
+  const int cnt = __builtin_amdgcn_readfirstlane(P.lu[A.n + i]);
+  const int off = __builtin_amdgcn_readfirstlane(P.lu[i]);
   cnt_out = cnt;
   off_out = off;
   bcoef = 0.0;
@@ -472,13 +474,135 @@ __device__ inline bool coop_site(const BfArgs &A, const BfProblem &P,
   return true;
 }
 
+// Per-wave LDS of the mixed wavefront kernel, in floats: C's lower triangle
+// and then its factor (m rows of m + 1), the vector c and the neighbor
+// offsets.
+__host__ __device__ inline int coop_mixed_wave_floats(int m) {
+  return m * (m + 1) + 3 * m;
+}
+
+// coop_site in mixed precision (the per-lane k_bf_mixed's scheme, one site
+// per wavefront): neighbor offsets from site i and the covariance (any
+// covModel, the matern by xnu_bessel_k32) in FP32, the FP32 factor one row
+// per lane in registers, B0 = C32^-1 c32 in FP32, the residual
+// r = c32 - C32 B0 of that FP32 system in FP64 (products of FP32 values are
+// exact in FP64; lane t keeps its whole row of C32 in registers for it),
+// one FP32 correction solve B = B0 + C32^-1 r, and F = s2 + t2 - c32.B and
+// the residuals e in FP64.  False when a pivot or F is not positive.
+template <int MM>
+__device__ inline bool coop_site_mixed(const BfArgs &A, const BfProblem &P,
+                                       const CovArgs &c,
+                                       const CovArgs32 &c32,
+                                       const BesselTab32 &bt, float *W, int i,
+                                       double &F, double (&e)[kMaxNZ],
+                                       double &bcoef, int &cnt_out,
+                                       int &off_out) {
+  const int t = lane_opaque((int)threadIdx.x & 63);
+  const int ld = A.m + 1;
+  float *Lm = W;
+  float *cv = W + A.m * ld;
+  float *px = cv + A.m;
+  float *py = px + A.m;
+  // wave-uniform (one site per wavefront): scalar registers, so the
+  // guards on cnt below are scalar branches, not exec-mask changes
+  const int cnt = __builtin_amdgcn_readfirstlane(P.lu[A.n + i]);
+  const int off = __builtin_amdgcn_readfirstlane(P.lu[i]);
+  cnt_out = cnt;
+  off_out = off;
+  bcoef = 0.0;
+  if (cnt == 0) {
+    F = c.diag;
+#pragma unroll
+    for (int j = 0; j < kMaxNZ; j++)
+      e[j] = j < A.nz ? P.Z[(size_t)i * A.zs + j] : 0.0;
+    return true;
+  }
+  const double2 s = P.xy[i];
+  int myid = 0;
+  if (t < cnt) {
+    myid = P.nnIndx[off + t];
+    const double2 q = P.xy[myid];
+    px[t] = (float)(q.x - s.x);
+    py[t] = (float)(q.y - s.y);
+  }
+  wave_sync();
+  // augmented lower triangle: rows 0..cnt-1 are C, row cnt is c (site i
+  // sits at the origin of the offsets)
+  const int np = (cnt + 1) * (cnt + 2) / 2 - 1;
+  for (int p = t; p < np; p += 64) {
+    const int k = tri_row(p);
+    const int l = p - k * (k + 1) / 2;
+    if (k < cnt && k == l) {
+      Lm[k * ld + k] = c32.diag;
+    } else {
+      const float ex = (k < cnt ? px[k] : 0.0f) - px[l];
+      const float ey = (k < cnt ? py[k] : 0.0f) - py[l];
+      const float val = s2_corr32_rt(fmaf(ex, ex, ey * ey), c32, bt);
+      if (k < cnt)
+        Lm[k * ld + l] = val;
+      else
+        cv[l] = val;
+    }
+  }
+  wave_sync();
+  float crow[MM], row[MM];
+#pragma unroll
+  for (int l = 0; l < MM; l++) {
+    crow[l] = (t < cnt && l < cnt)
+                  ? (l <= t ? Lm[t * ld + l] : Lm[l * ld + t])
+                  : 0.0f;
+    row[l] = l <= t ? crow[l] : 0.0f;
+  }
+  const float ct = t < cnt ? cv[t] : 0.0f;
+  wave_sync();  // C is in registers now; the factor takes its place
+  bool ok = wave_factor32<MM>(row, cnt, t);
+  double x = 0.0;
+  if (ok) {
+#pragma unroll
+    for (int l = 0; l < MM; l++)
+      if (t < cnt && l <= t) Lm[t * ld + l] = row[l];
+    wave_sync();
+    const float x0 = wave_solve32<MM>(row, Lm, ld, cnt, t, ct);
+    double r = (double)ct;
+#pragma unroll
+    for (int l = 0; l < MM; l++)
+      if (l < cnt) r = fma(-(double)crow[l], (double)readlane_f(x0, l), r);
+    const float d =
+        wave_solve32<MM>(row, Lm, ld, cnt, t, t < cnt ? (float)r : 0.0f);
+    x = t < cnt ? (double)x0 + (double)d : 0.0;
+    F = c.diag - wave_sum((double)ct * x);
+    ok = F > 0.0;
+  }
+  if (!ok) {
+    F = 1.0;  // keeps the sums finite; the pass reports the failure
+#pragma unroll
+    for (int j = 0; j < kMaxNZ; j++) e[j] = 0.0;
+    wave_sync();
+    return false;
+  }
+  bcoef = x;
+#pragma unroll
+  for (int j = 0; j < kMaxNZ; j++) {
+    e[j] = 0.0;
+    if (j < A.nz) {
+      const double part =
+          wave_sum(t < cnt ? x * P.Z[(size_t)myid * A.zs + j] : 0.0);
+      e[j] = P.Z[(size_t)i * A.zs + j] - part;
+    }
+  }
+  wave_sync();  // this LDS area is reused by the wave's next site
+  return true;
+}
+
 // Logical block b covers sites [b*chunk, (b+1)*chunk); its waves take every
 // nw-th site of that run.  A.coop_blocks logical blocks per problem.
-template <bool BATCH, int MM>
+// MIXED: the sites are solved by coop_site_mixed (single problem only).
+template <bool BATCH, int MM, bool MIXED>
 __global__ __launch_bounds__(64 * kCoopMaxWaves) void k_bf_coop(BfArgs A) {
   extern __shared__ double smem[];
   __shared__ double red[kCoopMaxWaves][kStatsMax];
   __shared__ BesselTab bt;
+  __shared__ BesselTab32 bt32;
   const int nw = (int)blockDim.x >> 6;
   const int w = (int)threadIdx.x >> 6;
   const int t = (int)threadIdx.x & 63;
@@ -487,10 +611,16 @@ __global__ __launch_bounds__(64 * kCoopMaxWaves) void k_bf_coop(BfArgs A) {
   const BfProblem P = bf_problem<BATCH>(A);
   const CovArgs c = BATCH ? A.covs[(int)blockIdx.y % A.ng] : A.c;
   if (c.cov == 2) {  // the matern's loop divisors for this block's mu
-    bessel_tab_fill(bt, c);
+    if constexpr (MIXED)
+      bessel_tab_fill32(bt32, c);
+    else
+      bessel_tab_fill(bt, c);
     __syncthreads();
   }
   double *Am = smem + (size_t)w * coop_wave_doubles(A.m);
+  float *Wf = reinterpret_cast<float *>(smem) +
+              (size_t)w * coop_mixed_wave_floats(A.m);
+  const CovArgs32 c32 = cov32(c);
   const int chunk = (A.n + A.coop_blocks - 1) / A.coop_blocks;
   const int i0 = b * chunk;
   const int i1 = min(A.n, i0 + chunk);
@@ -502,8 +632,12 @@ __global__ __launch_bounds__(64 * kCoopMaxWaves) void k_bf_coop(BfArgs A) {
   for (int i = i0 + w; i < i1; i += nw) {
     double F, e[kMaxNZ], bcoef;
     int cnt, off;
-    const bool ok =
-        coop_site<MM>(A, P, c, bt, Am, i, F, e, bcoef, cnt, off);
+    bool ok;
+    if constexpr (MIXED)
+      ok = coop_site_mixed<MM>(A, P, c, c32, bt32, Wf, i, F, e, bcoef, cnt,
+                               off);
+    else
+      ok = coop_site<MM>(A, P, c, bt, Am, i, F, e, bcoef, cnt, off);
     if (!ok && t == 0) atomicMin(P.err, i);
     if (!BATCH) {
       if (A.B && ok && t < cnt) A.B[off + t] = bcoef;
@@ -695,18 +829,18 @@ static void launch_lane(const BfArgs &a, int variant, const dim3 &grid,
 
 // The wavefront kernel's register-resident factor is sized by the smallest
 // of 16, 32 and 64 rows that holds m.
-template <bool BATCH>
+template <bool BATCH, bool MIXED>
 static void launch_coop(const BfArgs &a, const dim3 &grid, const dim3 &blk,
                         size_t shm, hipStream_t st) {
   switch (wave_rows_for(a.m)) {
     case 16:
-      hipLaunchKernelGGL((k_bf_coop<BATCH, 16>), grid, blk, shm, st, a);
+      hipLaunchKernelGGL((k_bf_coop<BATCH, 16, MIXED>), grid, blk, shm, st, a);
       break;
     case 32:
-      hipLaunchKernelGGL((k_bf_coop<BATCH, 32>), grid, blk, shm, st, a);
+      hipLaunchKernelGGL((k_bf_coop<BATCH, 32, MIXED>), grid, blk, shm, st, a);
       break;
     default:
-      hipLaunchKernelGGL((k_bf_coop<BATCH, 64>), grid, blk, shm, st, a);
+      hipLaunchKernelGGL((k_bf_coop<BATCH, 64, MIXED>), grid, blk, shm, st, a);
   }
 }
 
@@ -728,11 +862,15 @@ int bf_launch(const BfArgs &a0, const BfConfig &cfg, int nprob,
     else launch_lane<false>(a, variant, grid, blk, st);
   } else {
     a.coop_blocks = nrows;
-    const size_t shm = coop_shm(a.m, cfg.threads / 64);
+    const int waves = cfg.threads / 64;
     if (batch)
-      launch_coop<true>(a, grid, blk, shm, st);
+      launch_coop<true, false>(a, grid, blk, coop_shm(a.m, waves), st);
+    else if (a.mixed)
+      launch_coop<false, true>(
+          a, grid, blk,
+          (size_t)waves * coop_mixed_wave_floats(a.m) * sizeof(float), st);
     else
-      launch_coop<false>(a, grid, blk, shm, st);
+      launch_coop<false, false>(a, grid, blk, coop_shm(a.m, waves), st);
   }
   return (int)hipGetLastError();
 }
@@ -774,13 +912,14 @@ int bf_autotune(const BfArgs &a, int nprob, hipStream_t st, BfConfig *best,
     }
   }
   std::vector<BfConfig> cand;
-  if (a.mixed) {  // the mixed kernels: block size x covariance copy in LDS
+  if (a.mixed) {  // the per-lane mixed kernel: block size x covariance
+                  // copy in LDS; then the wavefront one
     for (int th : {64, 128, 256})
       for (int mode : {0, 1}) cand.push_back(BfConfig{0, th, mode});
   } else {
     for (int th : {64, 128, 256}) cand.push_back(BfConfig{0, th, 0});
-    for (int w : {1, 2, 4}) cand.push_back(BfConfig{1, 64 * w, 0});
   }
+  for (int w : {1, 2, 4}) cand.push_back(BfConfig{1, 64 * w, 0});
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess) return -1;
   if (hipEventCreate(&e1) != hipSuccess) {
@@ -804,7 +943,7 @@ int bf_autotune(const BfArgs &a, int nprob, hipStream_t st, BfConfig *best,
     float ms = 0.0f;
     if ((rc = (int)hipEventElapsedTime(&ms, e0, e1)) != 0) break;
     const double per = ms / reps;
-    if (rep && nc < 8) {
+    if (rep && nc < NNGP_TUNE_MAX) {
       rep->family[nc] = cfg.family;
       rep->threads[nc] = cfg.threads;
       rep->mode[nc] = cfg.mode;
@@ -820,7 +959,7 @@ int bf_autotune(const BfArgs &a, int nprob, hipStream_t st, BfConfig *best,
   (void)hipEventDestroy(e1);
   if (rc) return rc;
   if (rep) {
-    rep->n = nc < 8 ? nc : 8;
+    rep->n = nc < NNGP_TUNE_MAX ? nc : NNGP_TUNE_MAX;
     rep->best = pick;
     rep->best_ms = best_ms;
   }

@@ -797,7 +797,7 @@ int CvGrid::run(const double *thetaAlpha, int g, double *scores,
     det->bf_family = cfg.family;
     det->bf_threads = cfg.threads;
     det->tune_n = tune.n;
-    for (int t = 0; t < 8; t++) {
+    for (int t = 0; t < NNGP_TUNE_MAX; t++) {
       det->tune_family[t] = t < tune.n ? tune.family[t] : 0;
       det->tune_threads[t] = t < tune.n ? tune.threads[t] : 0;
       det->tune_ms[t] = t < tune.n ? tune.ms[t] : 0.0;

@@ -413,6 +413,9 @@ __device__ __forceinline__ bool chol_solve(double (&a)[M * (M + 1) / 2],
 // Covariance parameters rounded to FP32 once per launch.
 struct CovArgs32 {
   float s2, phi, inv_phi, phi2, diag;
+  // general matern (cov 2), as in CovArgs
+  float mnorm, mu, g1, g2, gp, gm, fa;
+  int nl, cov;
 };
 
 __device__ __forceinline__ CovArgs32 cov32(const CovArgs &a) {
@@ -422,6 +425,15 @@ __device__ __forceinline__ CovArgs32 cov32(const CovArgs &a) {
   c.inv_phi = (float)a.inv_phi;
   c.phi2 = (float)a.phi2;
   c.diag = (float)a.diag;
+  c.mnorm = (float)a.mnorm;
+  c.mu = (float)a.mu;
+  c.g1 = (float)a.g1;
+  c.g2 = (float)a.g2;
+  c.gp = (float)a.gp;
+  c.gm = (float)a.gm;
+  c.fa = (float)a.fa;
+  c.nl = a.nl;
+  c.cov = a.cov;
   return c;
 }
 
@@ -448,6 +460,152 @@ __device__ __forceinline__ float s2_corr32(float d2, const CovArgs32 &a) {
            (1.0f + x + x * x * (1.0f / 3.0f));
   } else {
     return a.s2 * __builtin_amdgcn_exp2f(-a.phi2 * d2 * kLog2e);
+  }
+}
+
+// The Bessel-K loop divisors of BesselTab in FP32 (each rounded once from
+// its FP64 value).
+constexpr int kBesTab32 = 64;
+struct BesselTab32 {
+  float ik[kBesTab32], ik2[kBesTab32], ikm[kBesTab32], ikp[kBesTab32],
+      iaa[kBesTab32];
+};
+
+// Every thread of the block calls it (then a barrier before use).
+__device__ inline void bessel_tab_fill32(BesselTab32 &t, const CovArgs &a) {
+  const double mu = a.mu;
+  for (int k = (int)threadIdx.x; k < kBesTab32; k += (int)blockDim.x) {
+    const double dk = (double)(k > 0 ? k : 1);
+    t.ik[k] = (float)(1.0 / dk);
+    t.ik2[k] = (float)(1.0 / (dk * dk - mu * mu));
+    t.ikm[k] = (float)(1.0 / (dk - mu));
+    t.ikp[k] = (float)(1.0 / (dk + mu));
+    t.iaa[k] = (float)(1.0 / (mu * mu - 0.25 - dk * (dk - 1.0)));
+  }
+}
+
+// x^nu K_nu(x) in FP32 for 0 < x < 80 (the mixed form's covariance build):
+// Temme's series (x <= 2) or Steed's continued fraction (x > 2) for K_mu
+// and K_(mu+1) as in xnu_bessel_k, stopped at FP32 accuracy, with x^mu (and
+// e^-x) folded into the two starting values a_0 = x^mu K_mu(x),
+// a_1 = x^(mu+1) K_(mu+1)(x).  The upward recurrence K_(v+1) = K_(v-1) +
+// (2v/x) K_v multiplied by x^(v+1) reads a_(j+1) = 2 (mu + j) a_j +
+// x^2 a_(j-1): it needs no 1/x and its terms stay finite as x -> 0, where
+// K_nu alone overflows FP32.  Hardware exp2 / log2 / reciprocal (about
+// 1 ulp each).
+__device__ inline float xnu_bessel_k32(float x, const CovArgs32 &a,
+                                       const BesselTab32 &t) {
+  constexpr float eps = 6e-8f;
+  constexpr float kLn2 = 0.69314718f, kLog2e = 1.44269504f;
+  const float mu = a.mu;
+  const float l2x = __builtin_amdgcn_logf(x);  // log2 x
+  float k0, k1;
+  if (x <= 2.0f) {
+    const float lg = (1.0f - l2x) * kLn2;  // -ln(x / 2)
+    const float sg = mu * lg;
+    const float es = __builtin_amdgcn_exp2f(sg * kLog2e);
+    const float ies = __builtin_amdgcn_rcpf(es);
+    float fs;  // sinh(sg) / sg
+    if (fabsf(sg) < 0.1f) {
+      const float s2 = sg * sg;
+      fs = 1.0f + s2 * (1.0f / 6.0f + s2 * (1.0f / 120.0f + s2 * (1.0f / 5040.0f)));
+    } else {
+      fs = 0.5f * (es - ies) * __builtin_amdgcn_rcpf(sg);
+    }
+    float f = a.fa * (a.g1 * (0.5f * (es + ies)) + a.g2 * fs * lg);
+    float pk = 0.5f * es * a.gp;
+    float qk = 0.5f * ies * a.gm;
+    float ck = 1.0f;
+    const float d = 0.25f * x * x;
+    float s0 = f, s1 = pk;
+    for (int k = 1; k < 100; k++) {
+      float i2, i1, im, ip;
+      const float fk = (float)k;
+      if (k < kBesTab32) {
+        i2 = t.ik2[k];
+        i1 = t.ik[k];
+        im = t.ikm[k];
+        ip = t.ikp[k];
+      } else {
+        i2 = __builtin_amdgcn_rcpf(fk * fk - mu * mu);
+        i1 = __builtin_amdgcn_rcpf(fk);
+        im = __builtin_amdgcn_rcpf(fk - mu);
+        ip = __builtin_amdgcn_rcpf(fk + mu);
+      }
+      f = (fk * f + pk + qk) * i2;
+      ck *= d * i1;
+      pk *= im;
+      qk *= ip;
+      const float t0 = ck * f;
+      s0 += t0;
+      s1 += ck * (pk - fk * f);
+      if (fabsf(t0) < fabsf(s0) * eps) break;
+    }
+    const float xmu = __builtin_amdgcn_exp2f(mu * l2x);  // x^mu
+    k0 = s0 * xmu;
+    k1 = 2.0f * s1 * xmu;  // x^(mu+1) (2 / x) s1
+  } else {
+    float b = 2.0f * (1.0f + x);
+    float dd = __builtin_amdgcn_rcpf(b);
+    float h = dd, dh = dd;
+    float q1 = 0.0f, q2 = 1.0f;
+    const float a1 = 0.25f - mu * mu;
+    float q = a1, c = a1;
+    float aa = -a1;
+    float s = 1.0f + q * dh;
+    for (int i = 2; i < 1000; i++) {
+      aa -= 2.0f * (float)(i - 1);
+      float inv_i, inv_aa;
+      if (i < kBesTab32) {
+        inv_i = t.ik[i];
+        inv_aa = t.iaa[i];
+      } else {
+        inv_i = __builtin_amdgcn_rcpf((float)i);
+        inv_aa = __builtin_amdgcn_rcpf(aa);
+      }
+      c = -aa * c * inv_i;
+      const float qn = (q1 - b * q2) * inv_aa;
+      q1 = q2;
+      q2 = qn;
+      q += c * qn;
+      b += 2.0f;
+      dd = __builtin_amdgcn_rcpf(b + aa * dd);
+      dh = (b * dd - 1.0f) * dh;
+      h += dh;
+      const float ds = q * dh;
+      s += ds;
+      if (fabsf(ds) < fabsf(s) * eps) break;
+    }
+    h = a1 * h;
+    // x^mu K_mu = sqrt(pi / 2) x^(mu - 1/2) e^-x / s
+    k0 = 1.25331414f *
+         __builtin_amdgcn_exp2f((mu - 0.5f) * l2x - x * kLog2e) *
+         __builtin_amdgcn_rcpf(s);
+    k1 = k0 * (mu + x + 0.5f - h);
+  }
+  const float x2 = x * x;
+  for (int j = 1; j <= a.nl; j++) {
+    const float kn = fmaf(2.0f * (mu + (float)j), k1, x2 * k0);
+    k0 = k1;
+    k1 = kn;
+  }
+  return k0;
+}
+
+// s2 * rho in FP32 for any covModel (the matern through xnu_bessel_k32),
+// from an FP32 squared distance.
+__device__ inline float s2_corr32_rt(float d2, const CovArgs32 &a,
+                                     const BesselTab32 &t) {
+  switch (a.cov) {
+    case 0: return s2_corr32<0>(d2, a);
+    case 1: return s2_corr32<1>(d2, a);
+    case 2: {
+      const float x = __builtin_amdgcn_sqrtf(d2) * a.phi;
+      if (!(x > 0.0f)) return a.s2;
+      if (!(x < 80.0f)) return 0.0f;  // below 1e-30 of s2
+      return a.mnorm * xnu_bessel_k32(x, a, t);
+    }
+    default: return s2_corr32<3>(d2, a);
   }
 }
 

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "../../include/nngp.h"
 #include "nngp_device.hpp"
 
 namespace nngp {
@@ -53,9 +54,9 @@ struct BfConfig {
 };
 
 struct BfTuneReport {
-  int n = 0;             // candidates timed
-  int family[8], threads[8], mode[8];
-  double ms[8];          // mean launch time of each
+  int n = 0;             // candidates timed (at most NNGP_TUNE_MAX kept)
+  int family[NNGP_TUNE_MAX], threads[NNGP_TUNE_MAX], mode[NNGP_TUNE_MAX];
+  double ms[NNGP_TUNE_MAX];  // mean launch time of each
   BfConfig best;
   double best_ms = 0.0;
 };

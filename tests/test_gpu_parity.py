@@ -376,6 +376,60 @@ def test_mixed_updateBF_close_to_oracle(oracle, nngp, cov, m):
     assert abs(out2["logDet"] - out["logDet"]) < 1e-9 * abs(out["logDet"])
 
 
+# The wavefront form (m > 16, or the Bessel-K matern at any m): FP32
+# covariance build, including an FP32 Bessel K (Temme / Steed to 6e-8, a
+# few ulp per entry), FP32 factor with one row per lane, one FP64 residual.
+# Entries carry ~1e-6 relative error, the systems kappa ~ 1e2 (1e3 for the
+# clustered sites), hence the bounds.
+@pytest.mark.gpu
+@pytest.mark.parametrize("cov,m,theta,clustered", [
+    (0, 32, [2.0, 0.2, 12.0], False),
+    (1, 32, [2.0, 0.2, 3.0], False),
+    (3, 20, [1.0, 0.3, 5.0], False),
+    (2, 15, [2.0, 0.2, 12.0, 1.2], False),
+    (2, 20, [1.0, 0.1, 8.0, 0.7], False),
+    (2, 32, [2.0, 0.2, 12.0, 2.5], False),
+    (2, 32, [1.0, 0.05, 20.0, 2.5], True),
+    (0, 48, [1.0, 0.1, 5.0], False),
+])
+def test_mixed_wavefront_close_to_oracle(oracle, nngp, cov, m, theta,
+                                         clustered):
+    print(f"mixed wavefront cov={cov} m={m} clustered={clustered}",
+          flush=True)
+    n = 4000
+    rng = np.random.default_rng(40 + m + cov)
+    if clustered:  # 50 tight clusters plus 10% uniform, x stretched by 3
+        nc = int(0.9 * n)
+        ce = rng.uniform(size=(50, 2))
+        c = np.vstack([ce[rng.integers(0, 50, nc)]
+                       + rng.normal(scale=0.02, size=(nc, 2)),
+                       rng.uniform(size=(n - nc, 2))])
+        c[:, 0] *= 3.0
+    else:
+        c = rng.uniform(size=(n, 2))
+    c = c[np.argsort(c[:, 0], kind="stable")]
+    X = np.column_stack([np.ones(n), rng.normal(size=n)])
+    y = X @ [1.0, -1.0] + rng.normal(size=n)
+    nn = oracle.mkNNIndxCB(c, m)
+    B0, F0, ld0 = oracle.updateBF(c.T.reshape(-1).copy(), nn["nnIndx"],
+                                  nn["nnIndxLU"], n, m, theta, cov)
+    beta = np.array([1.0, -1.0])
+    r = X @ beta - y
+    q0 = oracle.Q(B0, F0, r, r, nn["nnIndx"], nn["nnIndxLU"])
+    with nngp.Model(c, nn["nnIndx"], nn["nnIndxLU"], m, cov, X, y,
+                    precision=1) as md:
+        out = md.update(theta, slot=0, beta=beta)
+        B, F = md.get_bf(0)
+        md.autotune(theta)
+        out2 = md.update(theta, slot=1, beta=beta)
+    ll0 = -0.5 * ld0 - 0.5 * q0
+    ll = -0.5 * out["logDet"] - 0.5 * out["qResid"]
+    assert abs(ll - ll0) / abs(ll0) < 1e-5
+    assert np.max(np.abs(F - F0) / F0) < 5e-5
+    assert np.max(np.abs(B - B0)) / np.max(np.abs(B0)) < 1e-4
+    assert abs(out2["logDet"] - out["logDet"]) < 1e-9 * abs(out["logDet"])
+
+
 @pytest.mark.gpu
 def test_mixed_chain_matches_fp64_chain(oracle, nngp):
     """Same RNG tape: the mixed chain's posterior summaries equal the FP64
