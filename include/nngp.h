@@ -70,7 +70,11 @@ typedef struct nngp_opts {
   int (*interrupt)(void *); /* polled once per MCMC iteration / grid point;
                                nonzero stops with NNGP_ERR_INTERRUPT        */
   void *interrupt_arg;
-  int reserved[8];
+  int shape[3];             /* updateBF launch shape (family, threads, mode
+                               as in nngp_tune_report) pinned for a model or
+                               a grid context; threads == 0: the default,
+                               or the autotuner's choice                    */
+  int reserved[5];
 } nngp_opts;
 
 /* Message for the last nonzero status returned on this host thread. */
@@ -266,7 +270,7 @@ typedef struct nngp_tune_report {
   int n;                          /* candidates timed                     */
   int family[NNGP_TUNE_MAX];      /* 0: one site per lane, 1: per wavefront */
   int threads[NNGP_TUNE_MAX];     /* threads per block                    */
-  int mode[NNGP_TUNE_MAX];        /* NNGP_MIXED per-lane kernel: 1 = the
+  int mode[NNGP_TUNE_MAX];        /* family 0, NNGP_MIXED: 1 = the
                                      covariance copy kept in LDS for the
                                      refinement residual                  */
   double ms[NNGP_TUNE_MAX];       /* mean launch time                     */
@@ -281,6 +285,14 @@ typedef struct nngp_tune_report {
  * per-lane kernel applies). */
 int nngp_model_autotune(nngp_model *model, const double *theta,
                         nngp_tune_report *report);
+
+/* Pin the launch shape of this model's updateBF passes (family, threads,
+ * mode as in nngp_tune_report), e.g. one chosen by an earlier autotune, so
+ * that chains are bit-reproducible across processes.  NNGP_ERR_ARG when the
+ * shape does not apply to the model (m, covModel, precision).  No
+ * counterpart in the reference, which has a single CPU code path. */
+int nngp_model_set_shape(nngp_model *model, int family, int threads,
+                         int mode);
 
 /* ------------------------------------------------------------------------
  * Kriging server: rNNGPPredict (src/rNNGPPredict.cpp:25-222) together with
@@ -396,9 +408,11 @@ typedef struct nngp_cvgrid_detail {
   double *y0Hat, *y0HatVar;   /* nHo x (k g)                                */
   /* filled by every run: */
   double graph_ms;            /* device time of the run's graph             */
-  int bf_family, bf_threads;  /* launch shape in use (0: site per lane)    */
+  int bf_family, bf_threads, bf_mode;  /* launch shape in use, as in
+                                          nngp_tune_report                */
   int tune_n;                 /* candidates timed at the last tuning        */
-  int tune_family[NNGP_TUNE_MAX], tune_threads[NNGP_TUNE_MAX];
+  int tune_family[NNGP_TUNE_MAX], tune_threads[NNGP_TUNE_MAX],
+      tune_mode[NNGP_TUNE_MAX];
   double tune_ms[NNGP_TUNE_MAX];
 } nngp_cvgrid_detail;
 

@@ -40,7 +40,7 @@ EXPORTS = (
     "nngp_server_wait", "nngp_server_release", "nngp_server_predict",
     "nngp_server_get_stats",
     "nngp_cvgrid_create", "nngp_cvgrid_run", "nngp_cvgrid_destroy",
-    "nngp_model_autotune",
+    "nngp_model_autotune", "nngp_model_set_shape",
 )
 
 _RNG_FN = C.CFUNCTYPE(C.c_double, C.c_void_p)
@@ -55,7 +55,8 @@ class NngpRng(C.Structure):
 class NngpOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("device", C.c_int),
                 ("seed", C.c_ulonglong), ("interrupt", _INT_FN),
-                ("interrupt_arg", C.c_void_p), ("reserved", C.c_int * 8)]
+                ("interrupt_arg", C.c_void_p), ("shape", C.c_int * 3),
+                ("reserved", C.c_int * 5)]
 
 
 class NngpPassOut(C.Structure):
@@ -91,8 +92,10 @@ class NngpCvGridDetail(C.Structure):
                 ("bBInv", C.c_void_p), ("y0Hat", C.c_void_p),
                 ("y0HatVar", C.c_void_p), ("graph_ms", C.c_double),
                 ("bf_family", C.c_int), ("bf_threads", C.c_int),
+                ("bf_mode", C.c_int),
                 ("tune_n", C.c_int), ("tune_family", C.c_int * TUNE_MAX),
                 ("tune_threads", C.c_int * TUNE_MAX),
+                ("tune_mode", C.c_int * TUNE_MAX),
                 ("tune_ms", C.c_double * TUNE_MAX)]
 
 
@@ -189,6 +192,7 @@ def load():
                                   C.POINTER(NngpCvGridDetail)]
     L.nngp_cvgrid_destroy.argtypes = [_vp]
     L.nngp_model_autotune.argtypes = [_vp, _dp, C.POINTER(NngpTuneReport)]
+    L.nngp_model_set_shape.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
     L.nngp_cvgrid_destroy.restype = None
     _lib = L
     return _lib

@@ -926,6 +926,13 @@ int nngp_model_autotune(nngp_model *model, const double *theta,
   return NNGP_OK;
 }
 
+int nngp_model_set_shape(nngp_model *model, int family, int threads,
+                         int mode) {
+  clear_err();
+  if (!model) return set_err(NNGP_ERR_ARG, "bad arguments");
+  return model->md.pin_shape(family, threads, mode);
+}
+
 int nngp_model_get_bf(nngp_model *model, int slot, double *B, double *F) {
   clear_err();
   if (!model || slot < 0 || slot > 1) return set_err(NNGP_ERR_ARG, "bad arguments");

@@ -148,6 +148,77 @@ __device__ inline void block_site_terms(int nz, double *rows, int rows_cap,
   }
 }
 
+// One site per lane: B_i, F_i and the residual vector e of site i from its
+// neighbor count, offset and index slots (slots past the count are not its
+// neighbors; they are replaced by site 0 here, before any use).
+template <int M, int COV, bool BATCH>
+__device__ __forceinline__ void lane_site(const BfArgs &A, const BfProblem &P,
+                                          const CovArgs &c, int i, int cnt,
+                                          int off, int (&nb)[M], double &Fi,
+                                          double (&e)[kMaxNZ]) {
+  if (cnt == 0) {  // site 0: F = s2 + t2, residual = the data
+    Fi = c.diag;
+#pragma unroll
+    for (int j = 0; j < kMaxNZ; j++)
+      if (j < A.nz) e[j] = P.Z[(size_t)i * A.zs + j];
+    return;
+  }
+  if (cnt < M) {
+#pragma unroll
+    for (int k = 0; k < M; k++)
+      if (k >= cnt) nb[k] = 0;
+  }
+  double px[M], py[M];
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const double2 q = P.xy[nb[k]];
+    px[k] = q.x;
+    py[k] = q.y;
+  }
+  if (cnt < M) {
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+      if (k >= cnt) {  // far away and far apart: zero correlation
+        px[k] = 1e100 * (k + 1);
+        py[k] = -1e100 * (k + 1);
+      }
+    }
+  }
+  const double2 s = P.xy[i];
+
+  constexpr int T = M * (M + 1) / 2;
+  double a[T];  // packed lower triangle
+  double v[M];
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    v[k] = s2_corr<COV>(sqdist(s.x, s.y, px[k], py[k]), c);
+#pragma unroll
+    for (int l = 0; l < k; l++)
+      a[tri(k, l)] = s2_corr<COV>(sqdist(px[k], py[k], px[l], py[l]), c);
+    a[tri(k, k)] = c.diag;
+  }
+  double vv;
+  const bool ok = chol_solve<M>(a, v, vv);  // v <- B_i = C^-1 c
+  if (!BATCH && A.B) {
+#pragma unroll
+    for (int k = 0; k < M; k++)
+      if (k < cnt) A.B[off + k] = v[k];
+  }
+  Fi = c.diag - vv;
+  // residual vectors; padded coefficients are exactly 0
+#pragma unroll
+  for (int j = 0; j < kMaxNZ; j++) {
+    if (j < A.nz) {
+      double ej = P.Z[(size_t)i * A.zs + j];
+#pragma unroll
+      for (int k = 0; k < M; k++)
+        ej = fma(-v[k], P.Z[(size_t)nb[k] * A.zs + j], ej);
+      e[j] = ej;
+    }
+  }
+  if (!ok) atomicMin(P.err, i);
+}
+
 template <int M, int COV, bool BATCH>
 __global__ __launch_bounds__(kLaneMaxThreads) void k_bf_lane(BfArgs A) {
   const int nt = (int)blockDim.x;
@@ -165,73 +236,11 @@ __global__ __launch_bounds__(kLaneMaxThreads) void k_bf_lane(BfArgs A) {
   if (valid) {
     const int cnt = P.lu[A.n + i];
     const int off = P.lu[i];
-    if (cnt == 0) {  // site 0: F = s2 + t2, residual = the data
-      Fi = c.diag;
+    // The index array carries 64 padding entries, so off + k is in bounds.
+    int nb[M];
 #pragma unroll
-      for (int j = 0; j < kMaxNZ; j++)
-        if (j < A.nz) e[j] = P.Z[(size_t)i * A.zs + j];
-    } else {
-      // The index array carries 64 padding entries, so off + k is in
-      // bounds; entries past this site's count are not its neighbors and
-      // are replaced by index 0 before any use.
-      int nb[M];
-#pragma unroll
-      for (int k = 0; k < M; k++) nb[k] = P.nnIndx[off + k];
-      if (cnt < M) {
-#pragma unroll
-        for (int k = 0; k < M; k++)
-          if (k >= cnt) nb[k] = 0;
-      }
-      double px[M], py[M];
-#pragma unroll
-      for (int k = 0; k < M; k++) {
-        const double2 q = P.xy[nb[k]];
-        px[k] = q.x;
-        py[k] = q.y;
-      }
-      if (cnt < M) {
-#pragma unroll
-        for (int k = 0; k < M; k++) {
-          if (k >= cnt) {  // far away and far apart: zero correlation
-            px[k] = 1e100 * (k + 1);
-            py[k] = -1e100 * (k + 1);
-          }
-        }
-      }
-      const double2 s = P.xy[i];
-
-      constexpr int T = M * (M + 1) / 2;
-      double a[T];  // packed lower triangle
-      double v[M];
-#pragma unroll
-      for (int k = 0; k < M; k++) {
-        v[k] = s2_corr<COV>(sqdist(s.x, s.y, px[k], py[k]), c);
-#pragma unroll
-        for (int l = 0; l < k; l++)
-          a[tri(k, l)] = s2_corr<COV>(sqdist(px[k], py[k], px[l], py[l]), c);
-        a[tri(k, k)] = c.diag;
-      }
-      double vv;
-      const bool ok = chol_solve<M>(a, v, vv);  // v <- B_i = C^-1 c
-      if (!BATCH && A.B) {
-#pragma unroll
-        for (int k = 0; k < M; k++)
-          if (k < cnt) A.B[off + k] = v[k];
-      }
-      Fi = c.diag - vv;
-      // residual vectors; padded coefficients are exactly 0
-#pragma unroll
-      for (int j = 0; j < kMaxNZ; j++) {
-        if (j < A.nz) {
-          double ej = P.Z[(size_t)i * A.zs + j];
-#pragma unroll
-          for (int k = 0; k < M; k++)
-            ej = fma(-v[k], P.Z[(size_t)nb[k] * A.zs + j], ej);
-          e[j] = ej;
-        }
-      }
-      if (!ok) atomicMin(P.err, i);
-    }
+    for (int k = 0; k < M; k++) nb[k] = P.nnIndx[off + k];
+    lane_site<M, COV, BATCH>(A, P, c, i, cnt, off, nb, Fi, e);
     if (!BATCH && A.F) A.F[i] = Fi;
   }
   if (P.rows) block_site_terms(A.nz, P.rows, A.rows_cap, valid, Fi, e, b);
@@ -257,8 +266,10 @@ __device__ __forceinline__ float pair32(float dxk, float dyk, float dxl,
 }
 
 template <int M, int COV, bool COPY>
-__global__ __launch_bounds__(kLaneMaxThreads, COPY ? 1 : 2) void k_bf_mixed(BfArgs A) {
-  extern __shared__ float cl[];  // COPY: entry t of lane L at cl[t*nt + L]
+__global__ __launch_bounds__(kLaneMaxThreads, 2) void k_bf_mixed(BfArgs A) {
+  // COPY: covariance entry t of lane L at cl[t*nt + L]; otherwise the
+  // neighbor offsets dx_k at cl[k*nt + L], dy_k at cl[(M+k)*nt + L]
+  extern __shared__ float cl[];
   const int nt = (int)blockDim.x;
   const int b = xcd_block((A.n + nt - 1) / nt);
   if (b < 0) return;
@@ -268,9 +279,8 @@ __global__ __launch_bounds__(kLaneMaxThreads, COPY ? 1 : 2) void k_bf_mixed(BfAr
   const int i = b * nt + L;
   const bool valid = i < A.n;
   double Fi = 1.0;
-  double e[kMaxNZ];
-#pragma unroll
-  for (int j = 0; j < kMaxNZ; j++) e[j] = 0.0;
+  double e[kMaxNZ];  // set on every path just before use (no value is live
+                     // through the factorization)
 
   if (valid) {
     const int cnt = A.lu[A.n + i];
@@ -279,7 +289,7 @@ __global__ __launch_bounds__(kLaneMaxThreads, COPY ? 1 : 2) void k_bf_mixed(BfAr
       Fi = c.diag;
 #pragma unroll
       for (int j = 0; j < kMaxNZ; j++)
-        if (j < A.nz) e[j] = A.Z[(size_t)i * A.zs + j];
+        e[j] = j < A.nz ? A.Z[(size_t)i * A.zs + j] : 0.0;
     } else {
       const double2 s = A.xy[i];
       // as in k_bf_lane: padding entries of the index read as site 0, then
@@ -309,12 +319,21 @@ __global__ __launch_bounds__(kLaneMaxThreads, COPY ? 1 : 2) void k_bf_mixed(BfAr
           }
         }
       }
+      if (!COPY) {  // read back by the residual; registers stay free
+#pragma unroll
+        for (int k = 0; k < M; k++) {
+          cl[k * nt + L] = dx[k];
+          cl[(M + k) * nt + L] = dy[k];
+        }
+      }
       constexpr int T = M * (M + 1) / 2;
       float a[T], v[M];
 #pragma unroll
       for (int k = 0; k < M; k++) {
-        v[k] = s2_corr32<COV>(fmaf(dx[k], dx[k], dy[k] * dy[k]), c32);
-        if (COPY) cl[(T + k) * nt + L] = v[k];
+        if (COPY) {
+          v[k] = s2_corr32<COV>(fmaf(dx[k], dx[k], dy[k] * dy[k]), c32);
+          cl[(T + k) * nt + L] = v[k];
+        }
 #pragma unroll
         for (int l = 0; l < k; l++) {
           a[tri(k, l)] = pair32<COV>(dx[k], dy[k], dx[l], dy[l], c32);
@@ -323,38 +342,62 @@ __global__ __launch_bounds__(kLaneMaxThreads, COPY ? 1 : 2) void k_bf_mixed(BfAr
         a[tri(k, k)] = c32.diag;
       }
       const bool ok = chol_factor32<M>(a);
+      if (!COPY) {  // c after the factorization, from the offsets in LDS
+#pragma unroll
+        for (int k = 0; k < M; k++) {
+          const float xk = cl[k * nt + L], yk = cl[(M + k) * nt + L];
+          v[k] = s2_corr32<COV>(fmaf(xk, xk, yk * yk), c32);
+        }
+      }
       chol_apply32<M>(a, v);  // v <- B0
-      double bd[M], r[M];
+      // residual r = c - C B0 in FP64; B0 stays FP32 (converted where
+      // used), so only r is an FP64 array while the factor is live
+      double r[M];
       const double dg = (double)c32.diag;
 #pragma unroll
       for (int k = 0; k < M; k++) {
-        bd[k] = (double)v[k];
-        const float ck =
-            COPY ? cl[(T + k) * nt + L]
-                 : s2_corr32<COV>(fmaf(dx[k], dx[k], dy[k] * dy[k]), c32);
-        r[k] = fma(-dg, bd[k], (double)ck);
+        float ck;
+        if (COPY) {
+          ck = cl[(T + k) * nt + L];
+        } else {
+          const float xk = cl[k * nt + L], yk = cl[(M + k) * nt + L];
+          ck = s2_corr32<COV>(fmaf(xk, xk, yk * yk), c32);
+        }
+        r[k] = fma(-dg, (double)v[k], (double)ck);
       }
 #pragma unroll
       for (int k = 1; k < M; k++) {
+        float xk = 0.0f, yk = 0.0f;
+        if (!COPY) {
+          xk = cl[k * nt + L];
+          yk = cl[(M + k) * nt + L];
+        }
 #pragma unroll
         for (int l = 0; l < k; l++) {
           const double ckl =
               COPY ? (double)cl[tri(k, l) * nt + L]
-                   : (double)pair32<COV>(dx[k], dy[k], dx[l], dy[l], c32);
-          r[k] = fma(-ckl, bd[l], r[k]);
-          r[l] = fma(-ckl, bd[k], r[l]);
+                   : (double)pair32<COV>(xk, yk, cl[l * nt + L],
+                                         cl[(M + l) * nt + L], c32);
+          r[k] = fma(-ckl, (double)v[l], r[k]);
+          r[l] = fma(-ckl, (double)v[k], r[l]);
         }
       }
+      float d[M];
 #pragma unroll
-      for (int k = 0; k < M; k++) v[k] = (float)r[k];
-      chol_apply32<M>(a, v);  // v <- d
+      for (int k = 0; k < M; k++) d[k] = (float)r[k];
+      chol_apply32<M>(a, d);  // d <- C^-1 r
+      double bd[M];
       double cb = 0.0;
 #pragma unroll
       for (int k = 0; k < M; k++) {
-        bd[k] += (double)v[k];
-        const float ck =
-            COPY ? cl[(T + k) * nt + L]
-                 : s2_corr32<COV>(fmaf(dx[k], dx[k], dy[k] * dy[k]), c32);
+        bd[k] = (double)v[k] + (double)d[k];
+        float ck;
+        if (COPY) {
+          ck = cl[(T + k) * nt + L];
+        } else {
+          const float xk = cl[k * nt + L], yk = cl[(M + k) * nt + L];
+          ck = s2_corr32<COV>(fmaf(xk, xk, yk * yk), c32);
+        }
         cb = fma((double)ck, bd[k], cb);
       }
       if (A.B) {
@@ -363,8 +406,13 @@ __global__ __launch_bounds__(kLaneMaxThreads, COPY ? 1 : 2) void k_bf_mixed(BfAr
           if (k < cnt) A.B[off + k] = bd[k];
       }
       Fi = c.diag - cb;
+      // the neighbor slots again (cache hits) rather than held in
+      // registers through the factorization; padded coefficients are 0
+#pragma unroll
+      for (int k = 0; k < M; k++) nb[k] = k < cnt ? A.nnIndx[off + k] : 0;
 #pragma unroll
       for (int j = 0; j < kMaxNZ; j++) {
+        e[j] = 0.0;
         if (j < A.nz) {
           double ej = A.Z[(size_t)i * A.zs + j];
 #pragma unroll
@@ -376,6 +424,9 @@ __global__ __launch_bounds__(kLaneMaxThreads, COPY ? 1 : 2) void k_bf_mixed(BfAr
       if (!ok || !(Fi > 0.0)) atomicMin(A.err, i);
     }
     if (A.F) A.F[i] = Fi;
+  } else {
+#pragma unroll
+    for (int j = 0; j < kMaxNZ; j++) e[j] = 0.0;
   }
   if (A.rows) block_site_terms(A.nz, A.rows, A.rows_cap, valid, Fi, e, b);
 }
@@ -782,6 +833,12 @@ static size_t mixed_copy_shm(int m, int threads) {
   return (size_t)threads * (M * (M + 1) / 2 + M) * sizeof(float);
 }
 
+// LDS of the mixed per-lane kernel without the covariance copy: the
+// neighbor offsets.
+static size_t mixed_offsets_shm(int m, int threads) {
+  return (size_t)threads * 2 * lane_M(m) * sizeof(float);
+}
+
 template <int M, bool COPY>
 static void launch_mixed_mc(const BfArgs &a, int variant, const dim3 &grid,
                             const dim3 &blk, size_t shm, hipStream_t st) {
@@ -801,7 +858,8 @@ static void launch_mixed_m(const BfArgs &a, int variant, bool copy,
     launch_mixed_mc<M, true>(a, variant, grid, blk,
                              mixed_copy_shm(a.m, (int)blk.x), st);
   else
-    launch_mixed_mc<M, false>(a, variant, grid, blk, 0, st);
+    launch_mixed_mc<M, false>(a, variant, grid, blk,
+                              mixed_offsets_shm(a.m, (int)blk.x), st);
 }
 
 static void launch_mixed(const BfArgs &a, int variant, bool copy,

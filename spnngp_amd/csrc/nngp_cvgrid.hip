@@ -442,6 +442,8 @@ struct CvGrid {
   int graph_g = -1, graph_variant = -99;
   bool graph_y0 = false;
   BfConfig cfg;
+  BfConfig pinned{0, 0, 0};  // launch shape from nngp_opts.shape (threads 0:
+                             // none; the autotuner chooses)
   BfTuneReport tune;
   bool tuned = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -727,11 +729,16 @@ int CvGrid::run(const double *thetaAlpha, int g, double *scores,
     a.fs_lu = 2LL * nmod;
     a.fs_Z = (long long)nmod * zs;
     a.ps_rows = (long long)rows_cap * nv;
-    long long want = 4000000LL / (nmod > 0 ? nmod : 1);
-    const int ntune = (int)std::max(1LL, std::min<long long>(nprob, want));
     tune = BfTuneReport();
-    const int rc = bf_autotune(a, ntune, st.s, &cfg, &tune);
-    if (rc) return set_err(NNGP_ERR_HIP, "autotune launch failed (%d)", rc);
+    if (pinned.threads > 0 && bf_config_ok(pinned, m, variant) &&
+        bf_rows(a, pinned) <= rows_cap) {
+      cfg = pinned;
+    } else {
+      long long want = 4000000LL / (nmod > 0 ? nmod : 1);
+      const int ntune = (int)std::max(1LL, std::min<long long>(nprob, want));
+      const int rc = bf_autotune(a, ntune, st.s, &cfg, &tune);
+      if (rc) return set_err(NNGP_ERR_HIP, "autotune launch failed (%d)", rc);
+    }
     tuned = true;
     graph_g = -1;
   }
@@ -796,10 +803,12 @@ int CvGrid::run(const double *thetaAlpha, int g, double *scores,
     det->graph_ms = gms;
     det->bf_family = cfg.family;
     det->bf_threads = cfg.threads;
+    det->bf_mode = cfg.mode;
     det->tune_n = tune.n;
     for (int t = 0; t < NNGP_TUNE_MAX; t++) {
       det->tune_family[t] = t < tune.n ? tune.family[t] : 0;
       det->tune_threads[t] = t < tune.n ? tune.threads[t] : 0;
+      det->tune_mode[t] = t < tune.n ? tune.mode[t] : 0;
       det->tune_ms[t] = t < tune.n ? tune.ms[t] : 0.0;
     }
     if (want_sol) {
@@ -861,6 +870,18 @@ int nngp_cvgrid_create(nngp_cvgrid **out, const double *y, const double *X,
   if (st != NNGP_OK) {
     delete c;
     return st;
+  }
+  if (opts && opts->shape[1] > 0) {
+    const BfConfig pin{opts->shape[0], opts->shape[1], opts->shape[2]};
+    // a matern grid takes the per-lane shapes at nu = 0.5, 1.5, 2.5
+    const int variant = covModel == 2 ? kCovMatern15 : covModel;
+    if (!bf_config_ok(pin, m, variant) || (pin.family == 0 && pin.mode)) {
+      delete c;
+      return set_err(NNGP_ERR_ARG,
+                     "launch shape (%d, %d, %d) does not apply to this grid",
+                     pin.family, pin.threads, pin.mode);
+    }
+    c->cv.pinned = pin;
   }
   *out = c;
   return NNGP_OK;

@@ -327,6 +327,8 @@ int Model::create(int n_, int m_, int p_, const double *coords,
       (void)hipGetLastError();  // fall back to a copy per pass
   }
   NNGP_HIP(hipStreamSynchronize(st.s));
+  if (opts && opts->shape[1] > 0)
+    NNGP_TRY(pin_shape(opts->shape[0], opts->shape[1], opts->shape[2]));
   return NNGP_OK;
 }
 
@@ -365,6 +367,21 @@ int Model::bf(const CovArgs &c, int slot, bool storeB) {
     return set_err(NNGP_ERR_UNSUPPORTED, "n.neighbors = %d exceeds 64", m);
   if (rc) return set_err(NNGP_ERR_HIP, "updateBF launch failed: %s",
                          hipGetErrorString((hipError_t)rc));
+  return NNGP_OK;
+}
+
+int Model::pin_shape(int family, int threads, int mode) {
+  const BfConfig cfg{family, threads, mode};
+  // a matern model takes the per-lane shapes at the closed-form nu (0.5,
+  // 1.5, 2.5); passes at any other nu use the wavefront kernel's default
+  const int variant = cov == 2 ? kCovMatern15 : cov;
+  const bool mixed = precision == NNGP_MIXED;
+  if (!bf_config_ok(cfg, m, variant) || (!mixed && family == 0 && mode != 0))
+    return set_err(NNGP_ERR_ARG,
+                   "launch shape (%d, %d, %d) does not apply to this model",
+                   family, threads, mode);
+  tuned_cfg = cfg;
+  have_tuned = true;
   return NNGP_OK;
 }
 

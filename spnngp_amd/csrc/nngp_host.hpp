@@ -199,6 +199,8 @@ struct Model {
 
   // Time the launch shapes of the pass at c and use the fastest from now on.
   int autotune(const CovArgs &c, BfTuneReport *rep);
+  // Use this launch shape from now on (NNGP_ERR_ARG if it does not apply).
+  int pin_shape(int family, int threads, int mode);
 
   int create(int n, int m, int p, const double *coords, const int *nnIndx,
              const int *nnIndxLU, const double *X, const double *y, int cov,

@@ -46,9 +46,10 @@ def _cm(a):
     return _f64(a)
 
 
-def _opts(precision: int = 0, device: int = 0, seed: int = 1):
-    return NngpOpts(precision, device, seed, _lib._INT_FN(), None,
-                    (C.c_int * 8)())
+def _opts(precision: int = 0, device: int = 0, seed: int = 1, shape=None):
+    sh = (C.c_int * 3)(*(shape if shape is not None else (0, 0, 0)))
+    return NngpOpts(precision, device, seed, _lib._INT_FN(), None, sh,
+                    (C.c_int * 5)())
 
 
 def device_count() -> int:
@@ -94,6 +95,15 @@ def nn2(coords, coords0, k: int):
 # ---------------------------------------------------------------------------
 # kernels
 # ---------------------------------------------------------------------------
+def _shape_name(family, mode):
+    """Name of an updateBF launch shape of the autotuner: 'lane' (one site
+    per lane; '+lds-copy' for the mixed form's covariance copy) or 'wave'
+    (one site per wavefront)."""
+    if family == 0:
+        return "lane+lds-copy" if mode else "lane"
+    return "wave"
+
+
 def updateBF(coords, nnIndx, nnIndxLU, m: int, theta, covModel,
              precision: int = 0):
     """updateBF (src/rNNGP.cpp:46-101).  theta = (sigma.sq, tau.sq, phi[, nu]).
@@ -254,14 +264,14 @@ class Model:
     """Coordinates, neighbor index, X and y kept in HBM (nngp_model_*)."""
 
     def __init__(self, coords, nnIndx, nnIndxLU, m, covModel, X=None, y=None,
-                 precision=0):
+                 precision=0, shape=None):
         coords = _cm(coords)
         self.n = coords.size // 2
         self.m = m
         self.cov = cov_index(covModel)
         self.p = 0 if X is None else np.asarray(X).shape[1]
         self._h = C.c_void_p()
-        self._opts = _opts(precision)
+        self._opts = _opts(precision, shape=shape)
         Xc = None if X is None else _cm(X)
         yc = None if y is None else _f64(y)
         self._keep = (coords, Xc, yc)
@@ -325,11 +335,16 @@ class Model:
         th[:len(theta)] = theta
         r = _lib.NngpTuneReport()
         check(load().nngp_model_autotune(self._h, th, C.byref(r)))
-        kind = ("lane", "wave")
-        tag = ("", "+lds-copy")
-        return ([(kind[r.family[t]] + tag[r.mode[t]], r.threads[t], r.ms[t])
+        return ([(_shape_name(r.family[t], r.mode[t]), r.threads[t], r.ms[t])
                  for t in range(r.n)],
-                (kind[r.best_family] + tag[r.best_mode], r.best_threads))
+                (_shape_name(r.best_family, r.best_mode), r.best_threads))
+
+    def set_shape(self, family, threads, mode=0):
+        """Pin the updateBF launch shape: family 0 (one site per lane) or 1
+        (per wavefront), threads per block, and mode as in the autotuner's
+        report (1: the mixed per-lane kernel's covariance copy in LDS)."""
+        check(load().nngp_model_set_shape(self._h, int(family), int(threads),
+                                          int(mode)))
 
     def qresid(self, beta, slot=0):
         q = C.c_double()
@@ -457,7 +472,7 @@ class CvGrid:
     fixed at construction (ho: k x nHo held-out rows, 0-based), every run
     scores all folds x grid points in one graph launch."""
 
-    def __init__(self, y, X, coords, m, ho, sigmaSqIG, covModel):
+    def __init__(self, y, X, coords, m, ho, sigmaSqIG, covModel, shape=None):
         X = np.asarray(X, dtype=np.float64)
         self.n, self.p = X.shape
         ho = np.asarray(ho)
@@ -465,7 +480,7 @@ class CvGrid:
         self.m = m
         self.cov = cov_index(covModel)
         self._h = C.c_void_p()
-        opts = _opts()
+        opts = _opts(shape=shape)
         check(load().nngp_cvgrid_create(
             C.byref(self._h), _f64(y), _cm(X), self.p, self.n, _cm(coords),
             m, self.k, _i32(ho.reshape(-1)), self.nho, _f64(sigmaSqIG),
@@ -516,9 +531,9 @@ class CvGrid:
                                      C.byref(d)))
         out = {"scores": scores.reshape(2, g).T.copy(),
                "graph_ms": d.graph_ms,
-               "bf_config": ("lane" if d.bf_family == 0 else "wave",
+               "bf_config": (_shape_name(d.bf_family, d.bf_mode),
                              d.bf_threads),
-               "tune": [("lane" if d.tune_family[t] == 0 else "wave",
+               "tune": [(_shape_name(d.tune_family[t], d.tune_mode[t]),
                          d.tune_threads[t], d.tune_ms[t])
                         for t in range(d.tune_n)]}
         i = 0

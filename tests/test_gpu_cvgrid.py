@@ -88,6 +88,38 @@ def test_cvgrid_matches_oracle_per_fold(oracle, nngp, cov, nu, m):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("shape", [(0, 64, 0), (0, 256, 0), (1, 128, 0)])
+def test_cvgrid_every_pinned_shape_matches_oracle(oracle, nngp, shape):
+    """A grid pinned to an updateBF launch shape (per lane, per wavefront)
+    gives the oracle's per-fold solution and skips the autotuner."""
+    print(f"cvgrid shape={shape}", flush=True)
+    n, p, k, nho, m = 1500, 2, 2, 300, 10
+    c, X, y = data(n, p, seed=21)
+    ho = folds(n, k, nho, seed=22)
+    ig = [2.0, 1.0]
+    ta = np.array([[ph, al] for ph in (2.0, 9.0) for al in (0.05, 0.5)])
+    g = ta.shape[0]
+    with nngp.CvGrid(y, X, c, m, ho, ig, 0, shape=shape) as cv:
+        r = cv.run(ta, details=True)
+    assert r["tune"] == []
+    assert r["bf_config"][1] == shape[1]
+    for f in range(k):
+        keep = np.ones(n, bool)
+        keep[ho[f]] = False
+        cm, Xm, ym = c[keep], X[keep], y[keep]
+        c0, X0 = c[ho[f]], X[ho[f]]
+        nn = oracle.mkNNIndxCB(cm, m)
+        nn0, _ = oracle.nn2(cm, c0, m)
+        ref = oracle.cNNGP(ym, Xm.T.reshape(-1).copy(), p, len(ym),
+                           cm.T.reshape(-1).copy(), ta.reshape(-1).copy(),
+                           nn["nnIndx"], nn["nnIndxLU"],
+                           X0.T.reshape(-1).copy(), c0.T.reshape(-1).copy(),
+                           nho, nn0.T.reshape(-1).copy(), g, m, ig, 0)
+        assert max_rel(r["beta.hat"][f].T, ref["beta.hat"]) < RTOL
+        assert max_rel(r["ab"][f].T, ref["ab"]) < RTOL
+
+
+@pytest.mark.gpu
 def test_cvgrid_autotuner_reports_candidates(nngp):
     print("cvgrid autotune", flush=True)
     n, p, k, nho, m = 3000, 3, 2, 600, 15

@@ -121,6 +121,55 @@ def test_updateBF_matches_oracle(oracle, nngp, n, m, cov, theta):
     assert abs(ld1 - ld0) / abs(ld0) < RTOL
 
 
+SHAPES = [(0, 64, 0), (0, 128, 0), (0, 256, 0), (1, 64, 0), (1, 128, 0),
+          (1, 256, 0)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cov,m,theta", [(0, 15, [2.0, 0.2, 12.0]),
+                                         (2, 10, [1.0, 0.3, 6.0, 1.5]),
+                                         (1, 6, [1.5, 0.5, 2.0])])
+def test_every_launch_shape_matches_oracle(oracle, nngp, cov, m, theta):
+    """Each updateBF launch shape the autotuner may pick (one site per lane
+    in 64/128/256-thread blocks, one per wavefront in 1/2/4-wave blocks)
+    gives B, F, the log-determinant and Q within the parity bound; n is not
+    a multiple of 64, so the last block is ragged."""
+    n = 5000 + 37
+    print(f"shapes cov={cov} m={m}", flush=True)
+    c = uniform_coords(n, 300 + m)
+    rng = np.random.default_rng(m)
+    X = np.column_stack([np.ones(n), rng.normal(size=n)])
+    y = X @ [1.0, -1.0] + rng.normal(size=n)
+    nn = oracle.mkNNIndx(c, m)
+    B0, F0, ld0 = oracle.updateBF(c.T.reshape(-1).copy(), nn["nnIndx"],
+                                  nn["nnIndxLU"], n, m, theta, cov)
+    beta = np.array([1.0, -1.0])
+    r = X @ beta - y
+    q0 = oracle.Q(B0, F0, r, r, nn["nnIndx"], nn["nnIndxLU"])
+    with nngp.Model(c, nn["nnIndx"], nn["nnIndxLU"], m, cov, X, y) as md:
+        for fam, th, mode in SHAPES:
+            md.set_shape(fam, th, mode)
+            out = md.update(theta, slot=0, beta=beta)
+            B, F = md.get_bf(0)
+            assert per_site_B_err(B, B0, nn["nnIndxLU"], n) < RTOL
+            assert rel_err(F, F0) < RTOL
+            assert abs(out["logDet"] - ld0) < RTOL * abs(ld0)
+            assert abs(out["qResid"] - q0) < RTOL * abs(q0)
+
+
+@pytest.mark.gpu
+def test_set_shape_rejects_shapes_that_do_not_apply(nngp):
+    n, m = 500, 20
+    c = uniform_coords(n, 9)
+    nn = nngp.mkNNIndxCB(c, m)
+    with nngp.Model(c, nn["nnIndx"], nn["nnIndxLU"], m, 0) as md:
+        with pytest.raises(nngp.NngpError):
+            md.set_shape(0, 64, 0)  # m > 16: no per-lane kernel
+        with pytest.raises(nngp.NngpError):
+            md.set_shape(1, 96, 0)
+        md.set_shape(1, 128, 0)
+
+
 @pytest.mark.parametrize("cov,nu", [(0, 0.0), (2, 1.5), (3, 0.0)])
 def test_updateConjBF_matches_oracle(oracle, nngp, cov, nu):
     n, m = 1500, 15
