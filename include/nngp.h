@@ -145,6 +145,25 @@ int nngp_Q(const double *B, const double *F, const double *u,
 int nngp_prior_draw(const double *B, const double *F, const int *nnIndx,
                     const int *nnIndxLU, int n, const double *z, double *w);
 
+/* rNNGPReplicated — replaces src/rNNGPReplicated.cpp:78-230 (.Call at
+ * R/generics.R:147, fitted() of a conjugate fit).  For each sample s: B and
+ * F of theta_s (C = sigma.sq R + tau.sq I), the draw
+ * v_i = B_i . v_N(i) + sqrt(F_i) z_{s,i} over i = 0..n-1, and
+ * yRep[s*n + i] = v_i + sum_j X[i + j*n] beta[s + j*nSamples].
+ * X n x p, coords n x 2 (NNGP order), beta nSamples x p, theta nTheta x
+ * nSamples, all column-major; yRep n x nSamples.  The normals come from
+ * rng in the reference's order (all n of sample s, then sample s + 1);
+ * with rng == NULL they are Philox normals keyed by (opts->seed, site,
+ * sample), as nngp_server_submit draws them.  Runs on the device: the
+ * draw follows the neighbor DAG level by level, one sample per block. */
+int nngp_rNNGPReplicated(const double *X, int p, int n, int m,
+                         const double *coords, int covModel,
+                         const int *nnIndx, const int *nnIndxLU,
+                         const double *beta, const double *theta,
+                         int nSamples, int verbose, int nReport,
+                         const nngp_rng *rng, const nngp_opts *opts,
+                         double *yRep);
+
 /* ------------------------------------------------------------------------
  * .Call drivers (the argument lists follow the SEXP order of
  * src/spNNGP.h; outputs are caller-allocated)

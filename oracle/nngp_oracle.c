@@ -657,6 +657,44 @@ void or_nngp_draw(const double *B, const double *F, const int *nnIndx,
   }
 }
 
+/* src/rNNGPReplicated.cpp:78-230 (the sample loop at :181-199) */
+int or_rNNGPReplicated(const double *X, int p, int n, int m,
+                       const double *coords, int covModel, const int *nnIndx,
+                       const int *nnIndxLU, const double *beta,
+                       const double *theta, int nSamples, const or_rng *rng,
+                       double *yRep) {
+  const int nTheta = covModel == 2 ? 4 : 3;
+  const long long nIndx = (long long)(1 + m) * m / 2 + (long long)(n - m - 1) * m;
+  double *B = (double *)malloc(sizeof(double) * (size_t)(nIndx > 0 ? nIndx : 1));
+  double *F = (double *)malloc(sizeof(double) * (size_t)n);
+  double *z = (double *)malloc(sizeof(double) * (size_t)n);
+  if (!B || !F || !z) {
+    free(B);
+    free(F);
+    free(z);
+    return fail(OR_ERR_ARG, "allocation failed");
+  }
+  int st = OR_OK;
+  for (int s = 0; s < nSamples && st == OR_OK; s++) {
+    double ld;
+    st = or_updateBF(B, F, coords, nnIndx, nnIndxLU, n, m,
+                     theta + (size_t)s * nTheta, covModel, 1, &ld);
+    if (st) break;
+    for (int i = 0; i < n; i++) z[i] = r_rnorm(rng, 0.0, 1.0);
+    double *v = yRep + (size_t)s * n;
+    or_nngp_draw(B, F, nnIndx, nnIndxLU, n, z, v);
+    /* dgemv('N', n, p, 1, X, n, beta + s, nSamples, 1, v, 1) */
+    for (int j = 0; j < p; j++) {
+      const double bj = beta[s + (size_t)j * nSamples];
+      for (int i = 0; i < n; i++) v[i] += X[i + (size_t)j * n] * bj;
+    }
+  }
+  free(B);
+  free(F);
+  free(z);
+  return st;
+}
+
 /* ------------------------------------------------------------------ */
 /* rNNGP: response-model MCMC                                          */
 /* ------------------------------------------------------------------ */

@@ -23,6 +23,8 @@
  *   or_rNNGP          RNG draw order by hand-rolled single steps, posterior
  *                     recovery; dense-GP likelihood at m >= n-1
  *   or_rNNGPPredict   dense GP kriging when m = n (z = 0 isolates the mean)
+ *   or_nngp_draw,     at m >= n-1 the draw is v = W z with W W^T equal to the
+ *   or_rNNGPReplicated  dense covariance s2 R + t2 I (W = (I-B)^-1 F^1/2)
  *   or_cNNGP          dense GLS when m >= n-1
  *   or_philox*        Random123 known-answer vectors; moments and a KS test
  * Parity unpinned (no R in this image): R's RNG stream, the exact bits of
@@ -136,6 +138,16 @@ int or_cNNGP(const double *y, const double *X, int p, int n,
  * (src/rNNGP.cpp:24-42, src/rNNGPReplicated.cpp:183-199). */
 void or_nngp_draw(const double *B, const double *F, const int *nnIndx,
                   const int *nnIndxLU, int n, const double *z, double *w);
+
+/* src/rNNGPReplicated.cpp:78-230: for each sample s, B and F of
+ * theta[s*nTheta..] (or_updateBF, tau.sq on the diagonal), the draw of
+ * or_nngp_draw with n normals from rng (site order), then
+ * yRep[s*n + i] = v_i + sum_j X[i + j*n] beta[s + j*nSamples]. */
+int or_rNNGPReplicated(const double *X, int p, int n, int m,
+                       const double *coords, int covModel, const int *nnIndx,
+                       const int *nnIndxLU, const double *beta,
+                       const double *theta, int nSamples, const or_rng *rng,
+                       double *yRep);
 
 /* Philox4x32-10 (Salmon, Moraes, Dror & Shaw, SC'11) in place on the
  * counter c[4] under key (k0, k1), and the standard normal the kriging

@@ -40,7 +40,7 @@ EXPORTS = (
     "nngp_server_wait", "nngp_server_release", "nngp_server_predict",
     "nngp_server_get_stats",
     "nngp_cvgrid_create", "nngp_cvgrid_run", "nngp_cvgrid_destroy",
-    "nngp_model_autotune", "nngp_model_set_shape",
+    "nngp_model_autotune", "nngp_model_set_shape", "nngp_rNNGPReplicated",
 )
 
 _RNG_FN = C.CFUNCTYPE(C.c_double, C.c_void_p)
@@ -171,6 +171,8 @@ def load():
         + [_i, _i, _i, C.POINTER(NngpRng), C.POINTER(NngpOpts), _dp, _dp,
            C.POINTER(_i)])
     L.nngp_prior_draw.argtypes = [_dp, _dp, _ip, _ip, _i, _dp, _dp]
+    L.nngp_rNNGPReplicated.argtypes = [_vp, _i, _i, _i, _dp, _i, _ip, _ip,
+                                       _vp, _dp, _i, _i, _i, _vp, _vp, _dp]
     L.nngp_model_profile.argtypes = [_vp, _i]
     L.nngp_model_timing.argtypes = [_vp, _dp]
     L.nngp_server_create.argtypes = [C.POINTER(_vp), _dp, _dp, _dp, _i, _i,

@@ -1,0 +1,252 @@
+// nngp_rep.hip — rNNGPReplicated (src/rNNGPReplicated.cpp:78-230): replicate
+// data sets y.rep[, s] = v + X beta_s from the NNGP of posterior sample s,
+// where v_i = B_i . v_N(i) + sqrt(F_i) z_i is drawn site by site in the NNGP
+// order (:183-194) with B, F of theta_s (C = s2 R + t2 I, as in updateBF).
+//
+// Device design.
+//  * B and F of a group of G samples come from G updateBF launches (the
+//    same kernels as the samplers, without the sums) into per-sample
+//    buffers, so nothing crosses PCIe but the normals and the replicates.
+//  * The draw is sequential only along the neighbor DAG: site i waits for
+//    its neighbors, all of which precede it.  Sites are grouped by level
+//    (1 + the largest level among their neighbors, computed on the host
+//    once per neighbor index); one block per sample walks the levels in
+//    order with a barrier between them.  The depth grows like sqrt(n) for
+//    x-ordered sites (about 8.5e3 levels at n = 1e6, m = 15), so one
+//    sample's draw takes milliseconds, and the G samples of a group run on
+//    G CUs at once.
+//  * The normals: with a caller's generator (R's norm_rand through the
+//    shim) they are drawn on the host in the reference's order, all n of
+//    sample s before sample s + 1, so the replicates follow R's stream.
+//    Without one they are counter-based Philox normals keyed by
+//    (seed, site, sample), drawn in the kernel.
+#include <algorithm>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "nngp_host.hpp"
+
+namespace nngp {
+
+constexpr int kRepThreads = 1024;
+
+struct RepArgs {
+  const double *B, *F, *z, *X, *beta;
+  const int *nnIndx, *lu, *order, *loff;
+  double *v, *y;
+  long long bstride;  // doubles between samples' B
+  int n, nlev, p, nS, s0;
+  unsigned long long seed;  // Philox key when z == nullptr
+};
+
+// Block g: sample s0 + g.  Levels in order; within a level every site's
+// neighbors are final, so its threads may run in any order.
+__global__ __launch_bounds__(kRepThreads) void k_rep_draw(RepArgs A) {
+  const int g = (int)blockIdx.x;
+  const int n = A.n;
+  const double *B = A.B + (size_t)g * A.bstride;
+  const double *F = A.F + (size_t)g * n;
+  const double *z = A.z ? A.z + (size_t)g * n : nullptr;
+  double *v = A.v + (size_t)g * n;
+  const int s = A.s0 + g;
+  for (int L = 0; L < A.nlev; L++) {
+    const int a = A.loff[L], b = A.loff[L + 1];
+    for (int k = a + (int)threadIdx.x; k < b; k += (int)blockDim.x) {
+      const int i = A.order[k];
+      const int off = A.lu[i], cnt = A.lu[n + i];
+      double acc = 0.0;
+      for (int j = 0; j < cnt; j++) acc += B[off + j] * v[A.nnIndx[off + j]];
+      const double zi =
+          z ? z[i] : philox_normal(A.seed, (uint32_t)i, (uint32_t)s);
+      v[i] = acc + sqrt(F[i]) * zi;
+    }
+    __syncthreads();
+  }
+  // y.rep = v + X beta_s (dgemv at src/rNNGPReplicated.cpp:196)
+  for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) {
+    double yi = v[i];
+    for (int j = 0; j < A.p; j++)
+      yi += A.X[(size_t)j * n + i] * A.beta[s + (size_t)j * A.nS];
+    A.y[(size_t)g * n + i] = yi;
+  }
+}
+
+// Level schedule of the neighbor DAG: order[] lists the sites level by
+// level (ascending site index within a level), loff[L] .. loff[L+1] is
+// level L.  Fails if a neighbor does not precede its site.
+static int build_levels(int n, const int *nnIndx, const int *lu,
+                        std::vector<int> &order, std::vector<int> &loff) {
+  std::vector<int> lev((size_t)n, 0);
+  int nlev = 0;
+  for (int i = 0; i < n; i++) {
+    int l = 0;
+    const int off = lu[i], cnt = lu[(size_t)n + i];
+    for (int j = 0; j < cnt; j++) {
+      const int id = nnIndx[off + j];
+      if (id < 0 || id >= i)
+        return set_err(NNGP_ERR_ARG,
+                       "site %d has neighbor %d that is not a predecessor", i,
+                       id);
+      l = std::max(l, lev[id] + 1);
+    }
+    lev[i] = l;
+    nlev = std::max(nlev, l + 1);
+  }
+  loff.assign((size_t)nlev + 1, 0);
+  for (int i = 0; i < n; i++) loff[(size_t)lev[i] + 1]++;
+  for (int L = 0; L < nlev; L++) loff[(size_t)L + 1] += loff[L];
+  order.assign((size_t)n, 0);
+  std::vector<int> pos(loff.begin(), loff.end() - 1);
+  for (int i = 0; i < n; i++) order[(size_t)pos[lev[i]]++] = i;
+  return NNGP_OK;
+}
+
+}  // namespace nngp
+
+using namespace nngp;
+
+extern "C" int nngp_rNNGPReplicated(
+    const double *X, int p, int n, int m, const double *coords, int covModel,
+    const int *nnIndx, const int *nnIndxLU, const double *beta,
+    const double *theta, int nSamples, int verbose, int nReport,
+    const nngp_rng *rng, const nngp_opts *opts, double *yRep) {
+  clear_err();
+  if (covModel < 0 || covModel > 3)
+    return set_err(NNGP_ERR_COV_MODEL,
+                   "c++ Rf_error: cov.model is not correctly specified");
+  if (n < 2 || m < 1 || m >= n)
+    return set_err(NNGP_ERR_ARG, "need n >= 2 and 1 <= n.neighbors < n");
+  if (m > 64)
+    return set_err(NNGP_ERR_UNSUPPORTED, "n.neighbors = %d exceeds 64", m);
+  if (p < 0 || nSamples < 0 || !coords || !theta || (p > 0 && (!X || !beta)) ||
+      (nSamples > 0 && !yRep))
+    return set_err(NNGP_ERR_ARG, "NULL buffer or negative size");
+  NNGP_TRY(check_neighbors(n, m, nnIndx, nnIndxLU));
+  if (nSamples == 0) return NNGP_OK;
+  NNGP_TRY(use_device(opts));
+  std::vector<int> order, loff;
+  NNGP_TRY(build_levels(n, nnIndx, nnIndxLU, order, loff));
+  const int nlev = (int)loff.size() - 1;
+  const int nTheta = covModel == 2 ? 4 : 3;
+
+  Model md;  // coordinates and neighbor index resident; no X, y
+  NNGP_TRY(md.create(n, m, 0, coords, nnIndx, nnIndxLU, nullptr, nullptr,
+                     covModel, opts));
+  hipStream_t st = md.st.s;
+  // samples per group: per-sample device buffers within 16 GB
+  const long long bstride = (md.nIndx + 1) & ~1LL;
+  const double per = 8.0 * ((double)bstride + 4.0 * n);
+  int G = (int)std::min<double>(16e9 / per, 256.0);
+  G = std::max(1, std::min(G, nSamples));
+  DevBuf dB, dF, dz, dv, dy, dX, dbeta, dorder, dloff;
+  NNGP_TRY(dB.alloc(sizeof(double) * (size_t)bstride * G));
+  NNGP_TRY(dF.alloc(sizeof(double) * (size_t)n * G));
+  NNGP_TRY(dv.alloc(sizeof(double) * (size_t)n * G));
+  NNGP_TRY(dy.alloc(sizeof(double) * (size_t)n * G));
+  if (rng) NNGP_TRY(dz.alloc(sizeof(double) * (size_t)n * G));
+  if (p > 0) {
+    NNGP_TRY(dX.alloc(sizeof(double) * (size_t)n * p));
+    NNGP_TRY(dbeta.alloc(sizeof(double) * (size_t)nSamples * p));
+    NNGP_HIP(hipMemcpyAsync(dX.p, X, dX.bytes, hipMemcpyHostToDevice, st));
+    NNGP_HIP(hipMemcpyAsync(dbeta.p, beta, dbeta.bytes, hipMemcpyHostToDevice,
+                            st));
+  }
+  NNGP_TRY(dorder.alloc(sizeof(int) * (size_t)n));
+  NNGP_TRY(dloff.alloc(sizeof(int) * loff.size()));
+  NNGP_HIP(hipMemcpyAsync(dorder.p, order.data(), dorder.bytes,
+                          hipMemcpyHostToDevice, st));
+  NNGP_HIP(hipMemcpyAsync(dloff.p, loff.data(), sizeof(int) * loff.size(),
+                          hipMemcpyHostToDevice, st));
+  PinnedBuf hz;
+  if (rng) NNGP_TRY(hz.alloc(sizeof(double) * (size_t)n * G));
+  Rng R(rng, opts ? opts->seed : 1);
+  if (verbose)
+    std::printf("Computing %d replicates of the NNGP response model: %d "
+                "observations, %d neighbors, %d dependency levels.\n",
+                nSamples, n, m, nlev);
+  int status = 0;
+  for (int s0 = 0; s0 < nSamples; s0 += G) {
+    const int g = std::min(G, nSamples - s0);
+    for (int k = 0; k < g; k++) {
+      const double *th = theta + (size_t)(s0 + k) * nTheta;
+      const CovArgs c = make_cov(covModel, th[0], th[1], th[2],
+                                 covModel == 2 ? th[3] : 0.0);
+      BfArgs a;
+      a.xy = md.xy.as<double2>();
+      a.nnIndx = md.nnIndx.as<int>();
+      a.lu = md.lu.as<int>();
+      a.n = n;
+      a.m = m;
+      a.B = dB.as<double>() + (size_t)k * bstride;
+      a.F = dF.as<double>() + (size_t)k * n;
+      a.Z = nullptr;
+      a.nz = 0;
+      a.zs = 0;
+      a.err = md.err.as<int>();
+      a.rows = nullptr;
+      a.rows_cap = 0;
+      a.c = c;
+      a.variant = cov_variant(c.cov, c.nu);
+      BfConfig cfg = bf_default(m, a.variant);
+      if (md.have_tuned && bf_config_ok(md.tuned_cfg, m, a.variant))
+        cfg = md.tuned_cfg;
+      const int rc = bf_launch(a, cfg, 1, st);
+      if (rc)
+        return set_err(NNGP_ERR_HIP, "updateBF launch failed (%d)", rc);
+    }
+    if (rng) {  // n normals per sample, sample after sample (:184)
+      double *z = hz.as<double>();
+      for (int k = 0; k < g; k++)
+        for (int i = 0; i < n; i++) z[(size_t)k * n + i] = R.rnorm(0.0, 1.0);
+      NNGP_HIP(hipMemcpyAsync(dz.p, hz.p, sizeof(double) * (size_t)n * g,
+                              hipMemcpyHostToDevice, st));
+    }
+    RepArgs A;
+    A.B = dB.as<double>();
+    A.F = dF.as<double>();
+    A.z = rng ? dz.as<double>() : nullptr;
+    A.X = dX.as<double>();
+    A.beta = dbeta.as<double>();
+    A.nnIndx = md.nnIndx.as<int>();
+    A.lu = md.lu.as<int>();
+    A.order = dorder.as<int>();
+    A.loff = dloff.as<int>();
+    A.v = dv.as<double>();
+    A.y = dy.as<double>();
+    A.bstride = bstride;
+    A.n = n;
+    A.nlev = nlev;
+    A.p = p;
+    A.nS = nSamples;
+    A.s0 = s0;
+    A.seed = opts ? opts->seed : 1;
+    hipLaunchKernelGGL(k_rep_draw, dim3(g), dim3(kRepThreads), 0, st, A);
+    NNGP_HIP(hipGetLastError());
+    int herr = INT_MAX;
+    NNGP_HIP(hipMemcpyAsync(&herr, md.err.p, sizeof(int),
+                            hipMemcpyDeviceToHost, st));
+    NNGP_HIP(hipMemcpyAsync(yRep + (size_t)s0 * n, dy.p,
+                            sizeof(double) * (size_t)n * g,
+                            hipMemcpyDeviceToHost, st));
+    NNGP_HIP(hipStreamSynchronize(st));
+    if (herr != INT_MAX)
+      return set_err(NNGP_ERR_NOT_PD,
+                     "c++ Rf_error: dpotrf failed\n (site %d of a sample in "
+                     "%d..%d)",
+                     herr, s0, s0 + g - 1);
+    status += g;
+    if (verbose && nReport > 0 && status >= nReport) {
+      std::printf("Sampled: %d of %d, %3.2f%%\n", s0 + g, nSamples,
+                  100.0 * (s0 + g) / nSamples);
+      std::fflush(stdout);
+      status = 0;
+    }
+  }
+  if (verbose) {
+    std::printf("Sampled: %d of %d, %3.2f%%\n", nSamples, nSamples, 100.0);
+    std::fflush(stdout);
+  }
+  return NNGP_OK;
+}

@@ -257,6 +257,27 @@ def rNNGPPredict(X, y, coords, n, p, m, X0, coords0, q, nnIndx0,
     return {"p.y.0": out.reshape(nSamples, q).T.copy()}
 
 
+def rNNGPReplicated(X, p, n, m, coords, covModel, nnIndx, nnIndxLU, beta,
+                    theta, nSamples, nThreads=1, verbose=0, nReport=100,
+                    rng=None, seed=1):
+    """rNNGPReplicated (src/rNNGPReplicated.cpp:78-230), .Call order of
+    R/generics.R:147.  X n x p, coords n x 2 in NNGP order, beta nSamples x
+    p, theta nTheta x nSamples.  rng: as for rNNGP (normals in the
+    reference's order); None: Philox normals keyed by (seed, site,
+    sample).  Returns {'y.rep.samples': n x nSamples}."""
+    del nThreads  # accepted for signature parity; the device runs it
+    out = np.zeros(max(n * nSamples, 1))
+    opts = _opts(0, 0, seed)
+    Xc = _cm(X) if p else None
+    bc = _cm(np.asarray(beta, np.float64).reshape(nSamples, p)) if p else None
+    check(load().nngp_rNNGPReplicated(
+        None if Xc is None else Xc.ctypes.data_as(C.c_void_p), p, n, m,
+        _cm(coords), cov_index(covModel), _i32(nnIndx), _i32(nnIndxLU),
+        None if bc is None else bc.ctypes.data_as(C.c_void_p), _cm(theta),
+        nSamples, verbose, nReport, _rng_struct(rng), C.byref(opts), out))
+    return {"y.rep.samples": out[:n * nSamples].reshape(nSamples, n).T.copy()}
+
+
 # ---------------------------------------------------------------------------
 # device-resident model
 # ---------------------------------------------------------------------------

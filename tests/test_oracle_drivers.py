@@ -223,3 +223,45 @@ def test_cNNGP_is_dense_gls_when_m_is_n_minus_1(oracle):
         var = b / (a - 1) * (u @ np.linalg.solve(G, u) + 1 + alpha - w @ c0v)
         assert out["y.0.hat"][i, 0] == pytest.approx(yh, rel=1e-10)
         assert out["y.0.hat.var"][i, 0] == pytest.approx(var, rel=1e-10)
+
+
+@pytest.mark.parametrize("cov,theta", [(0, [2.0, 0.3, 5.0]),
+                                       (2, [1.0, 0.5, 4.0, 1.3])])
+def test_rNNGPReplicated_is_the_exact_draw_when_m_is_n_minus_1(oracle, cov,
+                                                               theta):
+    """At m = n - 1 the NNGP is the exact GP: the replicate is
+    v = (I - B)^-1 F^1/2 z + X beta, and (I - B)^-1 F (I - B)^-T equals the
+    dense covariance s2 R + t2 I.  A dropped nugget, a wrong sign or a
+    transposed index breaks one of the two."""
+    n, p, S = 30, 2, 3
+    m = n - 1
+    c = uniform_coords(n, 8)
+    rng = np.random.default_rng(8)
+    X = np.column_stack([np.ones(n), rng.normal(size=n)])
+    beta = rng.normal(size=(S, p))
+    th = np.array([theta] * S, float)
+    th[:, 2] *= np.array([1.0, 1.3, 0.8])
+    nn = oracle.mkNNIndx(c, m)
+    z = rng.normal(size=n * S)
+    y = oracle.rNNGPReplicated(X.T.reshape(-1).copy(), p, n, m,
+                               c.T.reshape(-1).copy(), cov, nn["nnIndx"],
+                               nn["nnIndxLU"], beta.T.reshape(-1).copy(),
+                               th.reshape(-1).copy(), S,
+                               oracle.TapeRng(z, []))
+    D = np.sqrt(((c[:, None] - c[None]) ** 2).sum(-1))
+    for s in range(S):
+        B, F, _ = oracle.updateBF(c.T.reshape(-1).copy(), nn["nnIndx"],
+                                  nn["nnIndxLU"], n, m, th[s], cov)
+        IB = np.eye(n)
+        lu = nn["nnIndxLU"]
+        for i in range(n):
+            for j in range(lu[n + i]):
+                IB[i, nn["nnIndx"][lu[i] + j]] -= B[lu[i] + j]
+        W = np.linalg.solve(IB, np.diag(np.sqrt(F)))
+        R = np.vectorize(lambda d: oracle.spCor(d, th[s, 2],
+                                                th[s, 3] if cov == 2 else 0.0,
+                                                cov))(D)
+        Cd = th[s, 0] * R + th[s, 1] * np.eye(n)
+        assert np.allclose(W @ W.T, Cd, rtol=1e-9, atol=1e-11)
+        want = W @ z[s * n:(s + 1) * n] + X @ beta[s]
+        assert np.allclose(y[:, s], want, rtol=1e-10, atol=1e-12)
