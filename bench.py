@@ -158,6 +158,51 @@ def posterior_samples(S, seed):
     return beta, th
 
 
+def replicates_bench(c, X, idx, m, S, seed, cpu_samples):
+    """rNNGPReplicated (fitted() replicates) for S posterior samples of the
+    fitted response model: per sample an updateBF pass and the sequential
+    NNGP draw (level-scheduled on the device), plus X beta.  The normals are
+    the library's counter-based stream.  The CPU leg is the oracle's
+    restatement on `cpu_samples` samples of the same model."""
+    import spnngp_amd as nn
+    n, p = X.shape
+    beta, th = posterior_samples(S, seed)
+    args = (X, p, n, m, c, 0, idx["nnIndx"], idx["nnIndxLU"])
+    nn.rNNGPReplicated(*args, beta.T[:2].copy(), th[:, :2].copy(), 2,
+                       seed=seed)  # warm-up
+    t0 = time.perf_counter()
+    rep = nn.rNNGPReplicated(*args, beta.T.copy(), th, S,
+                             seed=seed)["y.rep.samples"]
+    dt = time.perf_counter() - t0
+    out = {"n": n, "m": m, "samples": S, "seconds": dt,
+           "replicates_per_s": S / dt,
+           "site_samples_per_s": n * S / dt,
+           "finite": bool(np.all(np.isfinite(rep)))}
+    del rep
+    if cpu_samples > 0:
+        from oracle import oracle as O
+        nt = cores()
+        z = np.random.default_rng(seed).normal(size=n * cpu_samples)
+        t0 = time.perf_counter()
+        O.rNNGPReplicated(X.T.reshape(-1).copy(), p, n, m,
+                          c.T.reshape(-1).copy(), 0, idx["nnIndx"],
+                          idx["nnIndxLU"],
+                          # beta is p x S here; the oracle takes S x p
+                          # column-major, i.e. this p x S block row-major
+                          beta[:, :cpu_samples].copy().reshape(-1),
+                          th[:, :cpu_samples].T.reshape(-1).copy(),
+                          cpu_samples, O.TapeRng(z, []), nThreads=nt)
+        cdt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cpu_samples / cdt,
+                               "unit": "replicates/s", "cores": nt,
+                               "kind": "port",
+                               "sample": f"{cpu_samples} replicate(s) of the "
+                                         f"same n={n}, m={m} model (oracle "
+                                         f"rNNGPReplicated, OpenMP updateBF, "
+                                         f"serial draw), {cdt:.1f} s"}
+    return out
+
+
 def kriging_serving(c, X, y, m, seconds, rate, S, seed):
     """Kriging server under load: requests of q ~ floor(exp(U[0, ln 1e5]))
     new sites, Poisson arrivals at `rate` per second for `seconds`, against
@@ -437,6 +482,8 @@ def main():
     ap.add_argument("--grid-cpu-points", type=int, default=4)
     ap.add_argument("--stress-n", type=int, default=4_000_000)
     ap.add_argument("--stress-iters", type=int, default=5)
+    ap.add_argument("--rep-samples", type=int, default=500)
+    ap.add_argument("--rep-cpu-samples", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is "
@@ -487,6 +534,11 @@ def main():
     krig = kriging_serving(c, X, y, m, args.serve_seconds, args.serve_rate,
                            500, 4) if args.serve_seconds > 0 else None
 
+    print("[bench] replicates", file=sys.stderr, flush=True)
+    reps = replicates_bench(c, X, idx, m, args.rep_samples, 6,
+                            args.rep_cpu_samples) \
+        if args.rep_samples > 0 else None
+
     print("[bench] conjugate grid", file=sys.stderr, flush=True)
     grid = conj_grid_bench(args.grid_n, 15, 5, 3, args.grid_cpu_points) \
         if args.grid_n > 0 else None
@@ -533,6 +585,7 @@ def main():
         "kriging": krig,
         "conj_grid": grid,
         "stress": stress,
+        "replicates": reps,
         "mixed": mixed,
         "bf_tune": f64["tune"],
         "knn_s": knn_s,

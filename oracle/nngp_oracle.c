@@ -661,8 +661,8 @@ void or_nngp_draw(const double *B, const double *F, const int *nnIndx,
 int or_rNNGPReplicated(const double *X, int p, int n, int m,
                        const double *coords, int covModel, const int *nnIndx,
                        const int *nnIndxLU, const double *beta,
-                       const double *theta, int nSamples, const or_rng *rng,
-                       double *yRep) {
+                       const double *theta, int nSamples, int nThreads,
+                       const or_rng *rng, double *yRep) {
   const int nTheta = covModel == 2 ? 4 : 3;
   const long long nIndx = (long long)(1 + m) * m / 2 + (long long)(n - m - 1) * m;
   double *B = (double *)malloc(sizeof(double) * (size_t)(nIndx > 0 ? nIndx : 1));
@@ -678,7 +678,7 @@ int or_rNNGPReplicated(const double *X, int p, int n, int m,
   for (int s = 0; s < nSamples && st == OR_OK; s++) {
     double ld;
     st = or_updateBF(B, F, coords, nnIndx, nnIndxLU, n, m,
-                     theta + (size_t)s * nTheta, covModel, 1, &ld);
+                     theta + (size_t)s * nTheta, covModel, nThreads, &ld);
     if (st) break;
     for (int i = 0; i < n; i++) z[i] = r_rnorm(rng, 0.0, 1.0);
     double *v = yRep + (size_t)s * n;

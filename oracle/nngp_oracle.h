@@ -146,8 +146,8 @@ void or_nngp_draw(const double *B, const double *F, const int *nnIndx,
 int or_rNNGPReplicated(const double *X, int p, int n, int m,
                        const double *coords, int covModel, const int *nnIndx,
                        const int *nnIndxLU, const double *beta,
-                       const double *theta, int nSamples, const or_rng *rng,
-                       double *yRep);
+                       const double *theta, int nSamples, int nThreads,
+                       const or_rng *rng, double *yRep);
 
 /* Philox4x32-10 (Salmon, Moraes, Dror & Shaw, SC'11) in place on the
  * counter c[4] under key (k0, k1), and the standard normal the kriging

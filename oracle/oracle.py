@@ -84,7 +84,7 @@ def lib():
         L.or_nngp_draw.argtypes = [_dp, _dp, _ip, _ip, C.c_int, _dp, _dp]
         L.or_rNNGPReplicated.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _dp,
                                          C.c_int, _ip, _ip, _dp, _dp, C.c_int,
-                                         C.POINTER(OrRng), _dp]
+                                         C.c_int, C.POINTER(OrRng), _dp]
         L.or_philox4x32_10.argtypes = [C.POINTER(C.c_uint), C.c_uint,
                                        C.c_uint]
         L.or_philox_normal.restype = C.c_double
@@ -223,7 +223,7 @@ def nngp_draw(B, F, nnIndx, nnIndxLU, z):
 
 
 def rNNGPReplicated(X, p, n, m, coords_flat, covModel, nnIndx, nnIndxLU,
-                    beta, theta, nSamples, tape):
+                    beta, theta, nSamples, tape, nThreads=1):
     """src/rNNGPReplicated.cpp:78-230.  X n x p and beta nSamples x p
     column-major flat, theta nTheta x nSamples flat; normals from the
     TapeRng `tape` in the reference's order.  Returns y.rep, n x nSamples."""
@@ -232,7 +232,7 @@ def rNNGPReplicated(X, p, n, m, coords_flat, covModel, nnIndx, nnIndxLU,
     _check(lib().or_rNNGPReplicated(_f64(X), p, n, m, _f64(coords_flat),
                                     covModel, _i32(nnIndx), _i32(nnIndxLU),
                                     _f64(beta), _f64(theta), nSamples,
-                                    C.byref(r), y))
+                                    nThreads, C.byref(r), y))
     return y.reshape(nSamples, n).T.copy()
 
 

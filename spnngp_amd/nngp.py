@@ -275,7 +275,9 @@ def rNNGPReplicated(X, p, n, m, coords, covModel, nnIndx, nnIndxLU, beta,
         _cm(coords), cov_index(covModel), _i32(nnIndx), _i32(nnIndxLU),
         None if bc is None else bc.ctypes.data_as(C.c_void_p), _cm(theta),
         nSamples, verbose, nReport, _rng_struct(rng), C.byref(opts), out))
-    return {"y.rep.samples": out[:n * nSamples].reshape(nSamples, n).T.copy()}
+    # n x nSamples as a column-major view of the library's output (no copy
+    # of what can be gigabytes)
+    return {"y.rep.samples": out[:n * nSamples].reshape(nSamples, n).T}
 
 
 # ---------------------------------------------------------------------------

@@ -1,6 +1,7 @@
 """R-level front ends of spNNGP's hot path: spNNGP(method="response")
-(R/spNNGP.R), spConjNNGP() (R/spConjNNGP.R), predict() for both
-(R/generics.R:583-675) and the CV scores crps() / rmspe() (R/util.R:247-255).
+(R/spNNGP.R), spConjNNGP() (R/spConjNNGP.R), predict() and fitted() for
+both (R/generics.R:583-675, :1-268) and the CV scores crps() / rmspe()
+(R/util.R:247-255).
 
 These mirror the R functions' argument checks, defaults and host-side
 bookkeeping (x-ordering, OLS starting beta, holdout folds, argmin of the
@@ -381,17 +382,137 @@ def spConjNNGP(y, X, coords, *, theta_alpha, sigma_sq_IG, n_neighbors=15,
             "nn.indx": nn_indx, "nn.indx.lu": nn_indx_lu, "ord": ord,
             "nn.indx.run.time": nn_run_time}
     if n_samples > 0:  # exact posterior samples (R/spConjNNGP.R:423-438)
-        samp = np.zeros((n_samples, p + 2))
-        for i in range(n_samples):
-            s2 = 1.0 / rng.gamma(ab[0], 1.0 / ab[1])  # rigamma(1, a, b)
-            L = np.linalg.cholesky(s2 * bBinv).T      # chol(V), upper
-            beta = rng.normal(size=p) @ L + bBinv @ res["bb"]
-            samp[i] = np.concatenate([beta, [s2, best[1] * s2]])
-        res["p.beta.theta.samples"] = samp
+        res["p.beta.theta.samples"] = _conj_samples(ab, bBinv, res["bb"],
+                                                    best[1], n_samples, rng)
     res["coords"] = coords_o[np.argsort(ord, kind="stable")]
     res["X"] = X_o[np.argsort(ord, kind="stable")]
     res["y"] = y_o[np.argsort(ord, kind="stable")]
+    if fit_rep:  # R/spConjNNGP.R:441-495
+        samp = res["p.beta.theta.samples"]
+        beta = samp[:, :p]
+        res["y.hat.samples"] = res["X"] @ beta.T
+        res["y.hat.quants"] = _quants(res["y.hat.samples"])
+        res["y.rep.samples"] = _replicates(res, beta, _conj_theta(res, samp),
+                                           verbose, 100, seed)
+        res["y.rep.quants"] = _quants(res["y.rep.samples"])
+        res["sub.sample"] = {"start": 1, "end": n_samples, "thin": 1}
+        res["s.indx"] = np.arange(1, n_samples + 1)
     return res
+
+
+def _conj_samples(ab, bBinv, bb, alpha, S, rng):
+    """S exact posterior draws of (beta, sigma.sq, tau.sq) of the conjugate
+    model (R/spConjNNGP.R:423-438, R/generics.R:54-61): sigma.sq ~ IG(a, b),
+    beta ~ N(B^-1 b, sigma.sq B^-1), tau.sq = alpha sigma.sq."""
+    p = len(bb)
+    samp = np.zeros((S, p + 2))
+    for i in range(S):
+        s2 = 1.0 / rng.gamma(ab[0], 1.0 / ab[1])  # rigamma(1, a, b)
+        L = np.linalg.cholesky(s2 * bBinv).T      # chol(V), upper
+        beta = rng.normal(size=p) @ L + bBinv @ bb
+        samp[i] = np.concatenate([beta, [s2, alpha * s2]])
+    return samp
+
+
+def _conj_theta(obj, samp):
+    """theta (nTheta x S) of rNNGPReplicated for conjugate draws: sigma.sq,
+    tau.sq, then phi (and nu) of the chosen grid point repeated
+    (R/spConjNNGP.R:442-447)."""
+    S = samp.shape[0]
+    ta = obj["theta.alpha"]
+    rows = [samp[:, -2], samp[:, -1], np.full(S, ta["phi"])]
+    if obj["cov.model"] == "matern":
+        rows.append(np.full(S, ta["nu"]))
+    return np.vstack(rows)
+
+
+def _quants(a):
+    return np.quantile(a, [0.5, 0.05, 0.975], axis=1).T
+
+
+def _replicates(obj, beta, theta, verbose, n_report, seed):
+    """y.rep of rNNGPReplicated for the object's data (R/generics.R:223-261),
+    in the caller's row order.  beta S x p, theta nTheta x S."""
+    ni = obj.get("neighbor.info")
+    if ni is None:
+        raise ValueError("error: the neighbor.info is not in the model object, "
+                         "call the fitting function again with "
+                         "return_neighbor_info=True.")
+    ord = np.asarray(ni["ord"])
+    X_o = np.asarray(obj["X"], np.float64)[ord]
+    c_o = np.asarray(obj["coords"], np.float64)[ord]
+    n, p = X_o.shape
+    beta = np.asarray(beta, np.float64)
+    out = _nn.rNNGPReplicated(X_o, p, n, obj["n.neighbors"], c_o,
+                              obj["cov.model"], ni["nn.indx"],
+                              ni["nn.indx.lu"], beta, theta, beta.shape[0],
+                              verbose=int(verbose), nReport=n_report,
+                              seed=seed)
+    inv = np.argsort(ord, kind="stable")
+    return np.asarray(out["y.rep.samples"])[inv]
+
+
+# ---------------------------------------------------------------------------
+# fitted() (R/generics.R:1-268)
+# ---------------------------------------------------------------------------
+def fitted(obj, sub_sample=None, *, n_omp_threads=1, verbose=False,
+           n_report=100, seed=1):
+    """fitted.NNGP for the response and conjugate models: fitted values
+    X beta and replicates y.rep (rNNGPReplicated on the device) with their
+    median and 95% quantiles.  Conjugate: sub_sample is the number of exact
+    posterior draws (stored ones are returned when it matches or is None).
+    Response: sub_sample = {'start', 'end', 'thin'} over the stored chain
+    (1-based; default start floor(n.samples / 2)); without it the object's
+    fit.rep results are returned.  Parameter draws use numpy's Generator
+    and the replicate normals the library's counter-based stream, both
+    seeded by `seed` (R's RNG is not available)."""
+    del n_omp_threads
+    keys = ("y.hat.quants", "y.hat.samples", "y.rep.quants", "y.rep.samples")
+    if obj["type"][0] == "conjugate":
+        if sub_sample is not None and not (np.ndim(sub_sample) == 0
+                                           and sub_sample > 0):
+            raise ValueError("error: for a conjugate model, the sub.sample "
+                             "argument should be in integer value indicating "
+                             "the number of fitted and replicated samples to "
+                             "collect.")
+        if "y.rep.samples" in obj and (
+                sub_sample is None
+                or int(sub_sample) == obj["y.rep.samples"].shape[1]):
+            return {k: obj[k] for k in keys}
+        if sub_sample is None:
+            raise ValueError("error: for an conjugate model created with "
+                             "fit.rep=FALSE, the sub.sample argument should be "
+                             "provided to indicate the number of fitted and "
+                             "replicated samples to collect.")
+        S = int(sub_sample)
+        p = len(obj["bb"])
+        samp = _conj_samples(obj["ab"], obj["bB.inv"], obj["bb"],
+                             obj["theta.alpha"]["alpha"], S,
+                             np.random.default_rng(seed))
+        beta = samp[:, :p]
+        out = {"p.beta.theta.samples": samp,
+               "y.hat.samples": np.asarray(obj["X"]) @ beta.T}
+        out["y.hat.quants"] = _quants(out["y.hat.samples"])
+        out["y.rep.samples"] = _replicates(obj, beta, _conj_theta(obj, samp),
+                                           verbose, n_report, seed)
+        out["y.rep.quants"] = _quants(out["y.rep.samples"])
+        out["sub.sample"] = {"start": 1, "end": S, "thin": 1}
+        out["s.indx"] = np.arange(1, S + 1)
+        return out
+    if sub_sample is None and "s.indx" in obj:
+        return {k: obj[k] for k in keys + ("sub.sample", "s.indx")}
+    n_samples = np.asarray(obj["p.beta.samples"]).shape[0]
+    ss, s_indx = _sub_sample(sub_sample, n_samples,
+                             int(np.floor(0.5 * n_samples)))
+    beta = np.asarray(obj["p.beta.samples"])[s_indx - 1]
+    theta = np.asarray(obj["p.theta.samples"])[s_indx - 1].T.copy()
+    out = {"y.hat.samples": np.asarray(obj["X"]) @ beta.T}
+    out["y.hat.quants"] = _quants(out["y.hat.samples"])
+    out["y.rep.samples"] = _replicates(obj, beta, theta, verbose, n_report,
+                                       seed)
+    out["y.rep.quants"] = _quants(out["y.rep.samples"])
+    out["sub.sample"], out["s.indx"] = ss, s_indx
+    return out
 
 
 # ---------------------------------------------------------------------------

@@ -141,3 +141,76 @@ def test_spConjNNGP_kfold_and_final_fit(oracle, nngp):
     a, b = r["ab"][:, 0]
     assert fit["sigma.sq.hat"] == pytest.approx(b / (a - 1), rel=RTOL)
     assert fit["p.beta.theta.samples"].shape == (50, 4)
+
+
+def oracle_replicates(oracle, fit, beta, theta, seed):
+    """The oracle's rNNGPReplicated on the object's x-ordered data with the
+    library's counter-based normals for `seed`, back in the caller's order."""
+    ni = fit["neighbor.info"]
+    o = np.asarray(ni["ord"])
+    Xo, co = np.asarray(fit["X"])[o], np.asarray(fit["coords"])[o]
+    n, p = Xo.shape
+    S = beta.shape[0]
+    z = np.array([oracle.lib().or_philox_normal(seed, i, s)
+                  for s in range(S) for i in range(n)])
+    ref = oracle.rNNGPReplicated(Xo.T.reshape(-1).copy(), p, n,
+                                 fit["n.neighbors"], co.T.reshape(-1).copy(),
+                                 fit["cov.model.indx"], ni["nn.indx"],
+                                 ni["nn.indx.lu"], beta.T.reshape(-1).copy(),
+                                 theta.T.reshape(-1).copy(), S,
+                                 oracle.TapeRng(z, []))
+    return ref[np.argsort(o, kind="stable")]
+
+
+@pytest.mark.gpu
+def test_spConjNNGP_fit_rep_and_fitted(oracle, nngp):
+    """fit.rep of the conjugate model (R/spConjNNGP.R:441-495) and
+    fitted() (R/generics.R:17-150): X beta of the exact draws, and the
+    replicates of rNNGPReplicated on the chosen (phi, alpha)."""
+    print("spConjNNGP fit.rep / fitted", flush=True)
+    n, m, S, seed = 500, 8, 30, 4
+    c, X, y = gp_data(n, 3)
+    fit = nngp.spConjNNGP(y, X, c, theta_alpha={"phi": 6.0, "alpha": 0.2},
+                          sigma_sq_IG=[2, 5], n_neighbors=m, fit_rep=True,
+                          n_samples=S, seed=seed)
+    samp = fit["p.beta.theta.samples"]
+    assert fit["y.rep.samples"].shape == (n, S)
+    assert max_rel(fit["y.hat.samples"], X @ samp[:, :2].T) < RTOL
+    theta = np.vstack([samp[:, 2], samp[:, 3], np.full(S, 6.0)])
+    ref = oracle_replicates(oracle, fit, samp[:, :2], theta, seed)
+    assert max_rel(fit["y.rep.samples"], ref) < RTOL
+    q = fit["y.rep.quants"]
+    assert q.shape == (n, 3) and np.all(q[:, 1] <= q[:, 0])
+    assert np.all(q[:, 0] <= q[:, 2])
+    # fitted(): the stored replicates, or new draws on request
+    same = nngp.fitted(fit)
+    assert same["y.rep.samples"] is fit["y.rep.samples"]
+    new = nngp.fitted(fit, 12, seed=9)
+    assert new["y.rep.samples"].shape == (n, 12)
+    s2 = new["p.beta.theta.samples"][:, 2]
+    th = np.vstack([s2, 0.2 * s2, np.full(12, 6.0)])
+    ref = oracle_replicates(oracle, fit, new["p.beta.theta.samples"][:, :2],
+                            th, 9)
+    assert max_rel(new["y.rep.samples"], ref) < RTOL
+
+
+@pytest.mark.gpu
+def test_fitted_response_model(oracle, nngp):
+    """fitted() of a response fit (R/generics.R:152-268): X beta and the
+    replicates of the chain's samples start..end by thin."""
+    print("fitted response", flush=True)
+    n, m, S, seed = 400, 6, 40, 2
+    c, X, y = gp_data(n, 5)
+    fit = nngp.spNNGP(y, X, c, starting={"sigma.sq": 5, "tau.sq": 1, "phi": 6},
+                      tuning={"sigma.sq": 0.3, "tau.sq": 0.3, "phi": 0.3},
+                      priors={"sigma.sq.IG": [2, 5], "tau.sq.IG": [2, 1],
+                              "phi.Unif": [3, 300]},
+                      n_samples=S, n_neighbors=m, return_neighbor_info=True)
+    out = nngp.fitted(fit, {"start": 10, "end": 40, "thin": 5}, seed=seed)
+    s_indx = np.arange(10, 41, 5)
+    assert np.array_equal(out["s.indx"], s_indx)
+    beta = fit["p.beta.samples"][s_indx - 1]
+    theta = fit["p.theta.samples"][s_indx - 1].T
+    assert max_rel(out["y.hat.samples"], X @ beta.T) < RTOL
+    ref = oracle_replicates(oracle, fit, beta, theta, seed)
+    assert max_rel(out["y.rep.samples"], ref) < RTOL
