@@ -564,3 +564,98 @@ def predict(obj, X_0, coords_0, *, sub_sample=None, n_omp_threads=1,
                                rng=rng, seed=seed)
     return {"p.y.0": out["p.y.0"], "run.time": time.perf_counter() - t0,
             "sub.sample": sub_sample}
+
+
+# ---------------------------------------------------------------------------
+# fitted-model format: save / load a fit, and serve it
+# ---------------------------------------------------------------------------
+# The arrays of a fit object (the R list's elements with R's names:
+# p.beta.samples S x p, p.theta.samples S x nTheta, X, y, coords in the
+# caller's row order, neighbor.info's nn.indx / nn.indx.lu / ord, ...) go
+# into one .npz as plain numeric arrays; every non-array element (names,
+# scalars, nested dicts of scalars) goes into a JSON document stored as a
+# string array.  Loading needs no pickle (numpy.load with
+# allow_pickle=False), so a serving process can read fits it did not make.
+_FIT_FORMAT = "spnngp_amd.fit/1"
+
+
+def _flatten_fit(obj):
+    arrays, meta = {}, {}
+    for k, v in obj.items():
+        if k == "neighbor.info" and isinstance(v, dict):
+            ni = {}
+            for kk, vv in v.items():
+                if kk == "n.indx":
+                    continue  # rebuilt from nn.indx / nn.indx.lu
+                if isinstance(vv, np.ndarray):
+                    arrays["neighbor.info/" + kk] = vv
+                else:
+                    ni[kk] = vv
+            meta["neighbor.info"] = ni
+        elif isinstance(v, np.ndarray):
+            if v.dtype == object:
+                raise ValueError(f"fit element {k!r} is not a numeric array")
+            arrays[k] = v
+        else:
+            meta[k] = v
+    return arrays, meta
+
+
+def _json_default(v):
+    if isinstance(v, np.generic):
+        return v.item()
+    if isinstance(v, np.ndarray):
+        return v.tolist()
+    raise TypeError(f"cannot store {type(v).__name__} in a fit file")
+
+
+def save_fit(obj, path):
+    """Write a spNNGP / spConjNNGP fit (the dict these functions return) to
+    `path` (.npz): numeric arrays as they are, everything else as JSON."""
+    import json
+    arrays, meta = _flatten_fit(obj)
+    doc = json.dumps({"format": _FIT_FORMAT, "meta": meta},
+                     default=_json_default)
+    np.savez(path, __meta__=np.array(doc), **arrays)
+
+
+def load_fit(path):
+    """Read a fit written by save_fit (no pickle is loaded)."""
+    import json
+    with np.load(path, allow_pickle=False) as f:
+        doc = json.loads(str(f["__meta__"]))
+        if doc.get("format") != _FIT_FORMAT:
+            raise ValueError(f"{path}: not a {_FIT_FORMAT} file")
+        obj = dict(doc["meta"])
+        ni = obj.pop("neighbor.info", None)
+        for k in f.files:
+            if k == "__meta__":
+                continue
+            if k.startswith("neighbor.info/"):
+                ni = {} if ni is None else ni
+                ni[k.split("/", 1)[1]] = f[k]
+            else:
+                obj[k] = f[k]
+    if ni is not None:
+        if "nn.indx" in ni and "nn.indx.lu" in ni:
+            n = len(ni["nn.indx.lu"]) // 2
+            ni["n.indx"] = n_indx_list(ni["nn.indx"], ni["nn.indx.lu"], n)
+        obj["neighbor.info"] = ni
+    return obj
+
+
+def serve(obj, sub_sample=None, **server_opts):
+    """A resident kriging server (spnngp_amd.Server) for a response fit:
+    the fitted rows, their neighbor index and the posterior samples
+    sub_sample = {'start', 'end', 'thin'} (1-based, predict()'s default:
+    all samples) stay in HBM.  predict(obj, ..., server=...) then answers
+    from it; server_opts go to Server (device_bytes, host_bytes,
+    batch_sites, depth, device)."""
+    if obj["type"][0] != "response":
+        raise ValueError("serve() takes a response fit (spNNGP)")
+    S_all = np.asarray(obj["p.beta.samples"]).shape[0]
+    _, s_indx = _sub_sample(sub_sample, S_all, 1)
+    beta = np.asarray(obj["p.beta.samples"])[s_indx - 1].T
+    theta = np.asarray(obj["p.theta.samples"])[s_indx - 1].T
+    return _nn.Server(obj["X"], obj["y"], obj["coords"], obj["n.neighbors"],
+                      beta, theta, obj["cov.model"], **server_opts)

@@ -214,3 +214,30 @@ def test_fitted_response_model(oracle, nngp):
     assert max_rel(out["y.hat.samples"], X @ beta.T) < RTOL
     ref = oracle_replicates(oracle, fit, beta, theta, seed)
     assert max_rel(out["y.rep.samples"], ref) < RTOL
+
+
+@pytest.mark.gpu
+def test_saved_fit_serves_the_same_predictions(nngp, tmp_path):
+    """A fit written by save_fit and read back by another reader serves
+    bit-identical predictive draws (same request seed) to the original."""
+    print("fit file -> server", flush=True)
+    n, m, S = 500, 8, 60
+    c, X, y = gp_data(n, 7)
+    fit = nngp.spNNGP(y, X, c, starting={"sigma.sq": 5, "tau.sq": 1, "phi": 6},
+                      tuning={"sigma.sq": 0.3, "tau.sq": 0.3, "phi": 0.3},
+                      priors={"sigma.sq.IG": [2, 5], "tau.sq.IG": [2, 1],
+                              "phi.Unif": [3, 300]},
+                      n_samples=S, n_neighbors=m, return_neighbor_info=True)
+    nngp.save_fit(fit, tmp_path / "fit.npz")
+    back = nngp.load_fit(tmp_path / "fit.npz")
+    rng = np.random.default_rng(8)
+    c0 = rng.uniform(size=(150, 2))
+    X0 = np.column_stack([np.ones(150), rng.normal(size=150)])
+    ss = {"start": 21, "end": 60, "thin": 2}
+    with nngp.serve(fit, ss) as s1, nngp.serve(back, ss) as s2:
+        a = nngp.predict(fit, X0, c0, sub_sample=ss, server=s1,
+                         request_seed=5)["p.y.0"]
+        b = nngp.predict(back, X0, c0, sub_sample=ss, server=s2,
+                         request_seed=5)["p.y.0"]
+    assert a.shape == (150, 20)
+    assert np.array_equal(a, b)

@@ -47,3 +47,60 @@ def test_n_indx_list_matches_layout(oracle):
     for i in range(1, n):
         off, cnt = oracle.getNNIndx(i, m)
         assert list(lst[i]) == list(nn["nnIndx"][off:off + cnt] + 1)
+
+
+def _fake_fit(n=50, S=20, m=5, seed=0):
+    rng = np.random.default_rng(seed)
+    lu = np.zeros(2 * n, np.int32)
+    idx = []
+    for i in range(n):
+        k = min(i, m)
+        lu[i], lu[n + i] = len(idx), k
+        idx += list(range(i - k, i))
+    return {
+        "p.beta.samples": rng.normal(size=(S, 2)),
+        "p.theta.samples": rng.uniform(0.5, 2, size=(S, 3)),
+        "X": rng.normal(size=(n, 2)), "y": rng.normal(size=n),
+        "coords": rng.uniform(size=(n, 2)),
+        "theta.names": ["sigma.sq", "tau.sq", "phi"],
+        "x.names": ["x1", "x2"], "run.time": 1.5, "accept": 7,
+        "n.neighbors": m, "cov.model": "exponential", "cov.model.indx": 0,
+        "starting": {"sigma.sq": 1.0, "tau.sq": 0.5, "phi": 6.0},
+        "type": ["response", "gaussian"],
+        "neighbor.info": {"n.neighbors": m,
+                          "nn.indx": np.array(idx, np.int32),
+                          "nn.indx.lu": lu, "ord": np.arange(n),
+                          "nn.indx.run.time": 0.01},
+    }
+
+
+def test_fit_file_round_trip_without_pickle(tmp_path):
+    fit = _fake_fit()
+    path = tmp_path / "fit.npz"
+    R.save_fit(fit, path)
+    # readable with pickle disabled: only numeric arrays and one string
+    with np.load(path, allow_pickle=False) as f:
+        assert all(f[k].dtype != object for k in f.files)
+    back = R.load_fit(path)
+    for k, v in fit.items():
+        if k == "neighbor.info":
+            for kk, vv in v.items():
+                got = back[k][kk]
+                assert np.array_equal(got, vv) if isinstance(vv, np.ndarray) \
+                    else got == vv
+        elif isinstance(v, np.ndarray):
+            assert np.array_equal(back[k], v) and back[k].dtype == v.dtype
+        else:
+            assert back[k] == v
+    # the 1-based neighbor lists of R/util.R:272-279 are rebuilt on load
+    ni = back["neighbor.info"]
+    want = R.n_indx_list(ni["nn.indx"], ni["nn.indx.lu"], 50)
+    assert all((a is None and b is None) or np.array_equal(a, b)
+               for a, b in zip(ni["n.indx"], want))
+
+
+def test_fit_file_rejects_object_arrays(tmp_path):
+    fit = _fake_fit()
+    fit["bad"] = np.array([{"a": 1}], dtype=object)
+    with pytest.raises(ValueError):
+        R.save_fit(fit, tmp_path / "bad.npz")
