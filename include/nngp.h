@@ -62,6 +62,13 @@ typedef struct nngp_rng {
   double (*unif_rand)(void *state);
 } nngp_rng;
 
+/* R's default generator (RNGkind "Mersenne-Twister", normal.kind
+ * "Inversion") as an nngp_rng, seeded the way set.seed(seed) seeds it, for
+ * callers without R that want R's stream (no counterpart in the reference,
+ * which calls R's own generator).  nngp_rng_r_destroy frees the state. */
+int nngp_rng_r_create(unsigned int seed, nngp_rng *out);
+void nngp_rng_r_destroy(nngp_rng *rng);
+
 /* Per-call options.  NULL means all defaults (FP64, device 0, seed 1). */
 typedef struct nngp_opts {
   int precision;            /* enum nngp_precision                          */

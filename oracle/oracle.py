@@ -228,7 +228,7 @@ def rNNGPReplicated(X, p, n, m, coords_flat, covModel, nnIndx, nnIndxLU,
     column-major flat, theta nTheta x nSamples flat; normals from the
     TapeRng `tape` in the reference's order.  Returns y.rep, n x nSamples."""
     y = np.zeros(n * nSamples)
-    r = tape.struct()
+    r = tape.struct(OrRng)
     _check(lib().or_rNNGPReplicated(_f64(X), p, n, m, _f64(coords_flat),
                                     covModel, _i32(nnIndx), _i32(nnIndxLU),
                                     _f64(beta), _f64(theta), nSamples,
@@ -282,7 +282,7 @@ def rNNGP(y, X, p, n, m, coords_flat, covModel, nnIndx, nnIndxLU, sigmaSqIG,
     beta = np.zeros(p * nSamples)
     theta = np.zeros(nTheta * nSamples)
     acc = C.c_int()
-    rs = rng.struct()
+    rs = rng.struct(OrRng)
     rep = np.zeros(max(n * nRep, 1))
     ri = _i32(repIndx if repIndx is not None else np.zeros(nSamples))
     st = lib().or_rNNGP(

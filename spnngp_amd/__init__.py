@@ -19,14 +19,14 @@ from ._lib import (NNGP_ERR_ARG, NNGP_ERR_COV_MODEL, NNGP_ERR_HIP,
                    NNGP_ERR_INTERRUPT, NNGP_ERR_NOMEM, NNGP_ERR_NOT_PD,
                    NNGP_ERR_UNSUPPORTED, NNGP_FP64, NNGP_MIXED, NNGP_OK,
                    NngpError, exported_symbols, load)
-from .nngp import (COV_MODELS, CvGrid, Model, Q, Server, cNNGP, cov_index, device_count,
+from .nngp import (COV_MODELS, CvGrid, Model, Q, RRng, Server, cNNGP, cov_index, device_count,
                    mkNNIndx, mkNNIndxCB, n_indx, nn2, prior_draw, rNNGP,
                    rNNGPPredict, rNNGPReplicated, updateBF, updateConjBF)
 from .spnngp import (crps, fitted, load_fit, predict, rmspe, save_fit, serve,
                      spConjNNGP, spNNGP)
 
 __all__ = [
-    "COV_MODELS", "CvGrid", "Model", "NngpError", "Q", "Server", "cNNGP", "cov_index",
+    "COV_MODELS", "CvGrid", "Model", "NngpError", "Q", "RRng", "Server", "cNNGP", "cov_index",
     "device_count", "exported_symbols", "load", "mkNNIndx", "mkNNIndxCB",
     "n_indx", "nn2", "prior_draw", "rNNGP", "rNNGPPredict", "rNNGPReplicated",
     "updateBF",

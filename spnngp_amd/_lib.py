@@ -41,6 +41,7 @@ EXPORTS = (
     "nngp_server_get_stats",
     "nngp_cvgrid_create", "nngp_cvgrid_run", "nngp_cvgrid_destroy",
     "nngp_model_autotune", "nngp_model_set_shape", "nngp_rNNGPReplicated",
+    "nngp_rng_r_create", "nngp_rng_r_destroy",
 )
 
 _RNG_FN = C.CFUNCTYPE(C.c_double, C.c_void_p)
@@ -173,6 +174,9 @@ def load():
     L.nngp_prior_draw.argtypes = [_dp, _dp, _ip, _ip, _i, _dp, _dp]
     L.nngp_rNNGPReplicated.argtypes = [_vp, _i, _i, _i, _dp, _i, _ip, _ip,
                                        _vp, _dp, _i, _i, _i, _vp, _vp, _dp]
+    L.nngp_rng_r_create.argtypes = [C.c_uint, C.POINTER(NngpRng)]
+    L.nngp_rng_r_destroy.argtypes = [C.POINTER(NngpRng)]
+    L.nngp_rng_r_destroy.restype = None
     L.nngp_model_profile.argtypes = [_vp, _i]
     L.nngp_model_timing.argtypes = [_vp, _dp]
     L.nngp_server_create.argtypes = [C.POINTER(_vp), _dp, _dp, _dp, _i, _i,

@@ -157,6 +157,41 @@ def prior_draw(B, F, nnIndx, nnIndxLU, z):
 # ---------------------------------------------------------------------------
 # drivers
 # ---------------------------------------------------------------------------
+class RRng:
+    """R's default random number generator (Mersenne-Twister with
+    inversion normals) seeded as set.seed(seed) seeds it, running in
+    libnngp_hip.so.  Pass it as rng= to the samplers to follow R's stream;
+    unif(k) / norm(k) draw like R's runif(k) / rnorm(k)."""
+
+    def __init__(self, seed):
+        self._s = _lib.NngpRng()
+        check(load().nngp_rng_r_create(int(seed) & 0xFFFFFFFF,
+                                       C.byref(self._s)))
+
+    def struct(self, cls=None):
+        """The {state, norm_rand, unif_rand} record, as the library's
+        nngp_rng or as any structure `cls` with that layout."""
+        if cls is None or cls is _lib.NngpRng:
+            return self._s
+        return cls(self._s.state, self._s.norm_rand, self._s.unif_rand)
+
+    def unif(self, k=1):
+        return np.array([self._s.unif_rand(self._s.state) for _ in range(k)])
+
+    def norm(self, k=1):
+        return np.array([self._s.norm_rand(self._s.state) for _ in range(k)])
+
+    def close(self):
+        if self._s.state:
+            load().nngp_rng_r_destroy(C.byref(self._s))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _rng_struct(rng):
     if rng is None:
         return None

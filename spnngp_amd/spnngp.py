@@ -10,11 +10,13 @@ every O(n) step runs in libnngp_hip.so.  Argument names follow R with dots
 turned into underscores; lists become dicts with R's names.
 
 Differences from R that a caller sees:
-  * R's RNG is not available: holdout folds, fit.rep draws and the exact
-    conjugate samples use numpy's Generator (seeded by `seed`), and the
-    sampler uses the library's built-in generator unless `rng` passes a
-    host random source (a nngp_rng-compatible object, e.g. the oracle's
-    tape in tests).  Given the same draws the results are the reference's.
+  * R itself is not called: holdout folds, the exact conjugate samples and
+    fitted()'s parameter draws use numpy's Generator (seeded by `seed`),
+    replicate normals the library's counter-based stream, and the sampler
+    the library's built-in generator unless `rng` passes a host random
+    source (a nngp_rng-compatible object).  rng=RRng(s), the library's
+    copy of R's default generator, gives the stream of R's set.seed(s).
+    Given the same draws the results are the reference's.
   * Only method = "response" with family = "gaussian" is on this path
     (the latent, binomial and SLGP models are not).
 """

@@ -241,3 +241,31 @@ def test_saved_fit_serves_the_same_predictions(nngp, tmp_path):
                          request_seed=5)["p.y.0"]
     assert a.shape == (150, 20)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+def test_spNNGP_chain_from_r_set_seed_stream(oracle, nngp):
+    """With the library's R generator (set.seed(2024) stream) the chain is
+    the oracle's chain driven by the same stream."""
+    print("spNNGP with R's stream", flush=True)
+    n, m, S = 500, 8, 120
+    c, X, y = gp_data(n, 11)
+    kw = dict(starting={"sigma.sq": 5, "tau.sq": 1, "phi": 6},
+              tuning={"sigma.sq": 0.4, "tau.sq": 0.4, "phi": 0.4},
+              priors={"sigma.sq.IG": [2, 5], "tau.sq.IG": [2, 1],
+                      "phi.Unif": [3, 300]})
+    fit = nngp.spNNGP(y, X, c, n_samples=S, n_neighbors=m,
+                      rng=nngp.RRng(2024), **kw)
+    o = np.argsort(c[:, 0], kind="stable")
+    co, Xo, yo = c[o], X[o], y[o]
+    nn_ = oracle.mkNNIndxCB(co, m)
+    b0 = np.linalg.lstsq(Xo, yo, rcond=None)[0]
+    ref = oracle.rNNGP(yo, Xo, 2, n, m, co.T.reshape(-1).copy(), 0,
+                       nn_["nnIndx"], nn_["nnIndxLU"], sigmaSqIG=[2, 5],
+                       tauSqIG=[2, 1], phiUnif=[3, 300], nuUnif=None,
+                       betaStarting=b0, sigmaSqStarting=5.0,
+                       tauSqStarting=1.0, phiStarting=6.0, nuStarting=0.0,
+                       sigmaSqTuning=0.4, tauSqTuning=0.4, phiTuning=0.4,
+                       nuTuning=0.0, nSamples=S, rng=nngp.RRng(2024))
+    assert max_rel(fit["p.beta.samples"], ref["p.beta.samples"].T) < RTOL
+    assert max_rel(fit["p.theta.samples"], ref["p.theta.samples"].T) < RTOL
