@@ -234,8 +234,13 @@ __global__ __launch_bounds__(kLaneMaxThreads) void k_bf_lane(BfArgs A) {
   for (int j = 0; j < kMaxNZ; j++) e[j] = 0.0;
 
   if (valid) {
-    const int cnt = P.lu[A.n + i];
-    const int off = P.lu[i];
+    int cnt, off;
+    if (A.lu_closed) {
+      nn_closed(i, A.m, off, cnt);
+    } else {
+      cnt = P.lu[A.n + i];
+      off = P.lu[i];
+    }
     // The index array carries 64 padding entries, so off + k is in bounds.
     int nb[M];
 #pragma unroll
@@ -283,8 +288,13 @@ __global__ __launch_bounds__(kLaneMaxThreads, 2) void k_bf_mixed(BfArgs A) {
                      // through the factorization)
 
   if (valid) {
-    const int cnt = A.lu[A.n + i];
-    const int off = A.lu[i];
+    int cnt, off;
+    if (A.lu_closed) {
+      nn_closed(i, A.m, off, cnt);
+    } else {
+      cnt = A.lu[A.n + i];
+      off = A.lu[i];
+    }
     if (cnt == 0) {
       Fi = c.diag;
 #pragma unroll

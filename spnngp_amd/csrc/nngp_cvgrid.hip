@@ -729,6 +729,8 @@ int CvGrid::run(const double *thetaAlpha, int g, double *scores,
     a.fs_lu = 2LL * nmod;
     a.fs_Z = (long long)nmod * zs;
     a.ps_rows = (long long)rows_cap * nv;
+    a.lu_closed = 1;  // every fold's index comes from the library's kNN,
+                      // which lays it out in the closed form
     tune = BfTuneReport();
     if (pinned.threads > 0 && bf_config_ok(pinned, m, variant) &&
         bf_rows(a, pinned) <= rows_cap) {

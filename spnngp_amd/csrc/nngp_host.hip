@@ -298,6 +298,12 @@ int Model::create(int n_, int m_, int p_, const double *coords,
   NNGP_HIP(hipMemsetAsync(nnIndx.p, 0, nnIndx.bytes, st.s));
   NNGP_HIP(hipMemcpyAsync(nnIndx.p, nnI, sizeof(int) * (size_t)nIndx,
                           hipMemcpyHostToDevice, st.s));
+  lu_closed = true;
+  for (int i = 0; i < n && lu_closed; i++) {
+    int off, cnt;
+    nn_closed(i, m, off, cnt);
+    lu_closed = LU[i] == off && LU[(size_t)n + i] == cnt;
+  }
   NNGP_TRY(lu.alloc(sizeof(int) * 2 * (size_t)n));
   NNGP_HIP(hipMemcpyAsync(lu.p, LU, sizeof(int) * 2 * (size_t)n,
                           hipMemcpyHostToDevice, st.s));
@@ -352,6 +358,7 @@ int Model::bf(const CovArgs &c, int slot, bool storeB) {
   a.c = c;
   a.variant = cov_variant(c.cov, c.nu);
   a.mixed = precision == NNGP_MIXED ? 1 : 0;
+  a.lu_closed = lu_closed ? 1 : 0;
   BfConfig cfg = bf_default(m, a.variant);
   if (a.mixed) {  // untuned mixed passes keep the covariance copy in LDS
     const BfConfig copy{0, 64, 1};
@@ -403,6 +410,7 @@ int Model::autotune(const CovArgs &c, BfTuneReport *rep) {
   a.c = c;
   a.variant = cov_variant(c.cov, c.nu);
   a.mixed = precision == NNGP_MIXED ? 1 : 0;
+  a.lu_closed = lu_closed ? 1 : 0;
   BfConfig best;
   const int rc = bf_autotune(a, 1, st.s, &best, rep);
   if (rc) return set_err(NNGP_ERR_HIP, "autotune launch failed (%d)", rc);

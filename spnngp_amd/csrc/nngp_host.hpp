@@ -195,6 +195,7 @@ struct Model {
   BfConfig last_cfg;   // and its launch shape
   BfConfig tuned_cfg;  // launch shape chosen by autotune(), when have_tuned
   bool have_tuned = false;
+  bool lu_closed = false;  // nnIndxLU follows the closed form (nn_closed)
   PassOut slot_out[2]; // sums of the last completed pass into each slot
 
   // Time the launch shapes of the pass at c and use the fastest from now on.

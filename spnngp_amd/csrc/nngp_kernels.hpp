@@ -42,7 +42,23 @@ struct BfArgs {
   const CovArgs *covs = nullptr;
   long long fs_xy = 0, fs_nn = 0, fs_lu = 0, fs_Z = 0, ps_rows = 0;
   int coop_blocks = 0;   // set by bf_launch
+  // 1: lu follows the closed form of src/util.cpp:55-70 (offset
+  // i(i-1)/2 for i < m, else m(m-1)/2 + (i-m)m; count min(i, m)), checked
+  // on the host, so the per-lane kernel computes it instead of loading it
+  int lu_closed = 0;
 };
+
+// Offset and count of site i in the flat neighbor arrays when they follow
+// the closed form (src/util.cpp:55-70).
+__host__ __device__ inline void nn_closed(int i, int m, int &off, int &cnt) {
+  if (i < m) {
+    cnt = i;
+    off = i * (i - 1) / 2;
+  } else {
+    cnt = m;
+    off = m * (m - 1) / 2 + (i - m) * m;
+  }
+}
 
 // Launch shape of a pass: family 0 = one site per lane (threads 64, 128 or
 // 256 per block), family 1 = one site per wavefront (threads = 64 x waves,

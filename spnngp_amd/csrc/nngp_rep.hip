@@ -189,6 +189,7 @@ extern "C" int nngp_rNNGPReplicated(
       a.rows_cap = 0;
       a.c = c;
       a.variant = cov_variant(c.cov, c.nu);
+      a.lu_closed = md.lu_closed ? 1 : 0;
       BfConfig cfg = bf_default(m, a.variant);
       if (md.have_tuned && bf_config_ok(md.tuned_cfg, m, a.variant))
         cfg = md.tuned_cfg;

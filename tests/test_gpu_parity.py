@@ -157,6 +157,29 @@ def test_every_launch_shape_matches_oracle(oracle, nngp, cov, m, theta):
             assert abs(out["qResid"] - q0) < RTOL * abs(q0)
 
 
+def test_updateBF_with_a_non_canonical_layout(oracle, nngp):
+    """nnIndxLU need not follow the closed form of src/util.cpp:55-70: with
+    the sites' neighbor blocks stored in reverse order the kernels read the
+    offsets and counts instead of computing them, with the same result."""
+    n, m, theta = 3000, 12, [2.0, 0.3, 9.0]
+    c = uniform_coords(n, 41)
+    nn = oracle.mkNNIndx(c, m)
+    idx, lu = nn["nnIndx"], nn["nnIndxLU"]
+    tot = len(idx)
+    idx2 = np.zeros_like(idx)
+    lu2 = lu.copy()
+    for i in range(n):
+        o, k = lu[i], lu[n + i]
+        lu2[i] = tot - (o + k)
+        idx2[lu2[i]:lu2[i] + k] = idx[o:o + k]
+    B0, F0, ld0 = oracle.updateBF(c.T.reshape(-1).copy(), idx2, lu2, n, m,
+                                  theta, 0)
+    B1, F1, ld1 = nngp.updateBF(c, idx2, lu2, m, theta, 0)
+    assert per_site_B_err(B1, B0, lu2, n) < RTOL
+    assert rel_err(F1, F0) < RTOL
+    assert abs(ld1 - ld0) < RTOL * abs(ld0)
+
+
 @pytest.mark.gpu
 def test_set_shape_rejects_shapes_that_do_not_apply(nngp):
     n, m = 500, 20
