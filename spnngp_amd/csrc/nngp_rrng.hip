@@ -133,16 +133,15 @@ extern "C" int nngp_rng_r_create(unsigned int seed, nngp_rng *out) {
   if (!s) return NNGP_ERR_NOMEM;
   uint32_t x = seed;
   for (int j = 0; j < 50; j++) x = 69069u * x + 1u;  // initial scrambling
-  uint32_t first = 0;
-  for (int j = 0; j < kN + 1; j++) {
+  // 625 more values: the first lands in the position slot, which is then
+  // set to 624 (so the first draw regenerates the state); the rest are
+  // the 624 state words
+  x = 69069u * x + 1u;
+  for (int j = 0; j < kN; j++) {
     x = 69069u * x + 1u;
-    if (j == 0)
-      first = x;  // the position slot, replaced below
-    else
-      s->mt[j - 1] = x;
+    s->mt[j] = x;
   }
-  (void)first;
-  s->mti = kN;  // the first draw regenerates the state
+  s->mti = kN;
   out->state = s;
   out->norm_rand = r_norm;
   out->unif_rand = r_unif;
