@@ -203,6 +203,66 @@ def replicates_bench(c, X, idx, m, S, seed, cpu_samples):
     return out
 
 
+def latent_bench(c, X, y, idx, m, iters, site_iters, cpu_iters, seed):
+    """Latent model (sNNGP) on the same data: the neighbors-of-neighbors
+    index (mkUIndx, search type 2) and MCMC iterations/s with the coloured
+    sweep (colour classes in parallel) and with the site-order sweep (the
+    reference's scan, level by level).  The sweep's normals are the
+    device's counter-based stream.  The CPU leg is the oracle's sNNGP
+    (OpenMP updateBF, serial sweep) on `cpu_iters` iterations, fed by the
+    library's R generator through C callbacks."""
+    import spnngp_amd as nn
+    n, p = X.shape
+    t0 = time.perf_counter()
+    U = nn.mkUIndx(n, m, idx["nnIndx"], idx["nnIndxLU"], 2)
+    u_s = time.perf_counter() - t0
+    beta0 = np.linalg.lstsq(X, y, rcond=None)[0]
+    args = (y, X, p, n, m, c, 0, idx["nnIndx"], idx["nnIndxLU"], U["u.indx"],
+            U["u.indx.lu"], U["ui.indx"], [2.0, 2.0], [2.0, 0.2],
+            [3.0, 300.0], None, beta0, 2.0, 0.2, 12.0, 0.0, 0.0, 0.0, 0.02,
+            0.0)
+    out = {"n": n, "m": m, "mkUIndx_s": u_s}
+    for sweep, k in (("coloured", iters), ("site", site_iters)):
+        if k <= 0:
+            continue
+        nn.sNNGP(*args, 1, seed=seed, sweep=sweep)  # schedule + first pass
+        t0 = time.perf_counter()
+        nn.sNNGP(*args, 1, seed=seed, sweep=sweep)
+        t1 = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        r = nn.sNNGP(*args, k + 1, seed=seed, sweep=sweep)
+        tk = time.perf_counter() - t0
+        per = (tk - t1) / k  # set-up (schedule, uploads, first pass) removed
+        out[sweep] = {"iters_s": 1.0 / per, "ms_per_iter": per * 1e3,
+                      "setup_s": t1 - per, "iters": k,
+                      "acceptance": r["accept"] / (k + 1),
+                      "finite": bool(np.all(np.isfinite(r["p.w.samples"])))}
+        del r
+    if cpu_iters > 0:
+        from oracle import oracle as O
+        nt = cores()
+        t0 = time.perf_counter()
+        Uo = O.mkUIndx(n, m, idx["nnIndx"], idx["nnIndxLU"], 2)
+        cu = time.perf_counter() - t0
+        assert all(np.array_equal(Uo[k], U[k]) for k in U)
+        rr = nn.RRng(seed)
+        t0 = time.perf_counter()
+        O.sNNGP(y, X, p, n, m, c.T.reshape(-1).copy(), 0, idx["nnIndx"],
+                idx["nnIndxLU"], U["u.indx"], U["u.indx.lu"], U["ui.indx"],
+                [2.0, 2.0], [2.0, 0.2], [3.0, 300.0], None, beta0, 2.0, 0.2,
+                12.0, 0.0, 0.02, 0.0, cpu_iters, rr, nThreads=nt)
+        cdt = time.perf_counter() - t0
+        rr.close()
+        out["cpu_baseline"] = {
+            "value": cpu_iters / cdt, "unit": "iter/s", "cores": nt,
+            "kind": "port", "mkUIndx_s": cu,
+            "sample": f"{cpu_iters} iteration(s) of the same n={n}, m={m} "
+                      f"latent chain (oracle sNNGP: OpenMP updateBF, serial "
+                      f"sweep), {cdt:.1f} s; mkUIndx search type 2 on one "
+                      f"core, {cu:.2f} s"}
+    return out
+
+
 def kriging_serving(c, X, y, m, seconds, rate, S, seed):
     """Kriging server under load: requests of q ~ floor(exp(U[0, ln 1e5]))
     new sites, Poisson arrivals at `rate` per second for `seconds`, against
@@ -484,6 +544,9 @@ def main():
     ap.add_argument("--stress-iters", type=int, default=5)
     ap.add_argument("--rep-samples", type=int, default=500)
     ap.add_argument("--rep-cpu-samples", type=int, default=1)
+    ap.add_argument("--latent-iters", type=int, default=100)
+    ap.add_argument("--latent-site-iters", type=int, default=3)
+    ap.add_argument("--latent-cpu-iters", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is "
@@ -539,6 +602,11 @@ def main():
                             args.rep_cpu_samples) \
         if args.rep_samples > 0 else None
 
+    print("[bench] latent", file=sys.stderr, flush=True)
+    latent = latent_bench(c, X, y, idx, m, args.latent_iters,
+                          args.latent_site_iters, args.latent_cpu_iters, 7) \
+        if args.latent_iters > 0 else None
+
     print("[bench] conjugate grid", file=sys.stderr, flush=True)
     grid = conj_grid_bench(args.grid_n, 15, 5, 3, args.grid_cpu_points) \
         if args.grid_n > 0 else None
@@ -586,6 +654,7 @@ def main():
         "conj_grid": grid,
         "stress": stress,
         "replicates": reps,
+        "latent": latent,
         "mixed": mixed,
         "bf_tune": f64["tune"],
         "knn_s": knn_s,

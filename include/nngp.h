@@ -45,6 +45,13 @@ enum nngp_status {
   NNGP_ERR_TIMEOUT = 8      /* nngp_server_wait: nothing finished in time   */
 };
 
+/* Order of the latent sampler's w sweep (nngp_opts.sweep). */
+enum nngp_sweep {
+  NNGP_SWEEP_SITE_ORDER = 0, /* sites 0..n-1, the reference's Gibbs scan    */
+  NNGP_SWEEP_COLOURED = 1    /* colour classes of the moral graph, one
+                                parallel phase per colour                   */
+};
+
 enum nngp_precision {
   NNGP_FP64 = 0,            /* plain FP64: the reference's arithmetic       */
   NNGP_MIXED = 1            /* FP32 factor/solve + one FP64 refinement step */
@@ -81,7 +88,8 @@ typedef struct nngp_opts {
                                as in nngp_tune_report) pinned for a model or
                                a grid context; threads == 0: the default,
                                or the autotuner's choice                    */
-  int reserved[5];
+  int sweep;                /* enum nngp_sweep, for nngp_sNNGP              */
+  int reserved[4];
 } nngp_opts;
 
 /* Message for the last nonzero status returned on this host thread. */
@@ -111,6 +119,18 @@ int nngp_mkNNIndxCB(int n, int m, const double *coords, int *nnIndx,
  * distance ties are broken). */
 int nngp_mkNNIndx(int n, int m, const double *coords, int *nnIndx,
                   double *nnDist, int *nnIndxLU, int nThreads);
+
+/* mkUIndx — replaces src/nn.cpp:25-56 (.Call at R/nn.R:19; the scans
+ * mkUIndx0/1/2 of src/util.cpp:72-185 and the uiIndx fill of
+ * src/nn.cpp:46-53).  For each site i, the sites holding i as a neighbor
+ * are uIndx[uIndxLU[i] .. uIndxLU[i]+uIndxLU[n+i]), and uiIndx at the same
+ * position is i's slot in that site's neighbor list.  searchType 0 and 2
+ * list them in ascending order, 1 in descending order (the reference's
+ * "opposite sorting among the children").  Caller-allocated: uIndx[nIndx],
+ * uIndxLU[2n], uiIndx[nIndx].  Runs on the device (a stable radix sort of
+ * the neighbor entries). */
+int nngp_mkUIndx(int n, int m, const int *nnIndx, int *uIndx, int *uIndxLU,
+                 int *uiIndx, const int *nnIndxLU, int searchType);
 
 /* nn2 replacement (RANN, called at R/generics.R:642, R/spConjNNGP.R:158,311):
  * exact k nearest fitted sites of each of q query sites, no ordering
@@ -197,6 +217,37 @@ int nngp_rNNGP(const double *y, const double *X, int p, int n, int m,
                int nRep, const int *repIndx, const nngp_rng *rng,
                const nngp_opts *opts, double *pBetaSamples,
                double *pThetaSamples, double *yRepSamples, int *nAccept);
+
+/* sNNGP — replaces src/sNNGP.cpp:90-482 (.Call at R/spNNGP.R:409-413).
+ * Latent-model sampler: per iteration the w sweep (n normals, drawn in
+ * site order), beta | w (p normals), tau.sq and sigma.sq from their
+ * inverse-Gamma conditionals (one Gamma deviate each, by R's method), and a
+ * Metropolis step on phi [and nu].  The 29 reference arguments in order
+ * (uIndx, uIndxLU, uiIndx as nngp_mkUIndx writes them; the sigma.sq and
+ * tau.sq tunings are accepted and unused, as in the reference), then:
+ *   rng, opts       as for nngp_rNNGP; opts->sweep picks the w sweep order
+ *                   (NNGP_SWEEP_SITE_ORDER reproduces the reference's
+ *                   chain; NNGP_SWEEP_COLOURED is a colour-by-colour scan)
+ *   pBetaSamples    p x nSamples
+ *   pThetaSamples   nTheta x nSamples
+ *   pWSamples       n x nSamples
+ *   nAccept         optional: accepted phi [, nu] proposals
+ * With rng == NULL the sweep's normals are Philox normals keyed by
+ * (opts->seed, site, iteration) and drawn on the device; the other draws
+ * use the built-in generator. */
+int nngp_sNNGP(const double *y, const double *X, int p, int n, int m,
+               const double *coords, int covModel, const int *nnIndx,
+               const int *nnIndxLU, const int *uIndx, const int *uIndxLU,
+               const int *uiIndx, const double *sigmaSqIG,
+               const double *tauSqIG, const double *phiUnif,
+               const double *nuUnif, const double *betaStarting,
+               double sigmaSqStarting, double tauSqStarting,
+               double phiStarting, double nuStarting, double sigmaSqTuning,
+               double tauSqTuning, double phiTuning, double nuTuning,
+               int nSamples, int nThreads, int verbose, int nReport,
+               const nngp_rng *rng, const nngp_opts *opts,
+               double *pBetaSamples, double *pThetaSamples,
+               double *pWSamples, int *nAccept);
 
 /* cNNGP — replaces src/cNNGP.cpp:73-358 (.Call at R/spConjNNGP.R:336,378).
  * thetaAlpha is nTheta x g ({phi, alpha[, nu]} per column).  Outputs:

@@ -1065,3 +1065,388 @@ double or_philox_normal(unsigned long long seed, unsigned int i,
   double u2 = ((double)(b >> 11) + 0.5) / 9007199254740992.0;
   return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
 }
+
+/* ------------------------------------------------------------------ */
+/* latent model: neighbors-of-neighbors index and the Gibbs sampler     */
+/* ------------------------------------------------------------------ */
+
+/* src/util.cpp:248-258 */
+static int which_slot(int a, const int *b, int n) {
+  for (int i = 0; i < n; i++)
+    if (b[i] == a) return i;
+  return -1;
+}
+
+/* src/nn.cpp:25-56.  searchType 0: the full scan of src/util.cpp:72-91
+ * (children ascending); 1: the reversed scan of :93-112 (children
+ * descending); otherwise the CSR -> CSC transpose of :115-185, whose row
+ * pointers are the running sum of the counts (children ascending). */
+int or_mkUIndx(int n, int m, const int *nnIndx, int *uIndx, int *uIndxLU,
+               int *uiIndx, const int *nnIndxLU, int searchType) {
+  if (check_nn_args(n, m)) return OR_ERR_ARG;
+  const long long nIndx = n_indx(n, m);
+  if (searchType == 0 || searchType == 1) {
+    long long l = 0;
+    for (int i = 0; i < n; i++) {
+      uIndxLU[i] = (int)l;
+      int h = 0;
+      int j0 = searchType == 0 ? 0 : n - 1;
+      int j1 = searchType == 0 ? n : i;
+      int dj = searchType == 0 ? 1 : -1;
+      for (int j = j0; j != j1; j += dj) {
+        int off, cnt;
+        or_getNNIndx(j, m, &off, &cnt);
+        for (int k = 0; k < cnt; k++)
+          if (nnIndx[off + k] == i) uIndx[l + h++] = j;
+      }
+      l += h;
+      uIndxLU[n + i] = h;
+    }
+  } else {
+    int *rp = (int *)malloc(sizeof(int) * ((size_t)n + 1));
+    int *cp = (int *)malloc(sizeof(int) * ((size_t)n + 1));
+    if (!rp || !cp) {
+      free(rp);
+      free(cp);
+      return fail(OR_ERR_ARG, "allocation failed");
+    }
+    rp[0] = 0;
+    for (int i = 1; i < n; i++) rp[i] = rp[i - 1] + nnIndxLU[n + i - 1];
+    rp[n] = (int)nIndx;
+    /* counts per column, exclusive running sum, then rows scattered in
+     * ascending order (crs_csc) */
+    for (int c = 0; c <= n; c++) cp[c] = 0;
+    for (long long t = 0; t < nIndx; t++) cp[nnIndx[t]]++;
+    int run = 0;
+    for (int c = 0; c < n; c++) {
+      int k = cp[c];
+      cp[c] = run;
+      run += k;
+    }
+    cp[n] = (int)nIndx;
+    for (int r = 0; r < n; r++)
+      for (int t = rp[r]; t < rp[r + 1]; t++) uIndx[cp[nnIndx[t]]++] = r;
+    int last = 0;
+    for (int c = 0; c <= n; c++) {
+      int k = cp[c];
+      cp[c] = last;
+      last = k;
+    }
+    for (int i = 0; i < n; i++) {
+      uIndxLU[i] = cp[i];
+      uIndxLU[n + i] = cp[i + 1] - cp[i];
+    }
+    free(rp);
+    free(cp);
+  }
+  for (int i = 0; i < n; i++) {
+    for (int j = 0; j < uIndxLU[n + i]; j++) {
+      int k = uIndx[uIndxLU[i] + j];
+      int s = which_slot(i, &nnIndx[nnIndxLU[k]], nnIndxLU[n + k]);
+      if (s < 0) return fail(OR_ERR_ARG, "c++ Rf_error: which failed");
+      uiIndx[uIndxLU[i] + j] = s;
+    }
+  }
+  return OR_OK;
+}
+
+/* Standard exponential deviate: Ahrens & Dieter (1972), algorithm SA, as
+ * R's exp_rand.  qtab[k] = sum_{j=1..k+1} (ln 2)^j / j!. */
+static double r_exp_rand(const or_rng *r) {
+  static const double qtab[16] = {
+      0.6931471805599453, 0.933373687519046,  0.9888777961838676,
+      0.9984959252914961, 0.9998292811061389, 0.9999833164100728,
+      0.9999985691438767, 0.9999998906925558, 0.9999999924734159,
+      0.9999999995283275, 0.9999999999728814, 0.9999999999985598,
+      0.999999999999929,  0.9999999999999968, 0.9999999999999999, 1.0};
+  double acc = 0.0;
+  double u = r->unif_rand(r->state);
+  while (u <= 0.0 || u >= 1.0) u = r->unif_rand(r->state);
+  for (;;) {
+    u += u;
+    if (u > 1.0) break;
+    acc += qtab[0];
+  }
+  u -= 1.0;
+  if (u <= qtab[0]) return acc + u;
+  int i = 0;
+  double us = r->unif_rand(r->state), umin = us;
+  do {
+    us = r->unif_rand(r->state);
+    if (us < umin) umin = us;
+    i++;
+  } while (u > qtab[i]);
+  return acc + umin * qtab[0];
+}
+
+/* Gamma(a, scale) deviate for a >= 1: Ahrens & Dieter (1982), algorithm GD
+ * (Comm. ACM 25:47-54), the method of R's rgamma for a >= 1, with the
+ * hat-acceptance exp(q) - 1 taken as expm1(q).  Returns NaN for a < 1. */
+static double gd_quot(double t, double s, double s2, double q0) {
+  const double v = t / (s + s);
+  if (fabs(v) <= 0.25) {
+    double pa = 0.1233795;
+    pa = pa * v - 0.1367177;
+    pa = pa * v + 0.1423657;
+    pa = pa * v - 0.1662921;
+    pa = pa * v + 0.2000062;
+    pa = pa * v - 0.250003;
+    pa = pa * v + 0.3333333;
+    return q0 + 0.5 * t * t * pa * v;
+  }
+  return q0 - s * t + 0.25 * t * t + (s2 + s2) * log(1.0 + v);
+}
+
+double or_rgamma(const or_rng *r, double a, double scale) {
+  if (!(a >= 1.0) || !(scale > 0.0)) return NAN;
+  const double s2 = a - 0.5, s = sqrt(s2);
+  const double d = 5.656854 - 12.0 * s;
+  /* immediate acceptance: t >= 0 */
+  double t = r->norm_rand(r->state);
+  double x = s + 0.5 * t;
+  const double xx = x * x;
+  if (t >= 0.0) return scale * xx;
+  /* squeeze acceptance */
+  double u = r->unif_rand(r->state);
+  if (d * u <= t * t * t) return scale * xx;
+  const double rr = 1.0 / a;
+  double q0 = 2.424e-4;
+  q0 = q0 * rr + 2.4511e-4;
+  q0 = q0 * rr - 7.388e-5;
+  q0 = q0 * rr + 0.00144121;
+  q0 = q0 * rr + 0.00801191;
+  q0 = q0 * rr + 0.02083148;
+  q0 = q0 * rr + 0.04166669;
+  q0 = q0 * rr;
+  double b, si, c;
+  if (a <= 3.686) {
+    b = 0.463 + s + 0.178 * s2;
+    si = 1.235;
+    c = 0.195 / s - 0.079 + 0.16 * s;
+  } else if (a <= 13.022) {
+    b = 1.654 + 0.0076 * s2;
+    si = 1.68 / s + 0.275;
+    c = 0.062 / s + 0.024;
+  } else {
+    b = 1.77;
+    si = 0.75;
+    c = 0.1515 / s;
+  }
+  /* quotient acceptance */
+  if (x > 0.0 && log(1.0 - u) <= gd_quot(t, s, s2, q0)) return scale * xx;
+  /* double-exponential rejection */
+  for (;;) {
+    const double e = r_exp_rand(r);
+    u = r->unif_rand(r->state);
+    u = u + u - 1.0;
+    t = u < 0.0 ? b - si * e : b + si * e;
+    if (t < -0.71874483771719) continue;
+    const double q = gd_quot(t, s, s2, q0);
+    if (q <= 0.0) continue;
+    if (c * fabs(u) <= expm1(q) * exp(e - 0.5 * t * t)) break;
+  }
+  x = s + 0.5 * t;
+  return scale * x * x;
+}
+
+/* One site of the latent Gibbs sweep, src/sNNGP.cpp:260-287: the full
+ * conditional of w_i given its neighbors, the sites that hold it as a
+ * neighbor and their other neighbors; w_i <- mu var + sqrt(var) z. */
+static void latent_site(int i, int n, int p, const double *y, const double *X,
+                        const double *beta, double tau2, const double *B,
+                        const double *F, const int *nnIndx,
+                        const int *nnIndxLU, const int *uIndx,
+                        const int *uIndxLU, const int *uiIndx, double z,
+                        double *w) {
+  double a = 0.0, v = 0.0;
+  for (int j = 0; j < uIndxLU[n + i]; j++) {
+    const int jj = uIndx[uIndxLU[i] + j];
+    const int o = nnIndxLU[jj];
+    double b = 0.0;
+    for (int k = 0; k < nnIndxLU[n + jj]; k++) {
+      const int kk = nnIndx[o + k];
+      if (kk != i) b += B[o + k] * w[kk];
+    }
+    const double aij = w[jj] - b;
+    const double bij = B[o + uiIndx[uIndxLU[i] + j]];
+    a += bij * aij / F[jj];
+    v += bij * bij / F[jj];
+  }
+  double e = 0.0;
+  for (int j = 0; j < nnIndxLU[n + i]; j++)
+    e += B[nnIndxLU[i] + j] * w[nnIndx[nnIndxLU[i] + j]];
+  const double mu = (y[i] - ddot(p, &X[i], n, beta, 1)) / tau2 + e / F[i] + a;
+  const double var = 1.0 / (1.0 / tau2 + 1.0 / F[i] + v);
+  w[i] = mu * var + sqrt(var) * z;
+}
+
+/* sum_i (w_i - B_i . w_N(i))^2 / F_i (and sum log F_i), the loops at
+ * src/sNNGP.cpp:319-333 */
+static double latent_quad(const double *B, const double *F, const double *w,
+                          int n, const int *nnIndx, const int *nnIndxLU,
+                          double *logDet) {
+  double a = 0.0, ld = 0.0;
+  for (int i = 0; i < n; i++) {
+    double b;
+    if (nnIndxLU[n + i] > 0) {
+      double e = 0.0;
+      for (int j = 0; j < nnIndxLU[n + i]; j++)
+        e += B[nnIndxLU[i] + j] * w[nnIndx[nnIndxLU[i] + j]];
+      b = w[i] - e;
+    } else {
+      b = w[i];
+    }
+    a += b * b / F[i];
+    ld += log(F[i]);
+  }
+  if (logDet) *logDet = ld;
+  return a;
+}
+
+/* src/sNNGP.cpp:90-482: latent NNGP sampler.  Per iteration: the w sweep
+ * (n normals, site order), beta | w (p normals), tau.sq | w, beta and
+ * sigma.sq | w (one Gamma deviate each), then a Metropolis step on phi
+ * [and nu] with B, F recomputed at the new sigma.sq (C = sigma.sq R, no
+ * nugget; src/sNNGP.cpp:24-73).  visit == NULL sweeps the sites in order
+ * 0..n-1 as the reference does; otherwise in the order visit[0..n-1],
+ * with the same n normals drawn in site order before the sweep (site i
+ * always uses normal i).  pWSamples n x nSamples. */
+int or_sNNGP(const double *y, const double *X, int p, int n, int m,
+             const double *coords, int covModel, const int *nnIndx,
+             const int *nnIndxLU, const int *uIndx, const int *uIndxLU,
+             const int *uiIndx, const double *sigmaSqIG,
+             const double *tauSqIG, const double *phiUnif,
+             const double *nuUnif, const double *betaStarting,
+             double sigmaSqStarting, double tauSqStarting,
+             double phiStarting, double nuStarting, double phiTuning,
+             double nuTuning, int nSamples, int nThreads, const int *visit,
+             const or_rng *rng, double *pBetaSamples, double *pThetaSamples,
+             double *pWSamples, int *nAccept) {
+  if (covModel < 0 || covModel > 3)
+    return fail(OR_ERR_COV_MODEL,
+                "c++ Rf_error: cov.model is not correctly specified");
+  const int matern = covModel == 2;
+  const int nTheta = matern ? 4 : 3;
+  const long long nIndx = n_indx(n, m);
+  const size_t nb = (size_t)(nIndx > 0 ? nIndx : 1);
+  double *B = (double *)malloc(sizeof(double) * nb);
+  double *F = (double *)malloc(sizeof(double) * (size_t)n);
+  double *Bc = (double *)malloc(sizeof(double) * nb);
+  double *Fc = (double *)malloc(sizeof(double) * (size_t)n);
+  double *w = (double *)calloc((size_t)n, sizeof(double));
+  double *z = (double *)malloc(sizeof(double) * (size_t)n);
+  double *tn = (double *)malloc(sizeof(double) * (size_t)n);
+  double *XtX = (double *)malloc(sizeof(double) * (size_t)p * p + 1);
+  double *tpp = (double *)malloc(sizeof(double) * (size_t)p * p + 1);
+  double *tp = (double *)malloc(sizeof(double) * (size_t)p + 1);
+  double *tp2 = (double *)malloc(sizeof(double) * (size_t)p + 1);
+  double *beta = (double *)malloc(sizeof(double) * (size_t)p + 1);
+  int st = OR_OK;
+  if (!B || !F || !Bc || !Fc || !w || !z || !tn || !XtX || !tpp || !tp ||
+      !tp2 || !beta) {
+    st = fail(OR_ERR_ARG, "allocation failed");
+    goto done;
+  }
+  {
+    double th[4] = {sigmaSqStarting, tauSqStarting, phiStarting,
+                    matern ? nuStarting : 0.0};
+    const double sA = sigmaSqIG[0], sB = sigmaSqIG[1];
+    const double tA = tauSqIG[0], tB = tauSqIG[1];
+    const double pA = phiUnif[0], pB = phiUnif[1];
+    const double nA = matern ? nuUnif[0] : 0.0, nB = matern ? nuUnif[1] : 0.0;
+    for (int j = 0; j < p; j++) beta[j] = betaStarting[j];
+    /* X^T X (dgemm 'T','N', src/sNNGP.cpp:244) */
+    for (int j = 0; j < p; j++)
+      for (int i = 0; i < p; i++) {
+        double s = 0.0;
+        for (int l = 0; l < n; l++) s += X[l + (size_t)i * n] * X[l + (size_t)j * n];
+        XtX[i + j * p] = s;
+      }
+    st = bf_common(B, F, coords, nnIndx, nnIndxLU, n, m, th[0], 0.0, th[2],
+                   th[3], covModel, nThreads);
+    int accept = 0;
+    for (int s = 0; s < nSamples && st == OR_OK; s++) {
+      /* w | rest */
+      for (int i = 0; i < n; i++) z[i] = r_rnorm(rng, 0.0, 1.0);
+      for (int t = 0; t < n; t++) {
+        const int i = visit ? visit[t] : t;
+        latent_site(i, n, p, y, X, beta, th[1], B, F, nnIndx, nnIndxLU, uIndx,
+                    uIndxLU, uiIndx, z[i], w);
+      }
+      /* beta | rest (src/sNNGP.cpp:292-308) */
+      for (int i = 0; i < n; i++) tn[i] = (y[i] - w[i]) / th[1];
+      for (int j = 0; j < p; j++) tp[j] = ddot(n, &X[(size_t)j * n], 1, tn, 1);
+      for (int k = 0; k < p * p; k++) tpp[k] = XtX[k] / th[1];
+      if (or_potrf_L(p, tpp) || or_potri_L(p, tpp)) {
+        st = fail(OR_ERR_NOT_PD, "c++ Rf_error: dpotrf failed\n");
+        break;
+      }
+      or_symv_L(p, tpp, tp, tp2);
+      if (or_potrf_L(p, tpp)) {
+        st = fail(OR_ERR_NOT_PD, "c++ Rf_error: dpotrf failed\n");
+        break;
+      }
+      mvrnorm(rng, beta, tp2, tpp, p);
+      /* tau.sq | rest (:313-317) */
+      for (int i = 0; i < n; i++)
+        tn[i] = y[i] - w[i] - ddot(p, &X[i], n, beta, 1);
+      th[1] = 1.0 / or_rgamma(rng, tA + n / 2.0,
+                              1.0 / (tB + 0.5 * ddot(n, tn, 1, tn, 1)));
+      /* sigma.sq | rest (:322-341), B and F of the previous sigma.sq */
+      double a = latent_quad(B, F, w, n, nnIndx, nnIndxLU, NULL);
+      th[0] = 1.0 / or_rgamma(rng, sA + n / 2.0, 1.0 / (sB + 0.5 * a * th[0]));
+      /* phi [, nu] | rest: Metropolis on the logit scale (:343-420) */
+      st = bf_common(B, F, coords, nnIndx, nnIndxLU, n, m, th[0], 0.0, th[2],
+                     th[3], covModel, nThreads);
+      if (st) break;
+      double ld;
+      a = latent_quad(B, F, w, n, nnIndx, nnIndxLU, &ld);
+      double lpCur = -0.5 * ld - 0.5 * a;
+      lpCur += log(th[2] - pA) + log(pB - th[2]);
+      if (matern) lpCur += log(th[3] - nA) + log(nB - th[3]);
+      const double phiC =
+          logit_inv(r_rnorm(rng, logit_ab(th[2], pA, pB), phiTuning), pA, pB);
+      double nuC = 0.0;
+      if (matern)
+        nuC = logit_inv(r_rnorm(rng, logit_ab(th[3], nA, nB), nuTuning), nA,
+                        nB);
+      st = bf_common(Bc, Fc, coords, nnIndx, nnIndxLU, n, m, th[0], 0.0, phiC,
+                     nuC, covModel, nThreads);
+      if (st) break;
+      a = latent_quad(Bc, Fc, w, n, nnIndx, nnIndxLU, &ld);
+      double lpCand = -0.5 * ld - 0.5 * a;
+      lpCand += log(phiC - pA) + log(pB - phiC);
+      if (matern) lpCand += log(nuC - nA) + log(nB - nuC);
+      if (r_runif(rng, 0.0, 1.0) <= exp(lpCand - lpCur)) {
+        double *tb = B;
+        B = Bc;
+        Bc = tb;
+        double *tf = F;
+        F = Fc;
+        Fc = tf;
+        th[2] = phiC;
+        if (matern) th[3] = nuC;
+        accept++;
+      }
+      for (int j = 0; j < p; j++) pBetaSamples[(size_t)s * p + j] = beta[j];
+      for (int k = 0; k < nTheta; k++)
+        pThetaSamples[(size_t)s * nTheta + k] = th[k];
+      for (int i = 0; i < n; i++) pWSamples[(size_t)s * n + i] = w[i];
+    }
+    if (nAccept) *nAccept = accept;
+  }
+done:
+  free(B);
+  free(F);
+  free(Bc);
+  free(Fc);
+  free(w);
+  free(z);
+  free(tn);
+  free(XtX);
+  free(tpp);
+  free(tp);
+  free(tp2);
+  free(beta);
+  return st;
+}

@@ -27,6 +27,14 @@
  *   or_rNNGPReplicated  dense covariance s2 R + t2 I (W = (I-B)^-1 F^1/2)
  *   or_cNNGP          dense GLS when m >= n-1
  *   or_philox*        Random123 known-answer vectors; moments and a KS test
+ *   or_mkUIndx        the three search types against each other and a
+ *                     numpy transpose; every (child, slot) names the site
+ *   or_rgamma         Kolmogorov-Smirnov against scipy's gamma CDF over
+ *                     shapes in every branch of algorithm GD
+ *   or_sNNGP          the w update equals the full conditional of w_i from
+ *                     the dense precision (I-B)^T F^-1 (I-B) + I / tau.sq;
+ *                     a chain under a permuted visit order equals the
+ *                     permuted-scan sampler; posterior recovery
  * Parity unpinned (no R in this image): R's RNG stream, the exact bits of
  * Rmath's bessel_k_ex / gammafn, R's LAPACK rounding order, and the order
  * rsort_with_index gives to exactly tied distances.
@@ -159,6 +167,35 @@ int or_rNNGPReplicated(const double *X, int p, int n, int m,
 void or_philox4x32_10(unsigned int c[4], unsigned int k0, unsigned int k1);
 double or_philox_normal(unsigned long long seed, unsigned int i,
                         unsigned int s);
+
+/* src/nn.cpp:25-56 (mkUIndx): for each site i the sites that hold i as a
+ * neighbor, uIndx[uIndxLU[i] .. + uIndxLU[n+i]), and i's slot in each of
+ * their neighbor lists, uiIndx.  searchType 0 / 1: the scans of
+ * src/util.cpp:72-112 (children ascending / descending); otherwise the
+ * CSR -> CSC transpose of src/util.cpp:115-185 (ascending). */
+int or_mkUIndx(int n, int m, const int *nnIndx, int *uIndx, int *uIndxLU,
+               int *uiIndx, const int *nnIndxLU, int searchType);
+
+/* Gamma(a, scale) deviate, a >= 1, by Ahrens & Dieter's algorithm GD (the
+ * method of R's rgamma for a >= 1; exponential deviates by their algorithm
+ * SA, as R's exp_rand).  NaN for a < 1. */
+double or_rgamma(const or_rng *r, double a, double scale);
+
+/* src/sNNGP.cpp:90-482, the latent-model sampler.  visit == NULL sweeps w
+ * in site order (the reference); otherwise in the order visit[0..n-1]
+ * (site i always takes the i-th of the n normals drawn before the sweep).
+ * pBetaSamples p x S, pThetaSamples nTheta x S, pWSamples n x S. */
+int or_sNNGP(const double *y, const double *X, int p, int n, int m,
+             const double *coords, int covModel, const int *nnIndx,
+             const int *nnIndxLU, const int *uIndx, const int *uIndxLU,
+             const int *uiIndx, const double *sigmaSqIG,
+             const double *tauSqIG, const double *phiUnif,
+             const double *nuUnif, const double *betaStarting,
+             double sigmaSqStarting, double tauSqStarting,
+             double phiStarting, double nuStarting, double phiTuning,
+             double nuTuning, int nSamples, int nThreads, const int *visit,
+             const or_rng *rng, double *pBetaSamples, double *pThetaSamples,
+             double *pWSamples, int *nAccept);
 
 #ifdef __cplusplus
 }

@@ -89,6 +89,16 @@ def lib():
                                        C.c_uint]
         L.or_philox_normal.restype = C.c_double
         L.or_philox_normal.argtypes = [C.c_ulonglong, C.c_uint, C.c_uint]
+        L.or_mkUIndx.argtypes = [C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip,
+                                 C.c_int]
+        L.or_rgamma.restype = C.c_double
+        L.or_rgamma.argtypes = [C.POINTER(OrRng), C.c_double, C.c_double]
+        L.or_sNNGP.argtypes = (
+            [_dp, _dp, C.c_int, C.c_int, C.c_int, _dp, C.c_int, _ip, _ip,
+             _ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp]
+            + [C.c_double] * 6
+            + [C.c_int, C.c_int, _vp, C.POINTER(OrRng), _dp, _dp, _dp,
+               C.POINTER(C.c_int)])
         _lib = L
     return _lib
 
@@ -353,3 +363,50 @@ def philox_normals(seed, q, S):
         for s in range(S):
             z[i * S + s] = L.or_philox_normal(seed, i, s)
     return z
+
+
+def mkUIndx(n, m, nnIndx, nnIndxLU, searchType=2):
+    """src/nn.cpp:25-56: u.indx, u.indx.lu, ui.indx of R/nn.R:1-24."""
+    ni = n_indx(n, m)
+    u = np.zeros(max(ni, 1), np.int32)
+    ui = np.zeros(max(ni, 1), np.int32)
+    ulu = np.zeros(2 * n, np.int32)
+    _check(lib().or_mkUIndx(n, m, _i32(nnIndx), u, ulu, ui, _i32(nnIndxLU),
+                            searchType))
+    return {"u.indx": u[:ni], "u.indx.lu": ulu, "ui.indx": ui[:ni]}
+
+
+def rgamma(tape, a, scale, k=1):
+    """k Gamma(a, scale) deviates by algorithm GD from the TapeRng tape."""
+    r = tape.struct(OrRng)
+    return np.array([lib().or_rgamma(C.byref(r), a, scale) for _ in range(k)])
+
+
+def sNNGP(y, X, p, n, m, coords_flat, covModel, nnIndx, nnIndxLU, uIndx,
+          uIndxLU, uiIndx, sigmaSqIG, tauSqIG, phiUnif, nuUnif, betaStarting,
+          sigmaSqStarting, tauSqStarting, phiStarting, nuStarting, phiTuning,
+          nuTuning, nSamples, rng: TapeRng, nThreads=1, visit=None):
+    """src/sNNGP.cpp:90-482; X n x p (2-D) or column-major flat.  visit:
+    the site order of the w sweep (None: 0..n-1, the reference's)."""
+    nTheta = 4 if covModel == 2 else 3
+    beta = np.zeros(p * nSamples)
+    theta = np.zeros(nTheta * nSamples)
+    w = np.zeros(max(n * nSamples, 1))
+    acc = C.c_int()
+    rs = rng.struct(OrRng)
+    vis = None if visit is None else _i32(visit)
+    Xf = _f64(np.asarray(X, np.float64).T.reshape(-1)
+              if np.asarray(X).ndim == 2 else X)
+    _check(lib().or_sNNGP(
+        _f64(y), Xf, p, n, m, _f64(coords_flat), covModel, _i32(nnIndx),
+        _i32(nnIndxLU), _i32(uIndx), _i32(uIndxLU), _i32(uiIndx),
+        _f64(sigmaSqIG), _f64(tauSqIG), _f64(phiUnif),
+        _f64(nuUnif if nuUnif is not None else [0.0, 0.0]),
+        _f64(betaStarting), sigmaSqStarting, tauSqStarting, phiStarting,
+        nuStarting, phiTuning, nuTuning, nSamples, nThreads,
+        None if vis is None else vis.ctypes.data_as(C.c_void_p),
+        C.byref(rs), beta, theta, w, C.byref(acc)))
+    return {"p.beta.samples": beta.reshape(nSamples, p).T.copy(),
+            "p.theta.samples": theta.reshape(nSamples, nTheta).T.copy(),
+            "p.w.samples": w[:n * nSamples].reshape(nSamples, n).T.copy(),
+            "accept": acc.value}

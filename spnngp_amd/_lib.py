@@ -41,7 +41,7 @@ EXPORTS = (
     "nngp_server_get_stats",
     "nngp_cvgrid_create", "nngp_cvgrid_run", "nngp_cvgrid_destroy",
     "nngp_model_autotune", "nngp_model_set_shape", "nngp_rNNGPReplicated",
-    "nngp_rng_r_create", "nngp_rng_r_destroy",
+    "nngp_rng_r_create", "nngp_rng_r_destroy", "nngp_mkUIndx", "nngp_sNNGP",
 )
 
 _RNG_FN = C.CFUNCTYPE(C.c_double, C.c_void_p)
@@ -57,7 +57,7 @@ class NngpOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("device", C.c_int),
                 ("seed", C.c_ulonglong), ("interrupt", _INT_FN),
                 ("interrupt_arg", C.c_void_p), ("shape", C.c_int * 3),
-                ("reserved", C.c_int * 5)]
+                ("sweep", C.c_int), ("reserved", C.c_int * 4)]
 
 
 class NngpPassOut(C.Structure):
@@ -177,6 +177,12 @@ def load():
     L.nngp_rng_r_create.argtypes = [C.c_uint, C.POINTER(NngpRng)]
     L.nngp_rng_r_destroy.argtypes = [C.POINTER(NngpRng)]
     L.nngp_rng_r_destroy.restype = None
+    L.nngp_mkUIndx.argtypes = [_i, _i, _ip, _ip, _ip, _ip, _ip, _i]
+    L.nngp_sNNGP.argtypes = (
+        [_dp, _dp, _i, _i, _i, _dp, _i, _ip, _ip, _ip, _ip, _ip, _dp, _dp,
+         _dp, _vp, _dp] + [_d] * 8
+        + [_i, _i, _i, _i, C.POINTER(NngpRng), C.POINTER(NngpOpts), _dp, _dp,
+           _dp, C.POINTER(_i)])
     L.nngp_model_profile.argtypes = [_vp, _i]
     L.nngp_model_timing.argtypes = [_vp, _dp]
     L.nngp_server_create.argtypes = [C.POINTER(_vp), _dp, _dp, _dp, _i, _i,

@@ -278,7 +278,7 @@ int Model::create(int n_, int m_, int p_, const double *coords,
   p = p_;
   cov = cov_;
   precision = opts ? opts->precision : 0;
-  nz = (X && y) ? p + 1 : 0;
+  nz = (y && (X || p == 0)) ? p + 1 : 0;  // p == 0: y is the one column
   zs = (nz + 1) & ~1;
   nIndx = n_indx(n, m);
   NNGP_TRY(st.create());
@@ -340,7 +340,8 @@ int Model::create(int n_, int m_, int p_, const double *coords,
 
 int Model::bf(const CovArgs &c, int slot, bool storeB) {
   if (storeB && !B[slot].p)
-    NNGP_TRY(B[slot].alloc(sizeof(double) * (size_t)std::max(nIndx, 1LL)));
+    // 64 entries of padding: the latent sweep reads chunks of 16
+    NNGP_TRY(B[slot].alloc(sizeof(double) * ((size_t)std::max(nIndx, 1LL) + 64)));
   BfArgs a;
   a.xy = xy.as<double2>();
   a.nnIndx = nnIndx.as<int>();

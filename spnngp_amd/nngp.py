@@ -46,10 +46,11 @@ def _cm(a):
     return _f64(a)
 
 
-def _opts(precision: int = 0, device: int = 0, seed: int = 1, shape=None):
+def _opts(precision: int = 0, device: int = 0, seed: int = 1, shape=None,
+          sweep: int = 0):
     sh = (C.c_int * 3)(*(shape if shape is not None else (0, 0, 0)))
-    return NngpOpts(precision, device, seed, _lib._INT_FN(), None, sh,
-                    (C.c_int * 5)())
+    return NngpOpts(precision, device, seed, _lib._INT_FN(), None, sh, sweep,
+                    (C.c_int * 4)())
 
 
 def device_count() -> int:
@@ -231,6 +232,57 @@ def rNNGP(y, X, p, n, m, coords, covModel, nnIndx, nnIndxLU, sigmaSqIG,
     if nRep:
         out["y.rep.samples"] = rep.reshape(nRep, n).T.copy()
     return out
+
+
+def mkUIndx(n, m, nnIndx, nnIndxLU, search_type=2):
+    """mkUIndx (src/nn.cpp:25-56), R/nn.R:1-24: the sites holding each site
+    as a neighbor (u.indx, u.indx.lu) and its slot in their lists (ui.indx).
+    search_type 0 and 2 list them ascending, 1 descending."""
+    ni = n_indx(n, m)
+    u = np.zeros(max(ni, 1), np.int32)
+    ui = np.zeros(max(ni, 1), np.int32)
+    ulu = np.zeros(2 * n, np.int32)
+    check(load().nngp_mkUIndx(n, m, _i32(nnIndx), u, ulu, ui, _i32(nnIndxLU),
+                              int(search_type)))
+    return {"u.indx": u[:ni], "u.indx.lu": ulu, "ui.indx": ui[:ni]}
+
+
+SWEEPS = {"site": 0, "coloured": 1}
+
+
+def sNNGP(y, X, p, n, m, coords, covModel, nnIndx, nnIndxLU, uIndx, uIndxLU,
+          uiIndx, sigmaSqIG, tauSqIG, phiUnif, nuUnif, betaStarting,
+          sigmaSqStarting, tauSqStarting, phiStarting, nuStarting,
+          sigmaSqTuning, tauSqTuning, phiTuning, nuTuning, nSamples,
+          nThreads=1, verbose=0, nReport=100, rng=None, seed=1,
+          sweep="site"):
+    """sNNGP (src/sNNGP.cpp:90-482), arguments in the .Call order of
+    R/spNNGP.R:409-413.  sweep: "site" (the reference's order 0..n-1) or
+    "coloured" (colour classes of the moral graph).  rng as for rNNGP; None:
+    device Philox normals for the sweep and the built-in generator for the
+    rest, seeded by `seed`."""
+    cm = cov_index(covModel)
+    nTheta = 4 if cm == 2 else 3
+    beta = np.zeros(p * nSamples)
+    theta = np.zeros(nTheta * nSamples)
+    w = np.zeros(max(n * nSamples, 1))
+    acc = C.c_int()
+    opts = _opts(0, 0, seed, sweep=SWEEPS[sweep] if isinstance(sweep, str)
+                 else int(sweep))
+    nu_u = _f64(nuUnif if nuUnif is not None else [0.0, 0.0])
+    check(load().nngp_sNNGP(
+        _f64(y), _cm(X), p, n, m, _cm(coords), cm, _i32(nnIndx),
+        _i32(nnIndxLU), _i32(uIndx), _i32(uIndxLU), _i32(uiIndx),
+        _f64(sigmaSqIG), _f64(tauSqIG), _f64(phiUnif),
+        nu_u.ctypes.data_as(C.c_void_p), _f64(betaStarting),
+        sigmaSqStarting, tauSqStarting, phiStarting, nuStarting,
+        sigmaSqTuning, tauSqTuning, phiTuning, nuTuning, nSamples, nThreads,
+        verbose, nReport, _rng_struct(rng), C.byref(opts), beta, theta, w,
+        C.byref(acc)))
+    return {"p.beta.samples": beta.reshape(nSamples, p).T.copy(),
+            "p.theta.samples": theta.reshape(nSamples, nTheta).T.copy(),
+            "p.w.samples": w[:n * nSamples].reshape(nSamples, n).T.copy(),
+            "accept": acc.value}
 
 
 def cNNGP(y, X, p, n, coords, thetaAlpha, nnIndx, nnIndxLU, X0, coords0, n0,

@@ -1,5 +1,5 @@
-"""R-level front ends of spNNGP's hot path: spNNGP(method="response")
-(R/spNNGP.R), spConjNNGP() (R/spConjNNGP.R), predict() and fitted() for
+"""R-level front ends of spNNGP's hot path: spNNGP(method="response" or
+"latent") (R/spNNGP.R), spConjNNGP() (R/spConjNNGP.R), predict() and fitted() for
 both (R/generics.R:583-675, :1-268) and the CV scores crps() / rmspe()
 (R/util.R:247-255).
 
@@ -17,8 +17,9 @@ Differences from R that a caller sees:
     source (a nngp_rng-compatible object).  rng=RRng(s), the library's
     copy of R's default generator, gives the stream of R's set.seed(s).
     Given the same draws the results are the reference's.
-  * Only method = "response" with family = "gaussian" is on this path
-    (the latent, binomial and SLGP models are not).
+  * method = "response" and "latent" with family = "gaussian" are on this
+    path (the binomial and SLGP models are not).  predict() of a latent
+    fit is not (the reference's sNNGPPredict).
 """
 from __future__ import annotations
 
@@ -138,12 +139,20 @@ def spNNGP(y, X, coords, *, starting, tuning, priors, n_samples,
            n_omp_threads=1, search_type="cb", ord=None,
            return_neighbor_info=False, neighbor_info=None, fit_rep=False,
            sub_sample=None, verbose=False, n_report=100, rng=None, seed=1,
-           precision=0, x_names=None):
-    """spNNGP(y ~ X - 1, coords, method = "response", ...) of R/spNNGP.R.
-    Returns a dict with R's element names (p.beta.samples n.samples x p,
-    p.theta.samples n.samples x nTheta, ...)."""
-    if method.lower() != "response":
-        raise NotImplementedError("only method = 'response' is on this path")
+           precision=0, x_names=None, u_search_type=2, sweep="site"):
+    """spNNGP(y ~ X - 1, coords, method = "response" | "latent", ...) of
+    R/spNNGP.R.  Returns a dict with R's element names (p.beta.samples
+    n.samples x p, p.theta.samples n.samples x nTheta, and for the latent
+    model p.w.samples n x n.samples).  The latent model takes
+    u_search_type (R's u.search.type, default 2) and sweep, the order of
+    its w sweep ("site": the reference's; "coloured": colour classes of the
+    moral graph, a parallel Gibbs scan)."""
+    method = method.lower()
+    if method not in ("response", "latent"):
+        raise ValueError("error: specified method '" + method
+                         + "' is not a valid option; choose from response, "
+                         "latent")
+    latent = method == "latent"
     y = np.asarray(y, dtype=np.float64).reshape(-1)
     X = np.asarray(X, dtype=np.float64)
     if X.ndim == 1:
@@ -195,7 +204,8 @@ def spNNGP(y, X, coords, *, starting, tuning, priors, n_samples,
             raise ValueError(f"error: {k} must be specified in starting "
                              "value list")
     tu = _lower(tuning)
-    for k in ("sigma.sq", "tau.sq", "phi") + (("nu",) if matern else ()):
+    need_tu = ("phi",) if latent else ("sigma.sq", "tau.sq", "phi")
+    for k in need_tu + (("nu",) if matern else ()):
         if k not in tu:
             raise ValueError(f"error: {k} must be specified in tuning value "
                              "list")
@@ -219,6 +229,15 @@ def spNNGP(y, X, coords, *, starting, tuning, priors, n_samples,
         rep_indx = np.zeros(n_samples, dtype=np.int32)
         rep_indx[s_indx - 1] = 1
         n_rep = int(len(s_indx))
+
+    if latent:
+        return _sNNGP_fit(
+            y_o, X_o, coords_o, ord, p, n, n_neighbors, cov_model, matern,
+            nn_indx, nn_indx_lu, nn_run_time, neighbor_info, sigma_sq_IG,
+            tau_sq_IG, phi_Unif, nu_Unif, beta_starting, st, tu, first,
+            n_samples, n_omp_threads, verbose, n_report, rng, seed,
+            u_search_type, sweep, fit_rep, sub_sample, s_indx,
+            return_neighbor_info, x_names, starting, priors, tuning)
 
     t0 = time.perf_counter()
     out = _nn.rNNGP(
@@ -260,6 +279,79 @@ def spNNGP(y, X, coords, *, starting, tuning, priors, n_samples,
                 "cov.model.indx": COV_MODELS.index(cov_model),
                 "starting": starting, "priors": priors, "tuning": tuning,
                 "type": ["response", "gaussian"]})
+    return res
+
+
+def _normals(rng, k, gen):
+    """k standard normals from the caller's source when it has one (R's
+    rnorm(k) order), else from the numpy Generator gen."""
+    if rng is not None and hasattr(rng, "norm"):
+        return np.asarray(rng.norm(k), dtype=np.float64)
+    return gen.standard_normal(k)
+
+
+def _sNNGP_fit(y_o, X_o, coords_o, ord, p, n, m, cov_model, matern, nn_indx,
+               nn_indx_lu, nn_run_time, neighbor_info, sigma_sq_IG, tau_sq_IG,
+               phi_Unif, nu_Unif, beta_starting, st, tu, first, n_samples,
+               n_omp_threads, verbose, n_report, rng, seed, u_search_type,
+               sweep, fit_rep, sub_sample, s_indx, return_neighbor_info,
+               x_names, starting, priors, tuning):
+    """The latent branch of R/spNNGP.R: the neighbors-of-neighbors index
+    (:332-347, or from neighbor.info :117-123), .Call("sNNGP") (:409-413),
+    and the latent fitted values and replicates (:477-484)."""
+    t0 = time.perf_counter()
+    if neighbor_info is not None and "u.indx" in neighbor_info:
+        u = {k: np.asarray(neighbor_info[k], dtype=np.int32)
+             for k in ("u.indx", "u.indx.lu", "ui.indx")}
+        u_run_time = neighbor_info.get("u.indx.run.time", 0.0)
+    else:
+        u = _nn.mkUIndx(n, m, nn_indx, nn_indx_lu, u_search_type)
+        u_run_time = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    out = _nn.sNNGP(
+        y_o, X_o, p, n, m, coords_o, cov_model, nn_indx, nn_indx_lu,
+        u["u.indx"], u["u.indx.lu"], u["ui.indx"], sigma_sq_IG, tau_sq_IG,
+        phi_Unif, nu_Unif, beta_starting, first(st, "sigma.sq"),
+        first(st, "tau.sq"), first(st, "phi"), first(st, "nu") if matern
+        else 0.0, first(tu, "sigma.sq") if "sigma.sq" in tu else 0.0,
+        first(tu, "tau.sq") if "tau.sq" in tu else 0.0, first(tu, "phi"),
+        first(tu, "nu") if matern else 0.0, n_samples, n_omp_threads,
+        int(verbose), n_report, rng=rng, seed=seed, sweep=sweep)
+    run_time = time.perf_counter() - t0
+    res = {"p.beta.samples": out["p.beta.samples"].T.copy(),
+           "p.theta.samples": out["p.theta.samples"].T.copy(),
+           "theta.names": ["sigma.sq", "tau.sq", "phi"]
+                          + (["nu"] if matern else []),
+           "x.names": list(x_names) if x_names is not None
+                      else [f"x{j + 1}" for j in range(p)],
+           "run.time": run_time, "accept": out.get("accept")}
+    if return_neighbor_info:
+        res["neighbor.info"] = {
+            "n.neighbors": m, "n.indx": n_indx_list(nn_indx, nn_indx_lu, n),
+            "nn.indx": nn_indx, "nn.indx.lu": nn_indx_lu,
+            "u.indx": u["u.indx"], "u.indx.lu": u["u.indx.lu"],
+            "ui.indx": u["ui.indx"], "ord": ord,
+            "nn.indx.run.time": nn_run_time, "u.indx.run.time": u_run_time}
+    inv = np.argsort(ord, kind="stable")  # order(ord)
+    res["coords"], res["y"], res["X"] = coords_o[inv], y_o[inv], X_o[inv]
+    res["p.w.samples"] = out["p.w.samples"][inv]
+    if fit_rep:
+        bs = res["p.beta.samples"][s_indx - 1]
+        yh = res["X"] @ bs.T + res["p.w.samples"][:, s_indx - 1]
+        res["y.hat.samples"] = yh
+        res["y.hat.quants"] = np.quantile(yh, [0.5, 0.05, 0.975], axis=1).T
+        gen = np.random.default_rng(seed)
+        tau = res["p.theta.samples"][s_indx - 1, 1]
+        noise = np.column_stack([np.sqrt(t) * _normals(rng, n, gen)
+                                 for t in tau])
+        res["y.rep.samples"] = yh + noise
+        res["y.rep.quants"] = np.quantile(res["y.rep.samples"],
+                                          [0.5, 0.05, 0.975], axis=1).T
+        res["sub.sample"], res["s.indx"] = sub_sample, s_indx
+    res.update({"n.neighbors": m, "cov.model": cov_model,
+                "cov.model.indx": COV_MODELS.index(cov_model),
+                "starting": starting, "priors": priors, "tuning": tuning,
+                "type": ["latent", "gaussian"], "sweep": sweep})
     return res
 
 
@@ -459,8 +551,10 @@ def _replicates(obj, beta, theta, verbose, n_report, seed):
 # ---------------------------------------------------------------------------
 def fitted(obj, sub_sample=None, *, n_omp_threads=1, verbose=False,
            n_report=100, seed=1):
-    """fitted.NNGP for the response and conjugate models: fitted values
-    X beta and replicates y.rep (rNNGPReplicated on the device) with their
+    """fitted.NNGP: fitted values and replicates with their median and 95%
+    quantiles.  Latent: X beta + w and y.rep = that + tau.sq noise.
+    Response and conjugate: fitted values X beta and replicates y.rep
+    (rNNGPReplicated on the device) with their
     median and 95% quantiles.  Conjugate: sub_sample is the number of exact
     posterior draws (stored ones are returned when it matches or is None).
     Response: sub_sample = {'start', 'end', 'thin'} over the stored chain
@@ -506,6 +600,21 @@ def fitted(obj, sub_sample=None, *, n_omp_threads=1, verbose=False,
     n_samples = np.asarray(obj["p.beta.samples"]).shape[0]
     ss, s_indx = _sub_sample(sub_sample, n_samples,
                              int(np.floor(0.5 * n_samples)))
+    if obj["type"][0] == "latent":
+        # X beta + w per kept sample, replicates add tau.sq noise
+        # (R/generics.R:187-200)
+        beta = np.asarray(obj["p.beta.samples"])[s_indx - 1]
+        X = np.asarray(obj["X"])
+        yh = X @ beta.T + np.asarray(obj["p.w.samples"])[:, s_indx - 1]
+        gen = np.random.default_rng(seed)
+        tau = np.asarray(obj["p.theta.samples"])[s_indx - 1, 1]
+        out = {"y.hat.samples": yh, "y.hat.quants": _quants(yh),
+               "y.rep.samples": yh + np.column_stack(
+                   [np.sqrt(t) * gen.standard_normal(X.shape[0])
+                    for t in tau])}
+        out["y.rep.quants"] = _quants(out["y.rep.samples"])
+        out["sub.sample"], out["s.indx"] = ss, s_indx
+        return out
     beta = np.asarray(obj["p.beta.samples"])[s_indx - 1]
     theta = np.asarray(obj["p.theta.samples"])[s_indx - 1].T.copy()
     out = {"y.hat.samples": np.asarray(obj["X"]) @ beta.T}
@@ -539,6 +648,9 @@ def predict(obj, X_0, coords_0, *, sub_sample=None, n_omp_threads=1,
         raise ValueError("error: coords.0 must have two columns\n")
     q = X_0.shape[0]
     m = obj["n.neighbors"]
+    if obj["type"][0] == "latent":
+        raise NotImplementedError("predict() of a latent fit (the reference's "
+                                  "sNNGPPredict) is not on this path")
     if obj["type"][0] == "conjugate":
         ta = obj["theta.alpha"]
         r = spConjNNGP(y, X, coords, theta_alpha={k: [v] for k, v in

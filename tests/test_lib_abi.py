@@ -53,3 +53,25 @@ def test_no_device_is_a_clean_error(nngp):
     with pytest.raises(nngp.NngpError) as e:
         nngp.mkNNIndxCB(c, 5)
     assert e.value.status == nngp.NNGP_ERR_HIP
+
+
+def test_sNNGP_rejects_a_bad_transpose_before_the_device(nngp, oracle):
+    n, m = 100, 5
+    rng = np.random.default_rng(3)
+    c = rng.random((n, 2))
+    c = c[np.argsort(c[:, 0], kind="stable")]
+    r = oracle.mkNNIndxCB(c, m)
+    U = oracle.mkUIndx(n, m, r["nnIndx"], r["nnIndxLU"], 2)
+    bad = U["ui.indx"].copy()
+    bad[7] = (bad[7] + 1) % 5  # now names a slot holding another site
+    X = np.ones((n, 1))
+    with pytest.raises(nngp.NngpError) as e:
+        nngp.sNNGP(np.zeros(n), X, 1, n, m, c, 0, r["nnIndx"], r["nnIndxLU"],
+                   U["u.indx"], U["u.indx.lu"], bad, [2, 1], [2, 1], [1, 30],
+                   None, [0.0], 1, 1, 5, 0, 0, 0, 0.1, 0, 2)
+    assert e.value.status == nngp.NNGP_ERR_ARG
+    with pytest.raises(nngp.NngpError) as e:  # unknown sweep order
+        nngp.sNNGP(np.zeros(n), X, 1, n, m, c, 0, r["nnIndx"], r["nnIndxLU"],
+                   U["u.indx"], U["u.indx.lu"], U["ui.indx"], [2, 1], [2, 1],
+                   [1, 30], None, [0.0], 1, 1, 5, 0, 0, 0, 0.1, 0, 2, sweep=7)
+    assert e.value.status == nngp.NNGP_ERR_ARG
