@@ -25,10 +25,10 @@
 //    per colour.  This is a Gibbs scan in another site order (colour by
 //    colour), with the same stationary distribution.
 // Phases with many sites get a grid of their own; runs of small phases go
-// to one block that walks them with a barrier between phases.  The
-// per-site arithmetic keeps the reference's operation order and no fused
-// multiply-adds, so a sweep is bit-identical to the sequential one given
-// the same B, F and normals.
+// to one block that walks them with a barrier between phases.  Each site
+// reads its children's residuals w_j - B_j . w_N(j), kept current through
+// the sweep, instead of re-forming them from the children's neighbor lists
+// (sweep_site).
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -89,13 +89,13 @@ struct SweepArgs {
   const double *B, *F, *y, *X, *z;
   double *w;
   const int *nnIndx, *uIndx, *uiIndx, *phase;
-  // per schedule position k: {site i, uIndxLU[i], uIndxLU[n+i], nnIndxLU[i]}
-  // and nnIndxLU[n+i]; per transpose entry t: {nnIndxLU[j], nnIndxLU[n+j]}
-  // of the site j = uIndx[t] (the per-site lookups of the sweep, laid out
-  // in the order it reads them)
+  // per schedule position k: {site i, uIndxLU[i], uIndxLU[n+i], 0}; per
+  // transpose entry t: {nnIndxLU[j], nnIndxLU[n+j]} of the site j = uIndx[t]
+  // (the per-site lookups of the sweep, laid out in the order it reads them)
   const int4 *meta;
-  const int *mcnt;
   const int2 *uoc;
+  const int *lu;  // nnIndxLU
+  double *R;      // residuals w_i - B_i . w_N(i) at the current w
   double beta[kMaxP];
   double tau2;
   unsigned long long seed;  // Philox key when z == nullptr
@@ -134,54 +134,99 @@ __device__ __forceinline__ double gather_dot(const SweepArgs &A, int o,
 
 // w_i from its full conditional (src/sNNGP.cpp:260-287): precision
 // 1/tau.sq + 1/F_i + sum_j B_ji^2/F_j over the sites j holding i as a
-// neighbor; mean from the data, i's own neighbors and each j's residual
-// without i.  A group of kGroup lanes takes one site: lane l forms the
-// terms of items l, l + kGroup, ... at once (item j < uc: child j; item
-// uc: i's own neighbor sum), then every lane adds the child terms in child
-// order (shuffles), so the sums run in the reference's order.  All lanes
-// of a group run the same trip counts.
+// neighbor (its children); mean from the data, i's own neighbor sum and
+// each child's residual without i's term.
+//
+// The sweep keeps r_j = w_j - B_j . w_N(j) of every site at the current w
+// (k_resid_init sets them before a sweep).  Child j's residual without i is
+// then r_j + B_ji w_i and i's own neighbor sum is w_i - r_i, so a site
+// reads one value per child instead of the child's whole neighbor list.
+// After the draw, d = w_i(new) - w_i(old) moves r_i by +d and each child's
+// r_j by -B_ji d.  No two sites of a phase are neighbors in the moral graph,
+// so within a phase no two sites share a child and these updates never
+// collide.  (The reference forms each child's sum afresh; the two agree to
+// rounding.)
+//
+// A group of kGroup lanes takes one site, lane l the children l,
+// l + kGroup, ...; every lane adds the children's terms in child order
+// (shuffles) and forms the same draw.  All lanes of a group run the same
+// trip counts.
 constexpr int kGroup = 16;
 
 __device__ void sweep_site(const SweepArgs &A, int k, int lane) {
-#pragma clang fp contract(off)
   const int n = A.n, ws = A.ws;
-  const double *w = A.w;
+  double *w = A.w;
   const int4 mt = A.meta[k];
-  const int i = mt.x, uo = mt.y, uc = mt.z, oi = mt.w;
-  const int ci = A.mcnt[k];
+  const int i = mt.x, uo = mt.y, uc = mt.z;
+  const double wi = w[(size_t)i * ws], ri = A.R[i];
   const double yi = A.y[i], fi = A.F[i];
-  double a = 0.0, v = 0.0, e = 0.0;
-  for (int j0 = 0; j0 <= uc; j0 += kGroup) {
+  double xb = 0.0;
+  for (int q = 0; q < A.p; q++) xb += A.X[(size_t)q * n + i] * A.beta[q];
+  double a = 0.0, v = 0.0;
+  int jj0 = -1, jj1 = -1;  // children of the first two rounds, kept for
+  double b0 = 0.0, b1 = 0.0;  // the residual updates
+  for (int j0 = 0; j0 < uc; j0 += kGroup) {
     double ta = 0.0, tv = 0.0;
     const int j = j0 + lane;
     if (j < uc) {
       const int t = uo + j;
       const int jj = A.uIndx[t];
-      const int2 oc = A.uoc[t];
-      const double bij = A.B[oc.x + A.uiIndx[t]];
+      const double bij = A.B[A.uoc[t].x + A.uiIndx[t]];
       const double fj = A.F[jj];
-      const double aij = w[(size_t)jj * ws] - gather_dot(A, oc.x, oc.y, i);
+      const double aij = A.R[jj] + bij * wi;
       ta = bij * aij / fj;
       tv = bij * bij / fj;
-    } else if (j == uc) {
-      // i's own neighbors never include i, so nothing is skipped (-1)
-      ta = gather_dot(A, oi, ci, -1);
+      if (j0 == 0) {
+        jj0 = jj;
+        b0 = bij;
+      } else if (j0 == kGroup) {
+        jj1 = jj;
+        b1 = bij;
+      }
     }
     const int nt = min(kGroup, uc - j0);
     for (int t = 0; t < nt; t++) {
       a += __shfl(ta, t, kGroup);
       v += __shfl(tv, t, kGroup);
     }
-    if (uc - j0 < kGroup) e = __shfl(ta, uc - j0, kGroup);
   }
-  if (lane != 0) return;
-  double xb = 0.0;
-  for (int q = 0; q < A.p; q++) xb += A.X[(size_t)q * n + i] * A.beta[q];
+  const double e = wi - ri;
   const double mu = (yi - xb) / A.tau2 + e / fi + a;
   const double var = 1.0 / (1.0 / A.tau2 + 1.0 / fi + v);
   const double zi =
       A.z ? A.z[i] : philox_normal(A.seed, (uint32_t)i, (uint32_t)A.iter);
-  A.w[(size_t)i * ws] = mu * var + sqrt(var) * zi;
+  const double wn = mu * var + sqrt(var) * zi;
+  const double d = wn - wi;
+  if (lane == 0) {
+    w[(size_t)i * ws] = wn;
+    A.R[i] = ri + d;
+  }
+  for (int j0 = 0; j0 < uc; j0 += kGroup) {
+    const int j = j0 + lane;
+    if (j >= uc) continue;
+    int jj;
+    double bij;
+    if (j0 == 0) {
+      jj = jj0;
+      bij = b0;
+    } else if (j0 == kGroup) {
+      jj = jj1;
+      bij = b1;
+    } else {
+      const int t = uo + j;
+      jj = A.uIndx[t];
+      bij = A.B[A.uoc[t].x + A.uiIndx[t]];
+    }
+    A.R[jj] = A.R[jj] - bij * d;
+  }
+}
+
+// r_i = w_i - B_i . w_N(i) for every site, before a sweep
+__global__ __launch_bounds__(kSweepGrid) void k_resid_init(SweepArgs A) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= A.n) return;
+  A.R[i] = A.w[(size_t)i * A.ws] -
+           gather_dot(A, A.lu[i], A.lu[A.n + i], -1);
 }
 
 __global__ __launch_bounds__(kSweepBlock) void k_sweep_block(SweepArgs A) {
@@ -678,14 +723,14 @@ extern "C" int nngp_sNNGP(
                        covModel, opts));
   }
   hipStream_t st = md.st.s;
-  DevBuf dy, dX, du, dui, duoc, dmeta, dmcnt, dphase, dz, dpart, dsum, dwout;
+  DevBuf dy, dX, du, dui, duoc, dmeta, dR, dphase, dz, dpart, dsum, dwout;
   NNGP_TRY(dy.alloc(sizeof(double) * (size_t)n));
   NNGP_TRY(dX.alloc(sizeof(double) * (size_t)n * p));
   NNGP_TRY(du.alloc(sizeof(int) * (size_t)std::max(nIndx, 1LL)));
   NNGP_TRY(dui.alloc(sizeof(int) * (size_t)std::max(nIndx, 1LL)));
   NNGP_TRY(duoc.alloc(sizeof(int2) * (size_t)std::max(nIndx, 1LL)));
   NNGP_TRY(dmeta.alloc(sizeof(int4) * (size_t)n));
-  NNGP_TRY(dmcnt.alloc(sizeof(int) * (size_t)n));
+  NNGP_TRY(dR.alloc(sizeof(double) * (size_t)n));
   NNGP_TRY(dphase.alloc(sizeof(int) * phase.size()));
   NNGP_TRY(dpart.alloc(sizeof(double) * kRedMax * kLatStats));
   NNGP_TRY(dsum.alloc(sizeof(double) * kLatStats));
@@ -705,12 +750,9 @@ extern "C" int nngp_sNNGP(
                                 ok ? nnIndxLU[(size_t)n + jj] : 0);
   }
   std::vector<int4> hmeta((size_t)n);
-  std::vector<int> hmcnt((size_t)n);
   for (int k = 0; k < n; k++) {
     const int i = order[(size_t)k];
-    hmeta[(size_t)k] = make_int4(i, uIndxLU[i], uIndxLU[(size_t)n + i],
-                                 nnIndxLU[i]);
-    hmcnt[(size_t)k] = nnIndxLU[(size_t)n + i];
+    hmeta[(size_t)k] = make_int4(i, uIndxLU[i], uIndxLU[(size_t)n + i], 0);
   }
   if (ut > 0) {
     NNGP_HIP(hipMemcpyAsync(du.p, uIndx, sizeof(int) * (size_t)ut,
@@ -721,8 +763,6 @@ extern "C" int nngp_sNNGP(
                             hipMemcpyHostToDevice, st));
   }
   NNGP_HIP(hipMemcpyAsync(dmeta.p, hmeta.data(), dmeta.bytes,
-                          hipMemcpyHostToDevice, st));
-  NNGP_HIP(hipMemcpyAsync(dmcnt.p, hmcnt.data(), dmcnt.bytes,
                           hipMemcpyHostToDevice, st));
   NNGP_HIP(hipMemcpyAsync(dphase.p, phase.data(), dphase.bytes,
                           hipMemcpyHostToDevice, st));
@@ -824,8 +864,9 @@ extern "C" int nngp_sNNGP(
     A.uiIndx = dui.as<int>();
     A.phase = dphase.as<int>();
     A.meta = dmeta.as<int4>();
-    A.mcnt = dmcnt.as<int>();
     A.uoc = duoc.as<int2>();
+    A.lu = md.lu.as<int>();
+    A.R = dR.as<double>();
     for (int k = 0; k < kMaxP; k++) A.beta[k] = k < p ? beta[k] : 0.0;
     A.tau2 = th[1];
     A.seed = opts ? opts->seed : 1ull;
@@ -833,6 +874,9 @@ extern "C" int nngp_sNNGP(
     A.n = n;
     A.p = p;
     A.ws = md.zs;
+    hipLaunchKernelGGL(k_resid_init, dim3((n + kSweepGrid - 1) / kSweepGrid),
+                       dim3(kSweepGrid), 0, st, A);
+    NNGP_HIP(hipGetLastError());
     for (const Seg &g : segs) {
       A.ph0 = g.ph0;
       A.ph1 = g.ph1;
