@@ -102,6 +102,7 @@ struct SweepArgs {
   int iter;                 // Philox counter word of this sweep
   int n, p, ws;             // w[i * ws]
   int ph0, ph1;             // phases [ph0, ph1)
+  int nph;                  // phases in the schedule (phase[] has nph + 1)
 };
 
 // sum_k B[o+k] w[nnIndx[o+k]] over k < cnt, skipping the entry equal to
@@ -147,47 +148,58 @@ __device__ __forceinline__ double gather_dot(const SweepArgs &A, int o,
 // collide.  (The reference forms each child's sum afresh; the two agree to
 // rounding.)
 //
-// A group of kGroup lanes takes one site, lane l the children l,
-// l + kGroup, ...; every lane adds the children's terms in child order
-// (shuffles) and forms the same draw.  All lanes of a group run the same
-// trip counts.
-constexpr int kGroup = 16;
+// A group of kGroup lanes takes one site; its children are taken kChild at
+// a time, lane l holding children l, l + kGroup, l + 2 kGroup of the chunk,
+// so every load of a chunk is in flight at once.  Every lane adds the
+// children's terms in child order (shuffles) and forms the same draw.  All
+// lanes of a group run the same trip counts.
+constexpr int kGroup = 8;
+constexpr int kPer = 3;
+constexpr int kChild = kGroup * kPer;
 
-__device__ void sweep_site(const SweepArgs &A, int k, int lane) {
+__device__ void sweep_site(const SweepArgs &A, int4 mt, int lane) {
   const int n = A.n, ws = A.ws;
   double *w = A.w;
-  const int4 mt = A.meta[k];
   const int i = mt.x, uo = mt.y, uc = mt.z;
   const double wi = w[(size_t)i * ws], ri = A.R[i];
   const double yi = A.y[i], fi = A.F[i];
   double xb = 0.0;
   for (int q = 0; q < A.p; q++) xb += A.X[(size_t)q * n + i] * A.beta[q];
   double a = 0.0, v = 0.0;
-  int jj0 = -1, jj1 = -1;  // children of the first two rounds, kept for
-  double b0 = 0.0, b1 = 0.0;  // the residual updates
-  for (int j0 = 0; j0 < uc; j0 += kGroup) {
-    double ta = 0.0, tv = 0.0;
-    const int j = j0 + lane;
-    if (j < uc) {
-      const int t = uo + j;
-      const int jj = A.uIndx[t];
-      const double bij = A.B[A.uoc[t].x + A.uiIndx[t]];
-      const double fj = A.F[jj];
-      const double aij = A.R[jj] + bij * wi;
-      ta = bij * aij / fj;
-      tv = bij * bij / fj;
-      if (j0 == 0) {
-        jj0 = jj;
-        b0 = bij;
-      } else if (j0 == kGroup) {
-        jj1 = jj;
-        b1 = bij;
+  // the first chunk's children (index, B_ji, r_j), kept for the residual
+  // updates; no other site of the phase touches r_j, so the value read here
+  // is still current when it is updated
+  int jk[kPer];
+  double bk[kPer], rk[kPer];
+  for (int j0 = 0; j0 < uc; j0 += kChild) {
+    double ta[kPer], tv[kPer];
+#pragma unroll
+    for (int c = 0; c < kPer; c++) {
+      ta[c] = 0.0;
+      tv[c] = 0.0;
+      const int j = j0 + c * kGroup + lane;
+      if (j < uc) {
+        const int t = uo + j;
+        const int jj = A.uIndx[t];
+        const double bij = A.B[A.uoc[t].x + A.uiIndx[t]];
+        const double fj = A.F[jj];
+        const double rj = A.R[jj];
+        ta[c] = bij * (rj + bij * wi) / fj;
+        tv[c] = bij * bij / fj;
+        if (j0 == 0) {
+          jk[c] = jj;
+          bk[c] = bij;
+          rk[c] = rj;
+        }
       }
     }
-    const int nt = min(kGroup, uc - j0);
-    for (int t = 0; t < nt; t++) {
-      a += __shfl(ta, t, kGroup);
-      v += __shfl(tv, t, kGroup);
+    const int nt = min(kChild, uc - j0);
+#pragma unroll
+    for (int t = 0; t < kChild; t++) {
+      if (t < nt) {
+        a += __shfl(ta[t / kGroup], t % kGroup, kGroup);
+        v += __shfl(tv[t / kGroup], t % kGroup, kGroup);
+      }
     }
   }
   const double e = wi - ri;
@@ -201,23 +213,25 @@ __device__ void sweep_site(const SweepArgs &A, int k, int lane) {
     w[(size_t)i * ws] = wn;
     A.R[i] = ri + d;
   }
-  for (int j0 = 0; j0 < uc; j0 += kGroup) {
-    const int j = j0 + lane;
-    if (j >= uc) continue;
-    int jj;
-    double bij;
-    if (j0 == 0) {
-      jj = jj0;
-      bij = b0;
-    } else if (j0 == kGroup) {
-      jj = jj1;
-      bij = b1;
-    } else {
-      const int t = uo + j;
-      jj = A.uIndx[t];
-      bij = A.B[A.uoc[t].x + A.uiIndx[t]];
+  for (int j0 = 0; j0 < uc; j0 += kChild) {
+#pragma unroll
+    for (int c = 0; c < kPer; c++) {
+      const int j = j0 + c * kGroup + lane;
+      if (j >= uc) continue;
+      int jj;
+      double bij, rj;
+      if (j0 == 0) {
+        jj = jk[c];
+        bij = bk[c];
+        rj = rk[c];
+      } else {
+        const int t = uo + j;
+        jj = A.uIndx[t];
+        bij = A.B[A.uoc[t].x + A.uiIndx[t]];
+        rj = A.R[jj];
+      }
+      A.R[jj] = rj - bij * d;
     }
-    A.R[jj] = A.R[jj] - bij * d;
   }
 }
 
@@ -229,13 +243,24 @@ __global__ __launch_bounds__(kSweepGrid) void k_resid_init(SweepArgs A) {
            gather_dot(A, A.lu[i], A.lu[A.n + i], -1);
 }
 
+// One block walks phases [ph0, ph1) with a barrier between them.  The
+// boundaries of the phase after next and the first site of the next phase
+// are read one phase ahead, so a phase starts with its sites known.
 __global__ __launch_bounds__(kSweepBlock) void k_sweep_block(SweepArgs A) {
   const int g = (int)threadIdx.x / kGroup, lane = (int)threadIdx.x % kGroup;
   const int ng = (int)blockDim.x / kGroup;
+  int a = A.phase[A.ph0], b = A.phase[A.ph0 + 1];
+  int4 first = a + g < b ? A.meta[a + g] : make_int4(0, 0, 0, 0);
   for (int L = A.ph0; L < A.ph1; L++) {
-    const int a = A.phase[L], b = A.phase[L + 1];
-    for (int k = a + g; k < b; k += ng) sweep_site(A, k, lane);
+    const int c = L + 2 <= A.nph ? A.phase[L + 2] : b;
+    const int4 cur = first;
+    if (b + g < A.n) first = A.meta[b + g];  // checked against c at use
+    for (int k = a + g; k < b; k += ng)
+      sweep_site(A, k == a + g ? cur : A.meta[k], lane);
     __syncthreads();
+    a = b;
+    b = c;
+    if (a + g >= b) first = make_int4(0, 0, 0, 0);
   }
 }
 
@@ -243,7 +268,7 @@ __global__ __launch_bounds__(kSweepGrid) void k_sweep_grid(SweepArgs A) {
   const int a = A.phase[A.ph0], b = A.phase[A.ph0 + 1];
   const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int k = a + t / kGroup;
-  if (k < b) sweep_site(A, k, t % kGroup);
+  if (k < b) sweep_site(A, A.meta[k], t % kGroup);
 }
 
 // ---------------------------------------------------------------------------
@@ -251,16 +276,16 @@ __global__ __launch_bounds__(kSweepGrid) void k_sweep_grid(SweepArgs A) {
 // block order (deterministic)
 // ---------------------------------------------------------------------------
 constexpr int kRedBlock = 256;
-constexpr int kRedMax = 1024;  // partial blocks
+constexpr int kRedMax = 2048;  // partial blocks
 constexpr int kLatStats = kMaxP + 1;
 
 struct LatSumArgs {
-  const double *B, *F, *y, *X, *w;
-  const int *nnIndx, *lu;
+  const double *F, *y, *X, *w;
+  const double *R;  // residuals w_i - B_i . w_N(i), current after a sweep
   double beta[kMaxP];
   double tau2;
   int n, p, ws, mode;  // mode 0: X^T (y - w) / tau.sq and the w quadratic
-                       // form; mode 1: ||y - w - X beta||^2
+                       // form sum_i r_i^2 / F_i; mode 1: ||y - w - X beta||^2
   double *part;        // [kRedMax][kLatStats]
 };
 
@@ -287,11 +312,7 @@ __global__ __launch_bounds__(kRedBlock) void k_lat_partial(LatSumArgs A) {
     if (A.mode == 0) {
       const double r = (A.y[i] - wi) / A.tau2;
       for (int k = 0; k < A.p; k++) acc[k] += A.X[(size_t)k * n + i] * r;
-      const int o = A.lu[i], cnt = A.lu[n + i];
-      double e = 0.0;
-      for (int k = 0; k < cnt; k++)
-        e += A.B[o + k] * A.w[(size_t)A.nnIndx[o + k] * A.ws];
-      const double b = wi - e;
+      const double b = A.R[i];
       acc[kMaxP] += b * b / A.F[i];
     } else {
       double xb = 0.0;
@@ -814,13 +835,11 @@ extern "C" int nngp_sNNGP(
       std::min(kRedMax, std::max(1, (n + kRedBlock - 1) / kRedBlock));
   auto reduce = [&](int mode, int slot) -> int {
     LatSumArgs L;
-    L.B = md.B[slot].as<double>();
+    L.R = dR.as<double>();
     L.F = md.F[slot].as<double>();
     L.y = dy.as<double>();
     L.X = dX.as<double>();
     L.w = md.Z.as<double>();
-    L.nnIndx = md.nnIndx.as<int>();
-    L.lu = md.lu.as<int>();
     for (int k = 0; k < kMaxP; k++) L.beta[k] = k < p ? beta[k] : 0.0;
     L.tau2 = th[1];
     L.n = n;
@@ -874,6 +893,7 @@ extern "C" int nngp_sNNGP(
     A.n = n;
     A.p = p;
     A.ws = md.zs;
+    A.nph = nph;
     hipLaunchKernelGGL(k_resid_init, dim3((n + kSweepGrid - 1) / kSweepGrid),
                        dim3(kSweepGrid), 0, st, A);
     NNGP_HIP(hipGetLastError());
