@@ -172,6 +172,28 @@ int nngp_Q(const double *B, const double *F, const double *u,
 int nngp_prior_draw(const double *B, const double *F, const int *nnIndx,
                     const int *nnIndxLU, int n, const double *z, double *w);
 
+/* sNNGPPredict — replaces src/sNNGPPredict.cpp:25-241 (.Call at
+ * R/generics.R:671-672), predictions from a latent fit.  For every new
+ * site i and posterior sample s, with the m fitted neighbors N of i:
+ * C = sigma.sq R(N,N) (no nugget), c = sigma.sq R(N,i), u = C^-1 c,
+ *   pW0[s*q + i] = sqrt(sigma.sq - u.c) z[i*S + s] + u . w_s(N),
+ *   pY0[s*q + i] = sqrt(tau.sq) z[q*S + i*S + s] + x0_i . beta_s + pW0[s*q+i]
+ * with w_s = wSamples[s*n ..] (n x nSamples).  betaSamples p x S,
+ * thetaSamples nTheta x S ({sigma.sq, tau.sq, phi[, nu]}), nnIndx0 q x m.
+ * z: the 2 q S normals in the reference's order (all of w0's, then all of
+ * y0's), or NULL to draw them from rng (or the built-in generator).
+ * family 1 (Gaussian) only; y is accepted for signature parity.  A 1-thread
+ * reference run assigns the normals in this order; with more threads the
+ * reference's assignment depends on scheduling (src/sNNGPPredict.cpp:185). */
+int nngp_sNNGPPredict(const double *X, const double *y, const double *coords,
+                      int n, int p, int m, const double *X0,
+                      const double *coords0, int q, const int *nnIndx0,
+                      const double *betaSamples, const double *thetaSamples,
+                      const double *wSamples, int nSamples, int family,
+                      int covModel, int nThreads, int verbose, int nReport,
+                      const double *z, const nngp_rng *rng,
+                      const nngp_opts *opts, double *pY0, double *pW0);
+
 /* rNNGPReplicated — replaces src/rNNGPReplicated.cpp:78-230 (.Call at
  * R/generics.R:147, fitted() of a conjugate fit).  For each sample s: B and
  * F of theta_s (C = sigma.sq R + tau.sq I), the draw

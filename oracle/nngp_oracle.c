@@ -1450,3 +1450,63 @@ done:
   free(beta);
   return st;
 }
+
+/* src/sNNGPPredict.cpp:25-241, Gaussian family, in the single-thread order:
+ * normal index i*S + s for site i and sample s, w0's normals first, then
+ * y0's (z holds 2 q S).  Without a nugget: C = s2 R(N,N), c = s2 R(N,i). */
+int or_sNNGPPredict(const double *X0, const double *coords, int n, int p,
+                    int m, const double *coords0, int q, const int *nnIndx0,
+                    const double *beta, const double *theta, const double *w,
+                    int nSamples, int covModel, const double *z, double *y0,
+                    double *w0) {
+  if (covModel < 0 || covModel > 3)
+    return fail(OR_ERR_COV_MODEL,
+                "c++ Rf_error: cov.model is not correctly specified");
+  const int nTheta = covModel == 2 ? 4 : 3;
+  double *C = (double *)malloc(sizeof(double) * (size_t)m * m);
+  double *c = (double *)malloc(sizeof(double) * (size_t)m);
+  double *u = (double *)malloc(sizeof(double) * (size_t)m);
+  if (!C || !c || !u) {
+    free(C);
+    free(c);
+    free(u);
+    return fail(OR_ERR_ARG, "allocation failed");
+  }
+  const size_t qs = (size_t)q * nSamples;
+  int st = OR_OK;
+  for (int i = 0; i < q && st == OR_OK; i++) {
+    for (int s = 0; s < nSamples; s++) {
+      const double *th = theta + (size_t)s * nTheta;
+      const double s2 = th[0], t2 = th[1], phi = th[2];
+      const double nu = covModel == 2 ? th[3] : 0.0;
+      for (int k = 0; k < m; k++) {
+        const int a = nnIndx0[i + (size_t)q * k];
+        double d = or_dist2(coords[a], coords[n + a], coords0[i],
+                            coords0[q + i]);
+        c[k] = s2 * or_spCor(d, phi, nu, covModel);
+        for (int l = 0; l < m; l++) {
+          const int b = nnIndx0[i + (size_t)q * l];
+          d = or_dist2(coords[a], coords[n + a], coords[b], coords[n + b]);
+          C[l * m + k] = s2 * or_spCor(d, phi, nu, covModel);
+        }
+      }
+      if (or_potrf_L(m, C) || or_potri_L(m, C)) {
+        st = fail(OR_ERR_NOT_PD, "c++ Rf_error: dpotrf failed\n");
+        break;
+      }
+      or_symv_L(m, C, c, u);
+      double d = 0.0;
+      for (int k = 0; k < m; k++)
+        d += u[k] * w[(size_t)s * n + nnIndx0[i + (size_t)q * k]];
+      const size_t zi = (size_t)i * nSamples + s;
+      const double wv = sqrt(s2 - ddot(m, u, 1, c, 1)) * z[zi] + d;
+      w0[(size_t)s * q + i] = wv;
+      y0[(size_t)s * q + i] = sqrt(t2) * z[qs + zi] +
+                              ddot(p, &X0[i], q, &beta[(size_t)s * p], 1) + wv;
+    }
+  }
+  free(C);
+  free(c);
+  free(u);
+  return st;
+}

@@ -31,6 +31,8 @@
  *                     numpy transpose; every (child, slot) names the site
  *   or_rgamma         Kolmogorov-Smirnov against scipy's gamma CDF over
  *                     shapes in every branch of algorithm GD
+ *   or_sNNGPPredict   dense GP conditional of w at m = n (mean from z = 0,
+ *                     sd from z = 1) and the tau.sq noise term
  *   or_sNNGP          the w update equals the full conditional of w_i from
  *                     the dense precision (I-B)^T F^-1 (I-B) + I / tau.sq;
  *                     a chain under a permuted visit order equals the
@@ -196,6 +198,16 @@ int or_sNNGP(const double *y, const double *X, int p, int n, int m,
              double nuTuning, int nSamples, int nThreads, const int *visit,
              const or_rng *rng, double *pBetaSamples, double *pThetaSamples,
              double *pWSamples, int *nAccept);
+
+/* src/sNNGPPredict.cpp:25-241 (Gaussian family, single-thread normal
+ * order): w0[s*q+i] = sqrt(s2 - u.c) z[i*S+s] + u . w_s(N),
+ * y0[s*q+i] = sqrt(t2) z[q*S + i*S+s] + x0_i . beta_s + w0[s*q+i],
+ * u = (s2 R(N,N))^-1 s2 R(N,i).  w n x S, beta p x S, theta nTheta x S. */
+int or_sNNGPPredict(const double *X0, const double *coords, int n, int p,
+                    int m, const double *coords0, int q, const int *nnIndx0,
+                    const double *beta, const double *theta, const double *w,
+                    int nSamples, int covModel, const double *z, double *y0,
+                    double *w0);
 
 #ifdef __cplusplus
 }

@@ -89,6 +89,9 @@ def lib():
                                        C.c_uint]
         L.or_philox_normal.restype = C.c_double
         L.or_philox_normal.argtypes = [C.c_ulonglong, C.c_uint, C.c_uint]
+        L.or_sNNGPPredict.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_int,
+                                      _dp, C.c_int, _ip, _dp, _dp, _dp,
+                                      C.c_int, C.c_int, _dp, _dp, _dp]
         L.or_mkUIndx.argtypes = [C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip,
                                  C.c_int]
         L.or_rgamma.restype = C.c_double
@@ -410,3 +413,17 @@ def sNNGP(y, X, p, n, m, coords_flat, covModel, nnIndx, nnIndxLU, uIndx,
             "p.theta.samples": theta.reshape(nSamples, nTheta).T.copy(),
             "p.w.samples": w[:n * nSamples].reshape(nSamples, n).T.copy(),
             "accept": acc.value}
+
+
+def sNNGPPredict(X0_flat, coords_flat, n, p, m, coords0_flat, q, nnIndx0,
+                 beta, theta, w, nSamples, covModel, z):
+    """src/sNNGPPredict.cpp:25-241.  Flat column-major inputs; nnIndx0
+    q x m column-major flat; beta p x S, theta nTheta x S, w n x S flat;
+    z 2 q S normals.  Returns (p.y.0, p.w.0), each q x S."""
+    y0 = np.zeros(q * nSamples)
+    w0 = np.zeros(q * nSamples)
+    _check(lib().or_sNNGPPredict(_f64(X0_flat), _f64(coords_flat), n, p, m,
+                                 _f64(coords0_flat), q, _i32(nnIndx0),
+                                 _f64(beta), _f64(theta), _f64(w), nSamples,
+                                 covModel, _f64(z), y0, w0))
+    return (y0.reshape(nSamples, q).T.copy(), w0.reshape(nSamples, q).T.copy())

@@ -7,7 +7,8 @@ reference's native routines:
     mkNNIndxCB, mkNNIndx, nn2          neighbor search
     mkUIndx                            sites holding each site as a neighbor
     updateBF, updateConjBF, Q          per-site systems and quadratic forms
-    rNNGP, sNNGP, cNNGP, rNNGPPredict  the .Call drivers
+    rNNGP, sNNGP, cNNGP, rNNGPPredict,
+    sNNGPPredict                       the .Call drivers
     Model                              device-resident fitted data
     Server                             resident kriging service
     CvGrid                             conjugate k-fold grid search
@@ -22,8 +23,8 @@ from ._lib import (NNGP_ERR_ARG, NNGP_ERR_COV_MODEL, NNGP_ERR_HIP,
                    NngpError, exported_symbols, load)
 from .nngp import (COV_MODELS, CvGrid, Model, Q, RRng, Server, cNNGP, cov_index, device_count,
                    mkNNIndx, mkNNIndxCB, mkUIndx, n_indx, nn2, prior_draw,
-                   rNNGP, rNNGPPredict, rNNGPReplicated, sNNGP, updateBF,
-                   updateConjBF)
+                   rNNGP, rNNGPPredict, rNNGPReplicated, sNNGP, sNNGPPredict,
+                   updateBF, updateConjBF)
 from .spnngp import (crps, fitted, load_fit, predict, rmspe, save_fit, serve,
                      spConjNNGP, spNNGP)
 
@@ -31,7 +32,7 @@ __all__ = [
     "COV_MODELS", "CvGrid", "Model", "NngpError", "Q", "RRng", "Server", "cNNGP", "cov_index",
     "device_count", "exported_symbols", "load", "mkNNIndx", "mkNNIndxCB",
     "n_indx", "nn2", "prior_draw", "rNNGP", "rNNGPPredict", "rNNGPReplicated",
-    "mkUIndx", "sNNGP", "updateBF",
+    "mkUIndx", "sNNGP", "sNNGPPredict", "updateBF",
     "updateConjBF", "spNNGP", "spConjNNGP", "predict", "fitted", "crps",
     "rmspe", "save_fit", "load_fit", "serve",
     "NNGP_OK", "NNGP_ERR_ARG", "NNGP_ERR_NOT_PD", "NNGP_ERR_COV_MODEL",

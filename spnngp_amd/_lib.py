@@ -42,6 +42,7 @@ EXPORTS = (
     "nngp_cvgrid_create", "nngp_cvgrid_run", "nngp_cvgrid_destroy",
     "nngp_model_autotune", "nngp_model_set_shape", "nngp_rNNGPReplicated",
     "nngp_rng_r_create", "nngp_rng_r_destroy", "nngp_mkUIndx", "nngp_sNNGP",
+    "nngp_sNNGPPredict",
 )
 
 _RNG_FN = C.CFUNCTYPE(C.c_double, C.c_void_p)
@@ -177,6 +178,10 @@ def load():
     L.nngp_rng_r_create.argtypes = [C.c_uint, C.POINTER(NngpRng)]
     L.nngp_rng_r_destroy.argtypes = [C.POINTER(NngpRng)]
     L.nngp_rng_r_destroy.restype = None
+    L.nngp_sNNGPPredict.argtypes = [_dp, _vp, _dp, _i, _i, _i, _dp, _dp, _i,
+                                    _ip, _dp, _dp, _dp, _i, _i, _i, _i, _i,
+                                    _i, _vp, C.POINTER(NngpRng),
+                                    C.POINTER(NngpOpts), _dp, _dp]
     L.nngp_mkUIndx.argtypes = [_i, _i, _ip, _ip, _ip, _ip, _ip, _i]
     L.nngp_sNNGP.argtypes = (
         [_dp, _dp, _i, _i, _i, _dp, _i, _ip, _ip, _ip, _ip, _ip, _dp, _dp,

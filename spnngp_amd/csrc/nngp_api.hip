@@ -757,6 +757,135 @@ int nngp_rNNGPPredict(const double *X, const double *y, const double *coords,
 }
 
 // ---------------------------------------------------------------------------
+// sNNGPPredict (src/sNNGPPredict.cpp:25-241), Gaussian family
+// ---------------------------------------------------------------------------
+int nngp_sNNGPPredict(const double *X, const double *y, const double *coords,
+                      int n, int p, int m, const double *X0,
+                      const double *coords0, int q, const int *nnIndx0,
+                      const double *betaSamples, const double *thetaSamples,
+                      const double *wSamples, int nSamples, int family,
+                      int covModel, int nThreads, int verbose, int nReport,
+                      const double *z, const nngp_rng *rng,
+                      const nngp_opts *opts, double *pY0, double *pW0) {
+  (void)nThreads;
+  (void)nReport;
+  (void)y;  // the latent predictor conditions on w, not on y
+  clear_err();
+  NNGP_TRY(check_cov(covModel));
+  if (family != 1)
+    return set_err(NNGP_ERR_UNSUPPORTED,
+                   "family %d: only the Gaussian family (1) is supported",
+                   family);
+  if (n < 1 || m < 1 || m > n)
+    return set_err(NNGP_ERR_ARG, "need 1 <= n.neighbors <= n");
+  if (m > 64) return set_err(NNGP_ERR_UNSUPPORTED, "n.neighbors = %d exceeds 64", m);
+  if (p < 1 || p > kMaxP)
+    return set_err(NNGP_ERR_UNSUPPORTED, "p = %d covariates (1..%d supported)",
+                   p, kMaxP);
+  if (q < 0 || nSamples < 0) return set_err(NNGP_ERR_ARG, "negative size");
+  if (q == 0 || nSamples == 0) return NNGP_OK;
+  if (!X || !coords || !X0 || !coords0 || !nnIndx0 || !betaSamples ||
+      !thetaSamples || !wSamples || !pY0 || !pW0)
+    return set_err(NNGP_ERR_ARG, "NULL buffer");
+  for (long long k = 0; k < (long long)q * m; k++)
+    if (nnIndx0[k] < 0 || nnIndx0[k] >= n)
+      return set_err(NNGP_ERR_ARG, "nnIndx0 entry %lld outside [0, n)", k);
+  NNGP_TRY(use_device(opts));
+  const int nTheta = covModel == 2 ? 4 : 3;
+  if (verbose) {
+    printf("-------------------------------------------------\n");
+    printf("\t\tPredicting\n");
+    printf("-------------------------------------------------\n");
+    fflush(stdout);
+  }
+  // the reference draws all q*S normals of w0, then all q*S of y0, each
+  // site-major (z[i*S + s], the single-thread order of :135-147, 192)
+  const size_t qs = (size_t)q * nSamples;
+  std::vector<double> zown;
+  if (!z) {
+    Rng R(rng, opts ? opts->seed : 1ull);
+    zown.resize(2 * qs);
+    for (auto &v : zown) v = R.rnorm(0.0, 1.0);
+    z = zown.data();
+  }
+  Stream st;
+  NNGP_TRY(st.create());
+  DevBuf dxy, dX, dxy0, dX0, dnn0, dbeta, dtheta, dw, dz, dy0, dw0, dcov, derr;
+  NNGP_TRY(upload_xy(coords, n, dxy, st.s));
+  NNGP_TRY(upload_xy(coords0, q, dxy0, st.s));
+  NNGP_TRY(dX.alloc(sizeof(double) * (size_t)n * p));
+  NNGP_TRY(dX0.alloc(sizeof(double) * (size_t)q * p));
+  NNGP_TRY(dnn0.alloc(sizeof(int) * (size_t)q * m));
+  NNGP_TRY(dbeta.alloc(sizeof(double) * (size_t)p * nSamples));
+  NNGP_TRY(dtheta.alloc(sizeof(double) * (size_t)nTheta * nSamples));
+  NNGP_TRY(dw.alloc(sizeof(double) * (size_t)n * nSamples));
+  NNGP_TRY(dz.alloc(sizeof(double) * 2 * qs));
+  NNGP_TRY(dy0.alloc(sizeof(double) * qs));
+  NNGP_TRY(dw0.alloc(sizeof(double) * qs));
+  NNGP_TRY(derr.alloc(sizeof(int)));
+  NNGP_HIP(hipMemcpyAsync(dX.p, X, dX.bytes, hipMemcpyHostToDevice, st.s));
+  NNGP_HIP(hipMemcpyAsync(dX0.p, X0, dX0.bytes, hipMemcpyHostToDevice, st.s));
+  NNGP_HIP(hipMemcpyAsync(dnn0.p, nnIndx0, dnn0.bytes, hipMemcpyHostToDevice, st.s));
+  NNGP_HIP(hipMemcpyAsync(dbeta.p, betaSamples, dbeta.bytes, hipMemcpyHostToDevice, st.s));
+  NNGP_HIP(hipMemcpyAsync(dtheta.p, thetaSamples, dtheta.bytes, hipMemcpyHostToDevice, st.s));
+  NNGP_HIP(hipMemcpyAsync(dw.p, wSamples, dw.bytes, hipMemcpyHostToDevice, st.s));
+  NNGP_HIP(hipMemcpyAsync(dz.p, z, dz.bytes, hipMemcpyHostToDevice, st.s));
+  std::vector<CovArgs> hc;
+  const bool generic = covModel == 2 || m > 16;
+  if (generic) {
+    hc.resize(nSamples);
+    for (int s = 0; s < nSamples; s++) {
+      const double *th = thetaSamples + (size_t)s * nTheta;
+      hc[s] = make_cov(covModel, th[0], th[1], th[2], covModel == 2 ? th[3] : 0.0);
+    }
+    NNGP_TRY(dcov.alloc(sizeof(CovArgs) * hc.size()));
+    NNGP_HIP(hipMemcpyAsync(dcov.p, hc.data(), dcov.bytes, hipMemcpyHostToDevice, st.s));
+  }
+  if (fill_int_launch(derr.as<int>(), INT_MAX, st.s))
+    return set_err(NNGP_ERR_HIP, "flag reset kernel failed");
+  PredArgs a;
+  a.xy = dxy.as<double2>();
+  a.X = dX.as<double>();
+  a.y = nullptr;
+  a.n = n;
+  a.p = p;
+  a.m = m;
+  a.xy0 = dxy0.as<double2>();
+  a.X0 = dX0.as<double>();
+  a.q = q;
+  a.nn0 = dnn0.as<int>();
+  a.beta = dbeta.as<double>();
+  a.theta = dtheta.as<double>();
+  a.S = nSamples;
+  a.nTheta = nTheta;
+  a.cov = covModel;
+  a.z = dz.as<double>();
+  a.y0 = dy0.as<double>();
+  a.err = derr.as<int>();
+  a.unit = make_cov(covModel, 1.0, 0.0, 1.0, 0.0);
+  a.cov_s = generic ? dcov.as<CovArgs>() : nullptr;
+  a.wS = dw.as<double>();
+  a.z2 = dz.as<double>() + qs;
+  a.w0out = dw0.as<double>();
+  const int chunk = 65535;
+  for (int s0 = 0; s0 < q; s0 += chunk) {
+    a.site0 = s0;
+    a.nsite = std::min(chunk, q - s0);
+    if (pred_launch(a, st.s)) return set_err(NNGP_ERR_HIP, "kriging launch failed");
+  }
+  int bad = INT_MAX;
+  NNGP_HIP(hipMemcpyAsync(pY0, dy0.p, dy0.bytes, hipMemcpyDeviceToHost, st.s));
+  NNGP_HIP(hipMemcpyAsync(pW0, dw0.p, dw0.bytes, hipMemcpyDeviceToHost, st.s));
+  NNGP_HIP(hipMemcpyAsync(&bad, derr.p, sizeof(int), hipMemcpyDeviceToHost, st.s));
+  NNGP_HIP(hipStreamSynchronize(st.s));
+  if (bad != INT_MAX)
+    return set_err(NNGP_ERR_NOT_PD,
+                   "c++ Rf_error: dpotrf failed\n(kriging system of new site %d"
+                   " is not positive definite)", bad);
+  return NNGP_OK;
+}
+
+// ---------------------------------------------------------------------------
 // device-resident model
 // ---------------------------------------------------------------------------
 struct nngp_model {

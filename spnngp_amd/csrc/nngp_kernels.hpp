@@ -196,6 +196,14 @@ struct PredArgs {
   // above are then unused and z comes from each run's seed.
   const PredSeg *segs = nullptr;
   int nseg = 0;
+  // Latent form (src/sNNGPPredict.cpp:163-205) when wS != nullptr: C has no
+  // nugget, the weights apply to the posterior w samples wS (n x S,
+  // column s = sample s), w0 = d + sqrt(s2 - w.c) z and
+  // y0 = sqrt(t2) z2 + x0.beta + w0 with a second set of q*S normals z2
+  // (z2[i*S + s]); w0 goes to w0out (q x S).  Unbatched launches only.
+  const double *wS = nullptr;
+  const double *z2 = nullptr;
+  double *w0out = nullptr;
 };
 int pred_launch(const PredArgs &a, hipStream_t st);
 

@@ -9,6 +9,11 @@
 // its neighbors' coordinates, pairwise distances, X rows and y in LDS once,
 // and each lane solves the system of one sample.
 //
+// Latent model, src/sNNGPPredict.cpp:163-205: the same systems without the
+// nugget, weights applied to the posterior samples of w at the neighbors,
+//   w0[s*q+i] = sum_k w_k wS[s*n + N_k] + sqrt(s2_s - w.c) z[i*S+s],
+//   y0[s*q+i] = sqrt(t2_s) z2[i*S+s] + x0_i.beta_s + w0[s*q+i].
+//
 // Conjugate model, src/cNNGP.cpp:255-299: one system per held-out site
 // (one lane each) for the grid point's (phi, alpha):
 //   y0 = x0.beta + w.(y_N - X_N beta),  u = x0 - X_N^T w,
@@ -99,18 +104,20 @@ __global__ __launch_bounds__(kPredBlock) void k_pred_lane(PredArgs A) {
   __shared__ double sX[M * kPredMaxP];        // X rows of the neighbors
   __shared__ double sX0[kPredMaxP];
   __shared__ double sPx[M], sPy[M];
+  __shared__ int sId[M];
   const SiteRef R = pred_site(A, (int)blockIdx.x);
   const int i = R.i;
   const int t = (int)threadIdx.x;
   const int m = A.m;
   if (t < M) {
     double px = 0.0, py = 0.0, yy = 0.0;
+    int id = 0;
     if (t < m) {
-      const int id = R.nn0[(size_t)t * R.q + i];
+      id = R.nn0[(size_t)t * R.q + i];
       const double2 pt = A.xy[id];
       px = pt.x;
       py = pt.y;
-      yy = A.y[id];
+      yy = A.y ? A.y[id] : 0.0;  // no y in the latent form
       for (int j = 0; j < A.p; j++)
         sX[t * kPredMaxP + j] = A.X[(size_t)j * A.n + id];
     } else {
@@ -119,6 +126,7 @@ __global__ __launch_bounds__(kPredBlock) void k_pred_lane(PredArgs A) {
     sPx[t] = px;
     sPy[t] = py;
     sY[t] = yy;
+    sId[t] = id;
   }
   if (t < A.p) sX0[t] = R.X0[(size_t)t * R.q + i];
   __syncthreads();
@@ -137,6 +145,8 @@ __global__ __launch_bounds__(kPredBlock) void k_pred_lane(PredArgs A) {
   const double *th = A.theta + (size_t)s * A.nTheta;
   const double s2 = th[0], t2 = th[1], phi = th[2];
   const double ip = 1.0 / phi;
+  const bool latent = A.wS != nullptr;
+  const double nug = latent ? 0.0 : t2;
   constexpr int T = M * (M + 1) / 2;
   double a[T], v[M];
 #pragma unroll
@@ -146,7 +156,7 @@ __global__ __launch_bounds__(kPredBlock) void k_pred_lane(PredArgs A) {
     for (int l = 0; l < k; l++)
       a[tri(k, l)] =
           s2 * corr_d<COV>(sD[k * (k - 1) / 2 + l], phi, ip, A.unit);
-    a[tri(k, k)] = s2 * 1.0 + t2;
+    a[tri(k, k)] = s2 * 1.0 + nug;
   }
   double vv;
   const bool ok = chol_solve<M>(a, v, vv);  // v <- w
@@ -154,14 +164,24 @@ __global__ __launch_bounds__(kPredBlock) void k_pred_lane(PredArgs A) {
   double x0b = 0.0;
   for (int j = 0; j < A.p; j++) x0b += sX0[j] * bs[j];
   double d = 0.0;
-#pragma unroll
-  for (int k = 0; k < M; k++) {
-    double xb = 0.0;
-    for (int j = 0; j < A.p; j++) xb += sX[k * kPredMaxP + j] * bs[j];
-    d += v[k] * (sY[k] - xb);
-  }
   const double zz = pred_z(R, A.S, s);
-  R.y0[(size_t)s * R.q + i] = x0b + d + sqrt(s2 + t2 - vv) * zz;
+  if (latent) {  // padding rows have weight exactly 0 and id 0
+    const double *ws = A.wS + (size_t)s * A.n;
+#pragma unroll
+    for (int k = 0; k < M; k++) d += v[k] * ws[sId[k]];
+    const double w0 = sqrt(s2 - vv) * zz + d;
+    A.w0out[(size_t)s * R.q + i] = w0;
+    R.y0[(size_t)s * R.q + i] =
+        sqrt(t2) * A.z2[(size_t)i * A.S + s] + x0b + w0;
+  } else {
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+      double xb = 0.0;
+      for (int j = 0; j < A.p; j++) xb += sX[k * kPredMaxP + j] * bs[j];
+      d += v[k] * (sY[k] - xb);
+    }
+    R.y0[(size_t)s * R.q + i] = x0b + d + sqrt(s2 + t2 - vv) * zz;
+  }
   if (!ok) atomicMin(R.err, i);
 }
 
@@ -193,6 +213,7 @@ __global__ __launch_bounds__(64 * kPredCoopWaves) void k_pred_coop(
   const double *th = A.theta + (size_t)s * A.nTheta;
   const double s2 = th[0], t2 = th[1];
   const CovArgs &c = A.cov_s[s];
+  const bool latent = A.wS != nullptr;
   int id = 0;
   if (t < m) {
     id = R.nn0[(size_t)t * R.q + i];
@@ -208,7 +229,7 @@ __global__ __launch_bounds__(64 * kPredCoopWaves) void k_pred_coop(
     while ((k + 1) * (k + 2) / 2 <= p) k++;
     while (k * (k + 1) / 2 > p) k--;
     const int l = p - k * (k + 1) / 2;
-    Am[k * ld + l] = k == l ? c.diag
+    Am[k * ld + l] = k == l ? (latent ? c.s2 : c.diag)
                             : s2_corr_rt(sqdist(px[k], py[k], px[l], py[l]), c);
   }
   const double c0 =
@@ -223,16 +244,27 @@ __global__ __launch_bounds__(64 * kPredCoopWaves) void k_pred_coop(
   const double *bs = A.beta + (size_t)s * A.p;
   double part = 0.0;
   if (t < m) {
-    double xb = 0.0;
-    for (int j = 0; j < A.p; j++) xb += A.X[(size_t)j * A.n + id] * bs[j];
-    part = x * (A.y[id] - xb);
+    if (latent) {
+      part = x * A.wS[(size_t)s * A.n + id];
+    } else {
+      double xb = 0.0;
+      for (int j = 0; j < A.p; j++) xb += A.X[(size_t)j * A.n + id] * bs[j];
+      part = x * (A.y[id] - xb);
+    }
   }
   part = wave_sum(part);
   if (t == 0) {
     double x0b = 0.0;
     for (int j = 0; j < A.p; j++) x0b += R.X0[(size_t)j * R.q + i] * bs[j];
-    R.y0[(size_t)s * R.q + i] =
-        x0b + part + sqrt(s2 + t2 - vv) * pred_z(R, A.S, s);
+    if (latent) {
+      const double w0 = sqrt(s2 - vv) * pred_z(R, A.S, s) + part;
+      A.w0out[(size_t)s * R.q + i] = w0;
+      R.y0[(size_t)s * R.q + i] =
+          sqrt(t2) * A.z2[(size_t)i * A.S + s] + x0b + w0;
+    } else {
+      R.y0[(size_t)s * R.q + i] =
+          x0b + part + sqrt(s2 + t2 - vv) * pred_z(R, A.S, s);
+    }
   }
 }
 

@@ -344,6 +344,29 @@ def rNNGPPredict(X, y, coords, n, p, m, X0, coords0, q, nnIndx0,
     return {"p.y.0": out.reshape(nSamples, q).T.copy()}
 
 
+def sNNGPPredict(X, y, coords, n, p, m, X0, coords0, q, nnIndx0,
+                 betaSamples, thetaSamples, wSamples, nSamples, covModel,
+                 family=1, nThreads=1, verbose=0, nReport=100, z=None,
+                 rng=None, seed=1):
+    """sNNGPPredict (src/sNNGPPredict.cpp:25-241), .Call order of
+    R/generics.R:671.  wSamples n x S (the latent fit's p.w.samples);
+    z: 2*q*S normals (w0's then y0's, each z[i*S + s]) or None."""
+    out = np.zeros(q * nSamples)
+    w0 = np.zeros(q * nSamples)
+    opts = _opts(0, 0, seed)
+    nn0 = np.asarray(nnIndx0)
+    nn0 = _i32(nn0.T.reshape(-1)) if nn0.ndim == 2 else _i32(nn0)
+    zz = None if z is None else _f64(z)
+    check(load().nngp_sNNGPPredict(
+        _cm(X), None, _cm(coords), n, p, m, _cm(X0), _cm(coords0), q, nn0,
+        _cm(betaSamples), _cm(thetaSamples), _cm(wSamples), nSamples,
+        int(family), cov_index(covModel), nThreads, verbose, nReport,
+        None if zz is None else zz.ctypes.data_as(C.c_void_p),
+        _rng_struct(rng), C.byref(opts), out, w0))
+    return {"p.y.0": out.reshape(nSamples, q).T.copy(),
+            "p.w.0": w0.reshape(nSamples, q).T.copy()}
+
+
 def rNNGPReplicated(X, p, n, m, coords, covModel, nnIndx, nnIndxLU, beta,
                     theta, nSamples, nThreads=1, verbose=0, nReport=100,
                     rng=None, seed=1):

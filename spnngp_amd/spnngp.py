@@ -18,8 +18,8 @@ Differences from R that a caller sees:
     copy of R's default generator, gives the stream of R's set.seed(s).
     Given the same draws the results are the reference's.
   * method = "response" and "latent" with family = "gaussian" are on this
-    path (the binomial and SLGP models are not).  predict() of a latent
-    fit is not (the reference's sNNGPPredict).
+    path, with predict() and fitted() for both (the binomial and SLGP
+    models are not).
 """
 from __future__ import annotations
 
@@ -633,7 +633,8 @@ def predict(obj, X_0, coords_0, *, sub_sample=None, n_omp_threads=1,
             verbose=False, n_report=100, rng=None, seed=1, server=None,
             request_seed=None):
     """predict.NNGP for a response fit (posterior predictive samples p.y.0,
-    q x n.samples) or a conjugate fit (y.0.hat, y.0.hat.var).
+    q x n.samples), a latent fit (p.y.0 and the latent p.w.0) or a
+    conjugate fit (y.0.hat, y.0.hat.var).
 
     For a response fit, `server` (a spnngp_amd.Server built on the same
     fitted model) answers the request from HBM instead, with the counter-
@@ -648,9 +649,6 @@ def predict(obj, X_0, coords_0, *, sub_sample=None, n_omp_threads=1,
         raise ValueError("error: coords.0 must have two columns\n")
     q = X_0.shape[0]
     m = obj["n.neighbors"]
-    if obj["type"][0] == "latent":
-        raise NotImplementedError("predict() of a latent fit (the reference's "
-                                  "sNNGPPredict) is not on this path")
     if obj["type"][0] == "conjugate":
         ta = obj["theta.alpha"]
         r = spConjNNGP(y, X, coords, theta_alpha={k: [v] for k, v in
@@ -667,6 +665,18 @@ def predict(obj, X_0, coords_0, *, sub_sample=None, n_omp_threads=1,
     theta = obj["p.theta.samples"][s_indx - 1].T
     S = beta.shape[1]
     t0 = time.perf_counter()
+    if obj["type"][0] == "latent":
+        # src/sNNGPPredict.cpp via R/generics.R:600-672: the kept w samples
+        # at the fitted sites condition the new ones
+        nn0, _ = _nn.nn2(coords, coords_0, m)
+        w = np.asarray(obj["p.w.samples"])[:, s_indx - 1]
+        out = _nn.sNNGPPredict(X, y, coords, n, p, m, X_0, coords_0, q, nn0,
+                               beta, theta, w, S, obj["cov.model"],
+                               nThreads=n_omp_threads, verbose=int(verbose),
+                               nReport=n_report, rng=rng, seed=seed)
+        return {"p.y.0": out["p.y.0"], "p.w.0": out["p.w.0"],
+                "run.time": time.perf_counter() - t0,
+                "sub.sample": sub_sample}
     if server is not None:
         out = server.predict(X_0, coords_0,
                              seed if request_seed is None else request_seed)

@@ -192,3 +192,79 @@ def test_spNNGP_latent_front_end_follows_the_oracle(oracle):
     np.testing.assert_allclose(fit["y.hat.samples"], yh, rtol=1e-12)
     f2 = nn.fitted(fit, {"start": 21, "thin": 1})
     assert f2["y.hat.samples"].shape == (n, S - 20)
+
+
+@pytest.mark.parametrize("cov,m", [(0, 10), (1, 15), (3, 8), (2, 10),
+                                   (0, 24)])
+def test_sNNGPPredict_matches_oracle(oracle, cov, m):
+    """nngp_sNNGPPredict (src/sNNGPPredict.cpp:25-241) against the oracle
+    fed the same 2 q S normals: the lane kernel (m <= 16, closed-form
+    covariances) and the wavefront kernel (Matern, m > 16)."""
+    import spnngp_amd as nn
+    rng = np.random.default_rng(100 + cov + m)
+    n, q, S, p = 3000, 257, 37, 2
+    c = rng.random((n, 2))
+    c0 = rng.random((q, 2))
+    X = np.column_stack([np.ones(n), rng.standard_normal(n)])
+    X0 = np.column_stack([np.ones(q), rng.standard_normal(q)])
+    d2 = ((c0[:, None, :] - c[None, :, :]) ** 2).sum(-1)
+    nn0 = np.argsort(d2, axis=1, kind="stable")[:, :m].astype(np.int32)
+    beta = rng.standard_normal((p, S))
+    # no nugget in C: the smooth covariances (gaussian, matern with larger
+    # nu) need a short range for the neighbor systems to be conditioned
+    # well enough for 1e-10 agreement between two implementations
+    phi_lo, phi_hi = {3: (60.0, 120.0), 2: (20.0, 40.0)}.get(cov, (5.0, 20.0))
+    theta = np.vstack([rng.uniform(0.5, 2.0, S), rng.uniform(0.1, 0.5, S),
+                       rng.uniform(phi_lo, phi_hi, S)]
+                      + ([rng.uniform(0.4, 1.0, S)] if cov == 2 else []))
+    w = rng.standard_normal((n, S))
+    z = rng.standard_normal(2 * q * S)
+    print(f"[sNNGPPredict] cov={cov} m={m} q={q} S={S}", flush=True)
+    got = nn.sNNGPPredict(X, None, c, n, p, m, X0, c0, q, nn0, beta, theta,
+                          w, S, cov, z=z)
+    y0, w0 = oracle.sNNGPPredict(X0.T.reshape(-1).copy(),
+                                 c.T.reshape(-1).copy(), n, p, m,
+                                 c0.T.reshape(-1).copy(), q,
+                                 nn0.T.reshape(-1).copy(),
+                                 beta.T.reshape(-1).copy(),
+                                 theta.T.reshape(-1).copy(),
+                                 w.T.reshape(-1).copy(), S, cov, z)
+    np.testing.assert_allclose(got["p.w.0"], w0, rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(got["p.y.0"], y0, rtol=1e-10, atol=1e-10)
+
+
+def test_predict_of_a_latent_fit(oracle):
+    """predict() of spNNGP(method = "latent") (R/generics.R:583-675): nn2 on
+    the caller-order sites, the kept w samples, the oracle's sNNGPPredict
+    on the same inputs and normals."""
+    import spnngp_amd as nn
+    n, m, S = 300, 8, 12
+    rng = np.random.default_rng(23)
+    c = rng.random((n, 2))
+    X = np.column_stack([np.ones(n), rng.standard_normal(n)])
+    y = X @ np.array([1.0, 2.0]) + rng.standard_normal(n)
+    fit = nn.spNNGP(y, X, c, method="latent", n_neighbors=m,
+                    starting={"beta": [0.0, 0.0], "sigma.sq": 1.0,
+                              "tau.sq": 0.5, "phi": 5.0},
+                    tuning={"phi": 0.3},
+                    priors={"sigma.sq.IG": [2, 1], "tau.sq.IG": [2, 0.5],
+                            "phi.Unif": [1, 30]},
+                    n_samples=S, seed=3, sweep="coloured")
+    q = 40
+    c0 = rng.random((q, 2))
+    X0 = np.column_stack([np.ones(q), rng.standard_normal(q)])
+    zz = rng.standard_normal(2 * q * (S - 4))
+    tape = oracle.TapeRng(zz, [])
+    print(f"[predict latent] n={n} q={q}", flush=True)
+    out = nn.predict(fit, X0, c0, sub_sample={"start": 5}, rng=tape)
+    assert out["p.y.0"].shape == (q, S - 4) == out["p.w.0"].shape
+    nn0, _ = nn.nn2(c, c0, m)
+    keep = np.arange(4, S)
+    y0, w0 = oracle.sNNGPPredict(
+        X0.T.reshape(-1).copy(), fit["coords"].T.reshape(-1).copy(), n, 2, m,
+        c0.T.reshape(-1).copy(), q, nn0.T.reshape(-1).copy(),
+        fit["p.beta.samples"][keep].reshape(-1).copy(),
+        fit["p.theta.samples"][keep].reshape(-1).copy(),
+        fit["p.w.samples"][:, keep].T.reshape(-1).copy(), S - 4, 0, zz)
+    np.testing.assert_allclose(out["p.w.0"], w0, rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(out["p.y.0"], y0, rtol=1e-10, atol=1e-10)

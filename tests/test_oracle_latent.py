@@ -162,3 +162,57 @@ def test_sNNGP_posterior_recovers_truth(oracle):
     assert np.corrcoef(X @ bs.mean(1) + wm, X @ beta + w_true)[0, 1] > 0.95
     assert np.corrcoef(wm, w_true)[0, 1] > 0.8
     assert 0 < out["accept"] < S
+
+
+@pytest.mark.parametrize("cov", [0, 2, 3])
+def test_sNNGPPredict_is_the_dense_conditional_at_m_eq_n(oracle, cov):
+    """With every fitted site a neighbor, w0 = the GP conditional mean of w
+    at the new site given w at the fitted sites (z = 0), its sd is
+    sqrt(s2 - c^T C^-1 c) (z = 1), and y0 adds x0.beta + sqrt(t2) z2."""
+    rng = np.random.default_rng(9 + cov)
+    n, q, S, p = 12, 5, 3, 2
+    c = rng.random((n, 2))
+    c0 = rng.random((q, 2))
+    X0 = np.column_stack([np.ones(q), rng.standard_normal(q)])
+    nn0 = np.array([np.argsort(((c - c0[i]) ** 2).sum(1)) for i in range(q)],
+                   np.int32)  # q x n, all sites
+    beta = rng.standard_normal((p, S))
+    theta = np.vstack([[1.5, 2.0, 0.8], [0.3, 0.5, 0.2], [3.0, 5.0, 8.0]]
+                      + ([[0.7, 1.3, 2.1]] if cov == 2 else []))
+    w = rng.standard_normal((n, S))
+    from scipy import special
+    args = (X0.T.reshape(-1).copy(), c.T.reshape(-1).copy(), n, p, n,
+            c0.T.reshape(-1).copy(), q, nn0.T.reshape(-1).copy(),
+            beta.T.reshape(-1).copy(), theta.T.reshape(-1).copy(),
+            w.T.reshape(-1).copy(), S, cov)
+    z0 = np.zeros(2 * q * S)
+    y0a, w0a = oracle.sNNGPPredict(*args, z0)
+    z1 = np.concatenate([np.ones(q * S), np.full(q * S, 2.0)])
+    y0b, w0b = oracle.sNNGPPredict(*args, z1)
+
+    def rho(D, th):
+        phi = th[2]
+        if cov == 0:
+            return np.exp(-phi * D)
+        if cov == 3:
+            return np.exp(-(phi * D) ** 2)
+        nu = th[3]
+        x = np.where(D > 0, D * phi, 1.0)
+        r = x ** nu * special.kv(nu, x) / (2 ** (nu - 1) * special.gamma(nu))
+        return np.where(D > 0, r, 1.0)
+
+    D = np.sqrt(((c[:, None] - c[None]) ** 2).sum(-1))
+    for s in range(S):
+        th = theta[:, s]
+        Cd = th[0] * rho(D, th)
+        for i in range(q):
+            c_i = th[0] * rho(np.sqrt(((c - c0[i]) ** 2).sum(1)), th)
+            u = np.linalg.solve(Cd, c_i)
+            mean = u @ w[:, s]
+            sd = np.sqrt(th[0] - u @ c_i)
+            assert w0a[i, s] == pytest.approx(mean, rel=1e-8, abs=1e-9)
+            assert w0b[i, s] - w0a[i, s] == pytest.approx(sd, rel=1e-6)
+            xb = X0[i] @ beta[:, s]
+            assert y0a[i, s] == pytest.approx(xb + mean, rel=1e-8, abs=1e-9)
+            assert y0b[i, s] - w0b[i, s] - xb == pytest.approx(
+                2.0 * np.sqrt(th[1]), rel=1e-10)
