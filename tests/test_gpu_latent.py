@@ -268,3 +268,44 @@ def test_predict_of_a_latent_fit(oracle):
         fit["p.w.samples"][:, keep].T.reshape(-1).copy(), S - 4, 0, zz)
     np.testing.assert_allclose(out["p.w.0"], w0, rtol=1e-10, atol=1e-10)
     np.testing.assert_allclose(out["p.y.0"], y0, rtol=1e-10, atol=1e-10)
+
+
+def test_latent_edge_cases(oracle):
+    """The smallest layout (n = 2, m = 1), one covariate, no samples, and a
+    neighbor set that holds every predecessor (m = n - 1, the exact GP)."""
+    import spnngp_amd as nn
+    c = np.array([[0.1, 0.2], [0.4, 0.3]])
+    r = oracle.mkNNIndxCB(c, 1)
+    for t in (0, 1, 2):
+        got = nn.mkUIndx(2, 1, r["nnIndx"], r["nnIndxLU"], t)
+        want = oracle.mkUIndx(2, 1, r["nnIndx"], r["nnIndxLU"], t)
+        for k in want:
+            assert np.array_equal(got[k], want[k])
+    n = 40
+    P = _problem(n, n - 1, seed=8, p=1)
+    out = nn.sNNGP(*_args(P, 0))
+    assert out["p.w.samples"].shape == (n, 0)
+    S = 15
+    rng = np.random.default_rng(4)
+    normals = rng.standard_normal(S * (n + 12))
+    unifs = rng.random(S * 8)
+    print(f"[sNNGP exact GP] n={n} m={n - 1}", flush=True)
+    for sweep in ("site", "coloured"):
+        visit = None if sweep == "site" else _colour_order(P)
+        a_or = list(_args(P, S))
+        a_or[5] = P["c"].T.reshape(-1).copy()
+        want = oracle.sNNGP(*a_or[:21], a_or[23], a_or[24], S,
+                            oracle.TapeRng(normals, unifs), visit=visit)
+        got = nn.sNNGP(*_args(P, S), rng=oracle.TapeRng(normals, unifs),
+                       sweep=sweep)
+        _compare(got, want)
+
+
+def test_sNNGP_verbose_prints_the_reference_report(oracle, capfd):
+    import spnngp_amd as nn
+    P = _problem(120, 6, seed=12)
+    nn.sNNGP(*_args(P, 6), verbose=1, nReport=2, seed=3, sweep="coloured")
+    out = capfd.readouterr().out
+    assert "NNGP Latent model fit with 120 observations." in out
+    assert "Sampled: 6 of 6, 100.00%" in out
+    assert "colour classes" in out

@@ -75,3 +75,34 @@ def test_sNNGP_rejects_a_bad_transpose_before_the_device(nngp, oracle):
                    U["u.indx"], U["u.indx.lu"], U["ui.indx"], [2, 1], [2, 1],
                    [1, 30], None, [0.0], 1, 1, 5, 0, 0, 0, 0.1, 0, 2, sweep=7)
     assert e.value.status == nngp.NNGP_ERR_ARG
+
+
+def test_latent_entry_points_reject_bad_arguments(nngp):
+    import ctypes as C
+    L = nngp.load()
+    nn = np.zeros(1, np.int32)
+    lu = np.zeros(4, np.int32)
+    u = np.zeros(1, np.int32)
+    # n < 2 and m >= n are argument errors, before any device call
+    assert L.nngp_mkUIndx(1, 1, nn, u, lu, u, lu, 2) == nngp.NNGP_ERR_ARG
+    assert L.nngp_mkUIndx(2, 2, nn, u, lu, u, lu, 2) == nngp.NNGP_ERR_ARG
+    with pytest.raises(nngp.NngpError) as e:
+        nngp.sNNGP(np.zeros(3), np.ones((3, 1)), 1, 3, 1, np.zeros((3, 2)),
+                   7, np.zeros(2, np.int32), np.zeros(6, np.int32),
+                   np.zeros(2, np.int32), np.zeros(6, np.int32),
+                   np.zeros(2, np.int32), [2, 1], [2, 1], [1, 30], None,
+                   [0.0], 1, 1, 5, 0, 0, 0, 0.1, 0, 1)
+    assert e.value.status == nngp.NNGP_ERR_COV_MODEL
+    with pytest.raises(nngp.NngpError) as e:  # p = 0
+        nngp.sNNGP(np.zeros(3), np.ones((3, 1)), 0, 3, 1, np.zeros((3, 2)),
+                   0, np.zeros(2, np.int32), np.zeros(6, np.int32),
+                   np.zeros(2, np.int32), np.zeros(6, np.int32),
+                   np.zeros(2, np.int32), [2, 1], [2, 1], [1, 30], None,
+                   [0.0], 1, 1, 5, 0, 0, 0, 0.1, 0, 1)
+    assert e.value.status == nngp.NNGP_ERR_UNSUPPORTED
+    with pytest.raises(nngp.NngpError) as e:  # binomial family
+        nngp.sNNGPPredict(np.ones((3, 1)), None, np.zeros((3, 2)), 3, 1, 1,
+                          np.ones((1, 1)), np.zeros((1, 2)), 1,
+                          np.zeros((1, 1), np.int32), np.zeros((1, 1)),
+                          np.ones((3, 1)), np.zeros((3, 1)), 1, 0, family=2)
+    assert e.value.status == nngp.NNGP_ERR_UNSUPPORTED
