@@ -309,3 +309,31 @@ def test_sNNGP_verbose_prints_the_reference_report(oracle, capfd):
     assert "NNGP Latent model fit with 120 observations." in out
     assert "Sampled: 6 of 6, 100.00%" in out
     assert "colour classes" in out
+
+
+@pytest.mark.parametrize("sig_a,tau_a", [(0.5, 8.0), (12.0, 0.6)])
+def test_sNNGP_small_shapes_follow_the_oracle_gamma(oracle, sig_a, tau_a):
+    """With n = 6 the Gamma shapes IG.a + n/2 are small (3.5 and 11, or 15
+    and 3.6), where algorithm GD often leaves its immediate and squeeze
+    tests for the quotient test and the double-exponential loop; 300
+    iterations take every branch, so the library's Gamma deviates must
+    follow the oracle's draw for draw."""
+    import spnngp_amd as nn
+    P = _problem(6, 2, seed=31)
+    S = 300
+    rng = np.random.default_rng(77)
+    normals = rng.standard_normal(S * 40)
+    unifs = rng.random(S * 40)
+    a = list(_args(P, S))
+    a[12], a[13] = [sig_a, 1.0], [tau_a, 0.5]
+    a_or = list(a)
+    a_or[5] = P["c"].T.reshape(-1).copy()
+    t_or = oracle.TapeRng(normals, unifs)
+    want = oracle.sNNGP(*a_or[:21], a_or[23], a_or[24], S, t_or)
+    t_gpu = oracle.TapeRng(normals, unifs)
+    got = nn.sNNGP(*a, rng=t_gpu)
+    _compare(got, want)
+    assert (t_gpu.i_n, t_gpu.i_u) == (t_or.i_n, t_or.i_u)
+    # more uniforms than one per iteration: the Gamma draws went past the
+    # immediate-acceptance branch
+    assert t_or.i_u > 2 * S
