@@ -787,9 +787,36 @@ extern "C" int nngp_sNNGP(
                           hipMemcpyHostToDevice, st));
   NNGP_HIP(hipMemcpyAsync(dphase.p, phase.data(), dphase.bytes,
                           hipMemcpyHostToDevice, st));
-  PinnedBuf hz, hsum;
+  PinnedBuf hz, hsum, hw;
   if (rng) NNGP_TRY(hz.alloc(sizeof(double) * (size_t)n));
   NNGP_TRY(hsum.alloc(sizeof(double) * kLatStats));
+  // w samples leave through pinned staging on a second stream: the copy of
+  // iteration s and its move into the caller's array overlap iteration
+  // s + 1's sweep
+  NNGP_TRY(hw.alloc(sizeof(double) * (size_t)n));
+  Stream st2;
+  NNGP_TRY(st2.create());
+  struct Ev {
+    hipEvent_t e = nullptr;
+    ~Ev() {
+      if (e) (void)hipEventDestroy(e);
+    }
+  } evGather, evCopied;
+  NNGP_HIP(hipEventCreateWithFlags(&evGather.e, hipEventDisableTiming));
+  NNGP_HIP(hipEventCreateWithFlags(&evCopied.e, hipEventDisableTiming));
+  struct Drain {  // on every return, before hw and the events go away
+    hipStream_t s;
+    ~Drain() { (void)hipStreamSynchronize(s); }
+  } drain{st2.s};
+  int pendingS = -1;  // iteration whose w is in, or on its way to, hw
+  auto flush_w = [&]() -> int {
+    if (pendingS < 0) return NNGP_OK;
+    NNGP_HIP(hipEventSynchronize(evCopied.e));
+    std::memcpy(pWSamples + (size_t)pendingS * n, hw.p,
+                sizeof(double) * (size_t)n);
+    pendingS = -1;
+    return NNGP_OK;
+  };
 
   // launch plan of one sweep: [ph0, ph1) per launch, grid or block
   struct Seg {
@@ -912,6 +939,7 @@ extern "C" int nngp_sNNGP(
       }
       NNGP_HIP(hipGetLastError());
     }
+    NNGP_TRY(flush_w());  // the previous iteration's w, while the sweep runs
     // beta | rest (src/sNNGP.cpp:292-308) and the w quadratic form of the
     // current B, F for sigma.sq
     NNGP_TRY(reduce(0, cur));
@@ -958,14 +986,19 @@ extern "C" int nngp_sNNGP(
     for (int k = 0; k < p; k++) pBetaSamples[(size_t)s * p + k] = beta[k];
     for (int k = 0; k < nTheta; k++)
       pThetaSamples[(size_t)s * nTheta + k] = th[k];
+    // w of this iteration: gathered on st, copied to hw on st2 (the
+    // previous copy has been waited for in flush_w, so hw and dwout are
+    // free), moved to pWSamples during the next iteration
     hipLaunchKernelGGL(k_gather_w, dim3((n + 255) / 256), dim3(256), 0, st,
                        (const double *)md.Z.as<double>(), md.zs, n,
                        dwout.as<double>());
     NNGP_HIP(hipGetLastError());
-    NNGP_HIP(hipMemcpyAsync(pWSamples + (size_t)s * n, dwout.p,
-                            sizeof(double) * (size_t)n, hipMemcpyDeviceToHost,
-                            st));
-    NNGP_HIP(hipStreamSynchronize(st));
+    NNGP_HIP(hipEventRecord(evGather.e, st));
+    NNGP_HIP(hipStreamWaitEvent(st2.s, evGather.e, 0));
+    NNGP_HIP(hipMemcpyAsync(hw.p, dwout.p, sizeof(double) * (size_t)n,
+                            hipMemcpyDeviceToHost, st2.s));
+    NNGP_HIP(hipEventRecord(evCopied.e, st2.s));
+    pendingS = s;
     if (status == nReport) {
       if (verbose) {
         printf("Sampled: %i of %i, %3.2f%%\n", s, nSamples,
@@ -982,9 +1015,12 @@ extern "C" int nngp_sNNGP(
     }
     status++;
     if (nAccept) *nAccept = accept;
-    if (opts && opts->interrupt && opts->interrupt(opts->interrupt_arg))
+    if (opts && opts->interrupt && opts->interrupt(opts->interrupt_arg)) {
+      NNGP_TRY(flush_w());
       return set_err(NNGP_ERR_INTERRUPT, "interrupted after %d samples", s + 1);
+    }
   }
+  NNGP_TRY(flush_w());
   if (verbose) {
     printf("Sampled: %i of %i, %3.2f%%\n", nSamples, nSamples, 100.0);
     printf("Overall Metrop. Acceptance rate: %3.2f%%\n",
