@@ -95,7 +95,10 @@ struct SweepArgs {
   const int4 *meta;
   const int2 *uoc;
   const int *lu;  // nnIndxLU
-  double *R;      // residuals w_i - B_i . w_N(i) at the current w
+  // per site {F_i, r_i}: F of the sweep's B, F slot and the residual
+  // w_i - B_i . w_N(i) at the current w, side by side so that a child's two
+  // values come in one 16 B read
+  double2 *FR;
   double beta[kMaxP];
   double tau2;
   unsigned long long seed;  // Philox key when z == nullptr
@@ -161,8 +164,10 @@ __device__ void sweep_site(const SweepArgs &A, int4 mt, int lane) {
   const int n = A.n, ws = A.ws;
   double *w = A.w;
   const int i = mt.x, uo = mt.y, uc = mt.z;
-  const double wi = w[(size_t)i * ws], ri = A.R[i];
-  const double yi = A.y[i], fi = A.F[i];
+  const double wi = w[(size_t)i * ws];
+  const double2 fri = A.FR[i];
+  const double fi = fri.x, ri = fri.y;
+  const double yi = A.y[i];
   double xb = 0.0;
   for (int q = 0; q < A.p; q++) xb += A.X[(size_t)q * n + i] * A.beta[q];
   double a = 0.0, v = 0.0;
@@ -182,8 +187,8 @@ __device__ void sweep_site(const SweepArgs &A, int4 mt, int lane) {
         const int t = uo + j;
         const int jj = A.uIndx[t];
         const double bij = A.B[A.uoc[t].x + A.uiIndx[t]];
-        const double fj = A.F[jj];
-        const double rj = A.R[jj];
+        const double2 frj = A.FR[jj];
+        const double fj = frj.x, rj = frj.y;
         ta[c] = bij * (rj + bij * wi) / fj;
         tv[c] = bij * bij / fj;
         if (j0 == 0) {
@@ -211,7 +216,7 @@ __device__ void sweep_site(const SweepArgs &A, int4 mt, int lane) {
   const double d = wn - wi;
   if (lane == 0) {
     w[(size_t)i * ws] = wn;
-    A.R[i] = ri + d;
+    A.FR[i].y = ri + d;
   }
   for (int j0 = 0; j0 < uc; j0 += kChild) {
 #pragma unroll
@@ -228,19 +233,19 @@ __device__ void sweep_site(const SweepArgs &A, int4 mt, int lane) {
         const int t = uo + j;
         jj = A.uIndx[t];
         bij = A.B[A.uoc[t].x + A.uiIndx[t]];
-        rj = A.R[jj];
+        rj = A.FR[jj].y;
       }
-      A.R[jj] = rj - bij * d;
+      A.FR[jj].y = rj - bij * d;
     }
   }
 }
 
-// r_i = w_i - B_i . w_N(i) for every site, before a sweep
+// {F_i, r_i = w_i - B_i . w_N(i)} for every site, before a sweep
 __global__ __launch_bounds__(kSweepGrid) void k_resid_init(SweepArgs A) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (i >= A.n) return;
-  A.R[i] = A.w[(size_t)i * A.ws] -
-           gather_dot(A, A.lu[i], A.lu[A.n + i], -1);
+  A.FR[i] = make_double2(A.F[i], A.w[(size_t)i * A.ws] -
+                                     gather_dot(A, A.lu[i], A.lu[A.n + i], -1));
 }
 
 // One block walks phases [ph0, ph1) with a barrier between them.  The
@@ -280,8 +285,8 @@ constexpr int kRedMax = 2048;  // partial blocks
 constexpr int kLatStats = kMaxP + 1;
 
 struct LatSumArgs {
-  const double *F, *y, *X, *w;
-  const double *R;  // residuals w_i - B_i . w_N(i), current after a sweep
+  const double *y, *X, *w;
+  const double2 *FR;  // {F_i, w_i - B_i . w_N(i)}, current after a sweep
   double beta[kMaxP];
   double tau2;
   int n, p, ws, mode;  // mode 0: X^T (y - w) / tau.sq and the w quadratic
@@ -312,8 +317,8 @@ __global__ __launch_bounds__(kRedBlock) void k_lat_partial(LatSumArgs A) {
     if (A.mode == 0) {
       const double r = (A.y[i] - wi) / A.tau2;
       for (int k = 0; k < A.p; k++) acc[k] += A.X[(size_t)k * n + i] * r;
-      const double b = A.R[i];
-      acc[kMaxP] += b * b / A.F[i];
+      const double2 fr = A.FR[i];
+      acc[kMaxP] += fr.y * fr.y / fr.x;
     } else {
       double xb = 0.0;
       for (int k = 0; k < A.p; k++) xb += A.X[(size_t)k * n + i] * A.beta[k];
@@ -751,7 +756,7 @@ extern "C" int nngp_sNNGP(
   NNGP_TRY(dui.alloc(sizeof(int) * (size_t)std::max(nIndx, 1LL)));
   NNGP_TRY(duoc.alloc(sizeof(int2) * (size_t)std::max(nIndx, 1LL)));
   NNGP_TRY(dmeta.alloc(sizeof(int4) * (size_t)n));
-  NNGP_TRY(dR.alloc(sizeof(double) * (size_t)n));
+  NNGP_TRY(dR.alloc(sizeof(double2) * (size_t)n));
   NNGP_TRY(dphase.alloc(sizeof(int) * phase.size()));
   NNGP_TRY(dpart.alloc(sizeof(double) * kRedMax * kLatStats));
   NNGP_TRY(dsum.alloc(sizeof(double) * kLatStats));
@@ -862,8 +867,8 @@ extern "C" int nngp_sNNGP(
       std::min(kRedMax, std::max(1, (n + kRedBlock - 1) / kRedBlock));
   auto reduce = [&](int mode, int slot) -> int {
     LatSumArgs L;
-    L.R = dR.as<double>();
-    L.F = md.F[slot].as<double>();
+    (void)slot;  // the sweep's slot: FR holds its F
+    L.FR = dR.as<double2>();
     L.y = dy.as<double>();
     L.X = dX.as<double>();
     L.w = md.Z.as<double>();
@@ -912,7 +917,7 @@ extern "C" int nngp_sNNGP(
     A.meta = dmeta.as<int4>();
     A.uoc = duoc.as<int2>();
     A.lu = md.lu.as<int>();
-    A.R = dR.as<double>();
+    A.FR = dR.as<double2>();
     for (int k = 0; k < kMaxP; k++) A.beta[k] = k < p ? beta[k] : 0.0;
     A.tau2 = th[1];
     A.seed = opts ? opts->seed : 1ull;
