@@ -203,6 +203,34 @@ def replicates_bench(c, X, idx, m, S, seed, cpu_samples):
     return out
 
 
+def knn_bench(c, idx, m, gpu_s, cpu):
+    """mkNNIndxCB at the bench size: the device search's wall time (host
+    strip sort and copies included; a second, warm call) and, with cpu, the oracle's restatement of the
+    reference's code-book search (src/nn.cpp:128-257, OpenMP over sites) on
+    the same sites, whose neighbor lists must equal the device's bit for
+    bit."""
+    import spnngp_amd as nn
+    t0 = time.perf_counter()
+    nn.mkNNIndxCB(c, m)  # warm: the first call also loads the module
+    out = {"n": int(c.shape[0]), "m": m, "gpu_s": time.perf_counter() - t0,
+           "gpu_first_call_s": gpu_s}
+    if cpu:
+        from oracle import oracle as O
+        nt = cores()
+        t0 = time.perf_counter()
+        ref = O.mkNNIndxCB(c, m, nt)
+        cdt = time.perf_counter() - t0
+        out["bit_exact"] = bool(
+            np.array_equal(ref["nnIndx"], idx["nnIndx"])
+            and np.array_equal(ref["nnIndxLU"], idx["nnIndxLU"])
+            and np.array_equal(ref["nnDist"], idx["nnDist"]))
+        out["cpu_baseline"] = {
+            "value": cdt, "unit": "s", "cores": nt, "kind": "port",
+            "sample": f"the whole n={c.shape[0]}, m={m} search (oracle "
+                      f"mkNNIndxCB, OpenMP)"}
+    return out
+
+
 def latent_bench(c, X, y, idx, m, iters, site_iters, cpu_iters, seed):
     """Latent model (sNNGP) on the same data: the neighbors-of-neighbors
     index (mkUIndx, search type 2) and MCMC iterations/s with the coloured
@@ -547,6 +575,7 @@ def main():
     ap.add_argument("--latent-iters", type=int, default=100)
     ap.add_argument("--latent-site-iters", type=int, default=3)
     ap.add_argument("--latent-cpu-iters", type=int, default=2)
+    ap.add_argument("--knn-cpu", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is "
@@ -601,6 +630,9 @@ def main():
     reps = replicates_bench(c, X, idx, m, args.rep_samples, 6,
                             args.rep_cpu_samples) \
         if args.rep_samples > 0 else None
+
+    print("[bench] neighbor search", file=sys.stderr, flush=True)
+    knn = knn_bench(c, idx, m, knn_s, args.knn_cpu > 0)
 
     print("[bench] latent", file=sys.stderr, flush=True)
     latent = latent_bench(c, X, y, idx, m, args.latent_iters,
@@ -658,6 +690,7 @@ def main():
         "mixed": mixed,
         "bf_tune": f64["tune"],
         "knn_s": knn_s,
+        "knn": knn,
     }
     print(json.dumps(line), flush=True)
 
