@@ -291,7 +291,7 @@ def latent_bench(c, X, y, idx, m, iters, site_iters, cpu_iters, seed):
     return out
 
 
-def kriging_serving(c, X, y, m, seconds, rate, S, seed):
+def kriging_serving(c, X, y, m, seconds, rate, S, seed, cpu_sites=0):
     """Kriging server under load: requests of q ~ floor(exp(U[0, ln 1e5]))
     new sites, Poisson arrivals at `rate` per second for `seconds`, against
     the resident fitted model with S posterior samples.  Latency is result
@@ -369,6 +369,28 @@ def kriging_serving(c, X, y, m, seconds, rate, S, seed):
         ok += info["status"] == 0
         lat[k] = (info["t_done"] - (base + arrivals[k])) * 1e3
     mid = (sizes >= 5_000) & (sizes <= 20_000)
+    cpu = None
+    if cpu_sites > 0:
+        # the oracle's rNNGPPredict (src/rNNGPPredict.cpp:25-222, OpenMP over
+        # samples as the reference) on cpu_sites new sites x S samples
+        from oracle import oracle as O
+        X0, c0 = req(cpu_sites, 0)
+        nn0, _ = nn.nn2(c, c0, m)
+        z = rng.normal(size=cpu_sites * S)
+        nt = cores()
+        t0 = time.perf_counter()
+        O.rNNGPPredict(X.T.reshape(-1).copy(), y, c.T.reshape(-1).copy(), n,
+                       p, m, X0.T.reshape(-1).copy(), c0.T.reshape(-1).copy(),
+                       cpu_sites, nn0.T.reshape(-1).copy(),
+                       beta.T.reshape(-1).copy(), th.T.reshape(-1).copy(), S,
+                       0, z, nThreads=nt)
+        cdt = time.perf_counter() - t0
+        cpu = {"value": cpu_sites * S / cdt, "unit": "site_samples/s",
+               "cores": nt, "kind": "port",
+               "q1e4_request_ms": cdt * 1e3 * 10_000 / cpu_sites,
+               "sample": f"{cpu_sites} new sites x {S} samples of the same "
+                         f"model (oracle rNNGPPredict, OpenMP), {cdt:.2f} s; "
+                         f"the 10k-site figure scales it linearly"}
     return {"requests": int(nreq), "ok": int(ok), "rate_per_s": rate,
             "seconds": seconds, "samples": S,
             "q": "floor(exp(U[0, ln 1e5])), mean %.0f" % sizes.mean(),
@@ -383,6 +405,7 @@ def kriging_serving(c, X, y, m, seconds, rate, S, seed):
             "alone_q1e4_p50_ms": float(np.percentile(alone, 50)),
             "alone_q1e4_p99_ms": float(np.percentile(alone, 99)),
             "setup_s": setup_s,
+            "cpu_baseline": cpu,
             "note": "resident model (n=1e6, m=15, exponential), nn2 + "
                     "kriging on the GPU, results delivered to pinned host "
                     "memory; latency = result on host - scheduled arrival"}
@@ -576,6 +599,7 @@ def main():
     ap.add_argument("--latent-site-iters", type=int, default=3)
     ap.add_argument("--latent-cpu-iters", type=int, default=2)
     ap.add_argument("--knn-cpu", type=int, default=1)
+    ap.add_argument("--serve-cpu-sites", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is "
@@ -624,7 +648,8 @@ def main():
         if args.cpu_iters > 0 else None
     print("[bench] kriging", file=sys.stderr, flush=True)
     krig = kriging_serving(c, X, y, m, args.serve_seconds, args.serve_rate,
-                           500, 4) if args.serve_seconds > 0 else None
+                           500, 4, args.serve_cpu_sites) \
+        if args.serve_seconds > 0 else None
 
     print("[bench] replicates", file=sys.stderr, flush=True)
     reps = replicates_bench(c, X, idx, m, args.rep_samples, 6,
