@@ -444,6 +444,7 @@ def conj_grid_bench(n, m, k, seed, cpu_points):
     t0 = time.perf_counter()
     r = cv.run(ta)
     wall_s = time.perf_counter() - t0
+    det = cv.run(ta, details=True) if cpu_points > 0 else None  # untimed
     cv.close()
     best = int(np.argmin(r["scores"][:, 1]))
     nmod = n - nho
@@ -474,12 +475,21 @@ def conj_grid_bench(n, m, k, seed, cpu_points):
         sub = ta[:cpu_points]
         nt = cores()
         t0 = time.perf_counter()
-        O.cNNGP(ym, Xm.T.reshape(-1).copy(), 3, nmod,
+        ref = O.cNNGP(ym, Xm.T.reshape(-1).copy(), 3, nmod,
                 cm.T.reshape(-1).copy(), sub.reshape(-1).copy(),
                 nnm["nnIndx"], nnm["nnIndxLU"], X0.T.reshape(-1).copy(),
                 c0.T.reshape(-1).copy(), nho, nn0.T.reshape(-1).copy(),
                 cpu_points, m, [2.0, 1.0], 2, nThreads=nt)
         dt = time.perf_counter() - t0
+        # the device's fold 1 at the same grid points, against the oracle
+        bh = np.asarray(ref["beta.hat"]).T  # g x p
+        abo = np.asarray(ref["ab"]).T       # g x 2
+        res["cpu_check"] = {
+            "problems": cpu_points,
+            "beta_hat_max_rel": float(np.max(
+                np.abs(det["beta.hat"][0, :cpu_points] - bh) / np.abs(bh))),
+            "ab_max_rel": float(np.max(
+                np.abs(det["ab"][0, :cpu_points] - abo) / np.abs(abo)))}
         res["cpu_baseline"] = {
             "value": cpu_points * nmod / dt, "unit": "site-updates/s",
             "cores": nt, "kind": "port",
