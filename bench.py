@@ -609,7 +609,7 @@ def main():
     ap.add_argument("--latent-site-iters", type=int, default=3)
     ap.add_argument("--latent-cpu-iters", type=int, default=2)
     ap.add_argument("--knn-cpu", type=int, default=1)
-    ap.add_argument("--serve-cpu-sites", type=int, default=1000)
+    ap.add_argument("--serve-cpu-sites", type=int, default=5000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is "
