@@ -10,7 +10,7 @@ current theta, updateBF of the proposal with its log-det and quadratic
 forms, accept/reject).  Data, neighbor index and B/F buffers are resident
 in HBM before the timed region starts; the host keeps the sampler.
 
-Prints one JSON line.  --gpus > 1 is refused: this build runs on one GPU.
+Prints one JSON line.  --gpus > 1 is refused: the library runs on one GPU.
 """
 from __future__ import annotations
 
@@ -612,7 +612,7 @@ def main():
     ap.add_argument("--serve-cpu-sites", type=int, default=5000)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is "
+        print(f"bench.py: the library runs on one GPU; --gpus {args.gpus} is "
               "not supported (the multi-GPU sharding is a design only)",
               file=sys.stderr)
         sys.exit(2)

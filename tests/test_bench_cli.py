@@ -1,5 +1,6 @@
 """bench.py's command line: more than one GPU is refused with a message
-(this build runs on one GPU) before anything touches a device.  Host only."""
+(the library runs on one GPU) before anything touches a device.  Host
+only."""
 import os
 import subprocess
 import sys
