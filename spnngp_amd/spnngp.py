@@ -668,6 +668,9 @@ def predict(obj, X_0, coords_0, *, sub_sample=None, n_omp_threads=1,
     if obj["type"][0] == "latent":
         # src/sNNGPPredict.cpp via R/generics.R:600-672: the kept w samples
         # at the fitted sites condition the new ones
+        if server is not None:
+            raise ValueError("the kriging server serves response fits; a "
+                             "latent fit is predicted without one")
         nn0, _ = _nn.nn2(coords, coords_0, m)
         w = np.asarray(obj["p.w.samples"])[:, s_indx - 1]
         out = _nn.sNNGPPredict(X, y, coords, n, p, m, X_0, coords_0, q, nn0,

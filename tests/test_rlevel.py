@@ -104,3 +104,29 @@ def test_fit_file_rejects_object_arrays(tmp_path):
     fit["bad"] = np.array([{"a": 1}], dtype=object)
     with pytest.raises(ValueError):
         R.save_fit(fit, tmp_path / "bad.npz")
+
+
+def test_latent_fit_round_trip_and_its_guards(tmp_path):
+    """A latent fit (p.w.samples, the neighbors-of-neighbors index, the
+    sweep order) survives the fit file; serve() and a server for predict()
+    are refused for it, before any device work."""
+    fit = _fake_fit()
+    rng = np.random.default_rng(5)
+    n, S = fit["X"].shape[0], fit["p.beta.samples"].shape[0]
+    fit["type"] = ["latent", "gaussian"]
+    fit["sweep"] = "coloured"
+    fit["p.w.samples"] = rng.normal(size=(n, S))
+    fit["neighbor.info"]["u.indx"] = np.arange(10, dtype=np.int32)
+    fit["neighbor.info"]["ui.indx"] = np.zeros(10, np.int32)
+    path = tmp_path / "latent.npz"
+    R.save_fit(fit, path)
+    back = R.load_fit(path)
+    assert back["type"] == ["latent", "gaussian"]
+    assert back["sweep"] == "coloured"
+    assert np.array_equal(back["p.w.samples"], fit["p.w.samples"])
+    assert np.array_equal(back["neighbor.info"]["u.indx"],
+                          fit["neighbor.info"]["u.indx"])
+    with pytest.raises(ValueError):
+        R.serve(back)
+    with pytest.raises(ValueError):
+        R.predict(back, np.ones((2, 2)), np.zeros((2, 2)), server=object())
