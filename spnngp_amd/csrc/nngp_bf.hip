@@ -742,9 +742,25 @@ __global__ __launch_bounds__(kRowsThreads) void k_rows_sum(
   __shared__ double red[kRowsThreads / 64];
   const int k = (int)blockIdx.x, j = (int)blockIdx.y;
   const int w = (int)threadIdx.x >> 6, t = (int)threadIdx.x & 63;
-  double s = 0.0;
   const double *col = rows + (size_t)j * ps_rows + (size_t)k * cap;
-  for (int r = (int)threadIdx.x; r < nrows; r += kRowsThreads) s += col[r];
+  // eight independent partial sums, so that a thread's loads are in flight
+  // together; the order (per thread, then the fixed combination) is fixed
+  constexpr int U = 8;
+  double acc[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) acc[u] = 0.0;
+  int r = (int)threadIdx.x;
+  for (; r + (U - 1) * kRowsThreads < nrows; r += U * kRowsThreads) {
+#pragma unroll
+    for (int u = 0; u < U; u++) acc[u] += col[r + u * kRowsThreads];
+  }
+#pragma unroll
+  for (int u = 0; u < U; u++) {  // the tail: fewer than U rows left
+    const int rr = r + u * kRowsThreads;
+    if (rr < nrows) acc[u] += col[rr];
+  }
+  double s = ((acc[0] + acc[1]) + (acc[2] + acc[3])) +
+             ((acc[4] + acc[5]) + (acc[6] + acc[7]));
   s = wave_sum(s);
   if (t == 0) red[w] = s;
   __syncthreads();
