@@ -371,7 +371,11 @@ typedef struct nngp_tune_report {
   int threads[NNGP_TUNE_MAX];     /* threads per block                    */
   int mode[NNGP_TUNE_MAX];        /* family 0, NNGP_MIXED: 1 = the
                                      covariance copy kept in LDS for the
-                                     refinement residual                  */
+                                     refinement residual; family 0, FP64:
+                                     2 = the covariate rows of each site's
+                                     neighbors staged in LDS during the
+                                     factorization (same results bit for
+                                     bit as 0)                            */
   double ms[NNGP_TUNE_MAX];       /* mean launch time                     */
   int best_family, best_threads, best_mode;
 } nngp_tune_report;

@@ -35,6 +35,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <cstdint>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -148,14 +149,30 @@ __device__ inline void block_site_terms(int nz, double *rows, int rows_cap,
   }
 }
 
+// Direct-to-LDS load of 16 bytes a lane: lane t's bytes land at dst + t
+// (dst the same for the whole wavefront).  The load counts in vmcnt like any
+// other; the data are in LDS once it has been waited for.
+__device__ __forceinline__ void lds_load16(const double *src, double2 *dst) {
+  __builtin_amdgcn_global_load_lds(
+      (const __attribute__((address_space(1))) void *)src,
+      (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
+}
+
 // One site per lane: B_i, F_i and the residual vector e of site i from its
 // neighbor count, offset and index slots (slots past the count are not its
 // neighbors; they are replaced by site 0 here, before any use).
-template <int M, int COV, bool BATCH>
+// ZH > 0 stages the rows of Z (zs = 2 ZH doubles) of the M neighbor slots
+// and of the site itself in LDS right after the coordinate gathers, so that
+// they arrive while the system is built and factored: chunk h of slot k
+// (slot M = the site) at stg[(k * ZH + h) * 64 + lane].  ZH is a constant
+// so that the waits before the factorization count the staged loads
+// exactly.  The residuals add the same terms in the same order either way.
+template <int M, int COV, bool BATCH, int ZH = 0>
 __device__ __forceinline__ void lane_site(const BfArgs &A, const BfProblem &P,
                                           const CovArgs &c, int i, int cnt,
                                           int off, int (&nb)[M], double &Fi,
-                                          double (&e)[kMaxNZ]) {
+                                          double (&e)[kMaxNZ],
+                                          double2 *stg = nullptr) {
   if (cnt == 0) {  // site 0: F = s2 + t2, residual = the data
     Fi = c.diag;
 #pragma unroll
@@ -185,6 +202,15 @@ __device__ __forceinline__ void lane_site(const BfArgs &A, const BfProblem &P,
     }
   }
   const double2 s = P.xy[i];
+  if constexpr (ZH > 0) {
+#pragma unroll
+    for (int k = 0; k <= M; k++) {
+      const double *zr = P.Z + (size_t)(k < M ? nb[k] : i) * (2 * ZH);
+#pragma unroll
+      for (int h = 0; h < ZH; h++)
+        lds_load16(zr + 2 * h, stg + (k * ZH + h) * 64);
+    }
+  }
 
   constexpr int T = M * (M + 1) / 2;
   double a[T];  // packed lower triangle
@@ -206,6 +232,24 @@ __device__ __forceinline__ void lane_site(const BfArgs &A, const BfProblem &P,
   }
   Fi = c.diag - vv;
   // residual vectors; padded coefficients are exactly 0
+  if constexpr (ZH > 0) {
+    const int t = (int)threadIdx.x & 63;
+#pragma unroll
+    for (int h = 0; h < ZH; h++) {
+      const double2 zi = stg[(M * ZH + h) * 64 + t];
+      double e0 = zi.x, e1 = zi.y;
+#pragma unroll
+      for (int k = 0; k < M; k++) {
+        const double2 z = stg[(k * ZH + h) * 64 + t];
+        e0 = fma(-v[k], z.x, e0);
+        e1 = fma(-v[k], z.y, e1);
+      }
+      e[2 * h] = e0;
+      e[2 * h + 1] = e1;
+    }
+    if (!ok) atomicMin(P.err, i);
+    return;
+  }
 #pragma unroll
   for (int j = 0; j < kMaxNZ; j++) {
     if (j < A.nz) {
@@ -247,6 +291,44 @@ __global__ __launch_bounds__(kLaneMaxThreads) void k_bf_lane(BfArgs A) {
     for (int k = 0; k < M; k++) nb[k] = P.nnIndx[off + k];
     lane_site<M, COV, BATCH>(A, P, c, i, cnt, off, nb, Fi, e);
     if (!BATCH && A.F) A.F[i] = Fi;
+  }
+  if (P.rows) block_site_terms(A.nz, P.rows, A.rows_cap, valid, Fi, e, b);
+}
+
+// k_bf_lane (one problem) with the covariate rows staged in LDS: the
+// block's dynamic LDS holds, per wavefront, (M + 1) x ZH chunks of 64 x 16
+// bytes.  The launcher uses it only when Z is set, zs = 2 ZH and Z is
+// 16-byte aligned.
+template <int M, int COV, int ZH>
+__global__ __launch_bounds__(kLaneMaxThreads) void k_bf_lane_lds(BfArgs A) {
+  extern __shared__ double2 zstage[];
+  const int nt = (int)blockDim.x;
+  const int b = xcd_block((A.n + nt - 1) / nt);
+  if (b < 0) return;  // whole padding block
+  const BfProblem P = bf_problem<false>(A);
+  const CovArgs c = A.c;
+  const int i = b * nt + (int)threadIdx.x;
+  const bool valid = i < A.n;
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  double2 *stg = zstage + (size_t)w * (M + 1) * ZH * 64;
+  double Fi = 1.0;
+  double e[kMaxNZ];
+#pragma unroll
+  for (int j = 0; j < kMaxNZ; j++) e[j] = 0.0;
+
+  if (valid) {
+    int cnt, off;
+    if (A.lu_closed) {
+      nn_closed(i, A.m, off, cnt);
+    } else {
+      cnt = P.lu[A.n + i];
+      off = P.lu[i];
+    }
+    int nb[M];
+#pragma unroll
+    for (int k = 0; k < M; k++) nb[k] = P.nnIndx[off + k];
+    lane_site<M, COV, false, ZH>(A, P, c, i, cnt, off, nb, Fi, e, stg);
+    if (A.F) A.F[i] = Fi;
   }
   if (P.rows) block_site_terms(A.nz, P.rows, A.rows_cap, valid, Fi, e, b);
 }
@@ -807,7 +889,7 @@ bool bf_config_ok(const BfConfig &cfg, int m, int variant) {
   if (cfg.family == 0)
     return bf_lane_ok(m, variant) &&
            (cfg.threads == 64 || cfg.threads == 128 || cfg.threads == 256) &&
-           (cfg.mode == 0 ||
+           (cfg.mode == 0 || cfg.mode == 2 ||
             (cfg.mode == 1 && mixed_copy_shm(m, cfg.threads) <= 65536));
   const int waves = cfg.threads / 64;
   return m <= kCoopMaxM && cfg.threads % 64 == 0 && waves >= 1 &&
@@ -851,6 +933,52 @@ static void launch_lane_m(const BfArgs &a, int variant, const dim3 &grid,
 static int lane_M(int m) {
   return m <= 4 ? 4 : m <= 8 ? 8 : m <= 10 ? 10 : m <= 12 ? 12 : m <= 15 ? 15
                                                                           : 16;
+}
+
+// LDS of k_bf_lane_lds: per wavefront, M + 1 rows of zs doubles per lane.
+static size_t lane_lds_shm(int m, int zs, int threads) {
+  return (size_t)(threads / 64) * (lane_M(m) + 1) * (zs / 2) * 64 *
+         sizeof(double2);
+}
+
+// Whether an FP64 single-problem pass can stage its covariate rows in LDS
+// (mode 2: m > 8, where the factorization is long enough to cover the
+// loads); where it cannot, mode 2 runs the plain per-lane kernel, whose
+// results are the same bit for bit.
+static bool lane_lds_applies(const BfArgs &a, int threads) {
+  return !a.mixed && a.ng == 0 && a.Z && a.nz > 0 && a.m > 8 &&
+         (a.zs == 2 || a.zs == 4) && ((uintptr_t)a.Z & 15) == 0 &&
+         lane_lds_shm(a.m, a.zs, threads) <= 65536;
+}
+
+template <int M, int ZH>
+static void launch_lane_lds_mh(const BfArgs &a, int variant, const dim3 &grid,
+                               const dim3 &blk, size_t shm, hipStream_t st) {
+  switch (variant) {
+    case 0: hipLaunchKernelGGL((k_bf_lane_lds<M, 0, ZH>), grid, blk, shm, st, a); break;
+    case 1: hipLaunchKernelGGL((k_bf_lane_lds<M, 1, ZH>), grid, blk, shm, st, a); break;
+    case kCovMatern15: hipLaunchKernelGGL((k_bf_lane_lds<M, kCovMatern15, ZH>), grid, blk, shm, st, a); break;
+    case kCovMatern25: hipLaunchKernelGGL((k_bf_lane_lds<M, kCovMatern25, ZH>), grid, blk, shm, st, a); break;
+    default: hipLaunchKernelGGL((k_bf_lane_lds<M, 3, ZH>), grid, blk, shm, st, a); break;
+  }
+}
+
+template <int M>
+static void launch_lane_lds_m(const BfArgs &a, int variant, const dim3 &grid,
+                              const dim3 &blk, size_t shm, hipStream_t st) {
+  if (a.zs == 2) launch_lane_lds_mh<M, 1>(a, variant, grid, blk, shm, st);
+  else launch_lane_lds_mh<M, 2>(a, variant, grid, blk, shm, st);
+}
+
+static void launch_lane_lds(const BfArgs &a, int variant, const dim3 &grid,
+                            const dim3 &blk, hipStream_t st) {
+  const size_t shm = lane_lds_shm(a.m, a.zs, (int)blk.x);
+  switch (lane_M(a.m)) {
+    case 10: launch_lane_lds_m<10>(a, variant, grid, blk, shm, st); break;
+    case 12: launch_lane_lds_m<12>(a, variant, grid, blk, shm, st); break;
+    case 15: launch_lane_lds_m<15>(a, variant, grid, blk, shm, st); break;
+    default: launch_lane_lds_m<16>(a, variant, grid, blk, shm, st); break;
+  }
 }
 
 // LDS of the mixed kernel's covariance copy: (M(M+1)/2 + M) floats a lane.
@@ -943,6 +1071,8 @@ int bf_launch(const BfArgs &a0, const BfConfig &cfg, int nprob,
   if (cfg.family == 0) {
     if (batch) launch_lane<true>(a, variant, grid, blk, st);
     else if (a.mixed) launch_mixed(a, variant, cfg.mode == 1, grid, blk, st);
+    else if (cfg.mode == 2 && lane_lds_applies(a, cfg.threads))
+      launch_lane_lds(a, variant, grid, blk, st);
     else launch_lane<false>(a, variant, grid, blk, st);
   } else {
     a.coop_blocks = nrows;
@@ -1000,8 +1130,11 @@ int bf_autotune(const BfArgs &a, int nprob, hipStream_t st, BfConfig *best,
                   // copy in LDS; then the wavefront one
     for (int th : {64, 128, 256})
       for (int mode : {0, 1}) cand.push_back(BfConfig{0, th, mode});
-  } else {
-    for (int th : {64, 128, 256}) cand.push_back(BfConfig{0, th, 0});
+  } else {  // the per-lane kernel: block size x covariate rows in LDS
+    for (int th : {64, 128, 256}) {
+      cand.push_back(BfConfig{0, th, 0});
+      if (lane_lds_applies(a, th)) cand.push_back(BfConfig{0, th, 2});
+    }
   }
   for (int w : {1, 2, 4}) cand.push_back(BfConfig{1, 64 * w, 0});
   hipEvent_t e0, e1;

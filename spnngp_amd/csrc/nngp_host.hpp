@@ -115,12 +115,93 @@ int check_neighbors(int n, int m, const int *nnIndx, const int *nnIndxLU);
 
 // ---------------------------------------------------------------------------
 // small dense algebra (lower triangle, column-major, the reference's
-// dpotrf / dpotri / dsymv / dtrmv sequence of src/rNNGP.cpp:288-294)
+// dpotrf / dpotri / dsymv / dtrmv sequence of src/rNNGP.cpp:288-294): LAPACK
+// dpotf2 / dtrti2 + dlauu2, BLAS dsymv / dtrmv loop orders, lda = n; host
+// and device (the resident chain), without contracted multiply-adds
 // ---------------------------------------------------------------------------
-int h_potrf_L(int n, double *A);
-int h_potri_L(int n, double *A);
-void h_symv_L(int n, const double *A, const double *x, double *y);
-void h_trmv_L(int n, const double *A, double *x);
+#define A_(i, j) A[(size_t)(j) * n + (i)]
+__host__ __device__ inline int h_potrf_L(int n, double *A) {
+#pragma clang fp contract(off)
+  for (int j = 0; j < n; j++) {
+    double dot = 0.0;
+    for (int k = 0; k < j; k++) dot += A_(j, k) * A_(j, k);
+    double ajj = A_(j, j) - dot;
+    if (ajj <= 0.0 || ajj != ajj) return j + 1;
+    ajj = sqrt(ajj);
+    A_(j, j) = ajj;
+    for (int k = 0; k < j; k++) {
+      const double t = -A_(j, k);
+      for (int i = j + 1; i < n; i++) A_(i, j) += t * A_(i, k);
+    }
+    const double r = 1.0 / ajj;
+    for (int i = j + 1; i < n; i++) A_(i, j) *= r;
+  }
+  return 0;
+}
+
+__host__ __device__ inline int h_potri_L(int n, double *A) {
+#pragma clang fp contract(off)
+  for (int j = 0; j < n; j++)
+    if (A_(j, j) == 0.0) return j + 1;
+  for (int j = n - 1; j >= 0; j--) {
+    A_(j, j) = 1.0 / A_(j, j);
+    const double ajj = -A_(j, j);
+    const int nn = n - j - 1;
+    for (int jj = nn - 1; jj >= 0; jj--) {
+      const double t = A_(j + 1 + jj, j);
+      if (t != 0.0) {
+        for (int ii = nn - 1; ii > jj; ii--)
+          A_(j + 1 + ii, j) += t * A_(j + 1 + ii, j + 1 + jj);
+        A_(j + 1 + jj, j) = t * A_(j + 1 + jj, j + 1 + jj);
+      }
+    }
+    for (int ii = 0; ii < nn; ii++) A_(j + 1 + ii, j) *= ajj;
+  }
+  for (int i = 0; i < n; i++) {
+    const double aii = A_(i, i);
+    if (i < n - 1) {
+      double dot = 0.0;
+      for (int k = i; k < n; k++) dot += A_(k, i) * A_(k, i);
+      A_(i, i) = dot;
+      for (int jj = 0; jj < i; jj++) {
+        double t = 0.0;
+        for (int k = i + 1; k < n; k++) t += A_(k, jj) * A_(k, i);
+        A_(i, jj) = t + aii * A_(i, jj);
+      }
+    } else {
+      for (int jj = 0; jj <= i; jj++) A_(i, jj) *= aii;
+    }
+  }
+  return 0;
+}
+
+__host__ __device__ inline void h_symv_L(int n, const double *A,
+                                         const double *x, double *y) {
+#pragma clang fp contract(off)
+  for (int i = 0; i < n; i++) y[i] = 0.0;
+  for (int j = 0; j < n; j++) {
+    const double t1 = x[j];
+    double t2 = 0.0;
+    y[j] += t1 * A_(j, j);
+    for (int i = j + 1; i < n; i++) {
+      y[i] += t1 * A_(i, j);
+      t2 += A_(i, j) * x[i];
+    }
+    y[j] += t2;
+  }
+}
+
+__host__ __device__ inline void h_trmv_L(int n, const double *A, double *x) {
+#pragma clang fp contract(off)
+  for (int j = n - 1; j >= 0; j--) {
+    const double t = x[j];
+    if (t != 0.0) {
+      for (int i = n - 1; i > j; i--) x[i] += t * A_(i, j);
+      x[j] = t * A_(j, j);
+    }
+  }
+}
+#undef A_
 
 // ---------------------------------------------------------------------------
 // random numbers with R's argument semantics

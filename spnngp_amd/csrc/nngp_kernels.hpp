@@ -66,7 +66,9 @@ __host__ __device__ inline void nn_closed(int i, int m, int &off, int &cnt) {
 struct BfConfig {
   int family = 0;
   int threads = 256;
-  int mode = 0;          // mixed-precision passes: 1 = covariance copy in LDS
+  int mode = 0;          // mixed-precision passes: 1 = covariance copy in
+                         // LDS; FP64 per-lane passes: 2 = covariate rows
+                         // staged in LDS
 };
 
 struct BfTuneReport {

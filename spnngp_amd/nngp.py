@@ -98,10 +98,11 @@ def nn2(coords, coords0, k: int):
 # ---------------------------------------------------------------------------
 def _shape_name(family, mode):
     """Name of an updateBF launch shape of the autotuner: 'lane' (one site
-    per lane; '+lds-copy' for the mixed form's covariance copy) or 'wave'
+    per lane; '+lds-copy' for the mixed form's covariance copy in LDS,
+    '+lds-z' for the FP64 form's covariate rows staged in LDS) or 'wave'
     (one site per wavefront)."""
     if family == 0:
-        return "lane+lds-copy" if mode else "lane"
+        return ("lane", "lane+lds-copy", "lane+lds-z")[mode]
     return "wave"
 
 
@@ -475,7 +476,8 @@ class Model:
     def set_shape(self, family, threads, mode=0):
         """Pin the updateBF launch shape: family 0 (one site per lane) or 1
         (per wavefront), threads per block, and mode as in the autotuner's
-        report (1: the mixed per-lane kernel's covariance copy in LDS)."""
+        report (1: the mixed per-lane kernel's covariance copy in LDS; 2: the
+        FP64 per-lane kernel's covariate rows staged in LDS)."""
         check(load().nngp_model_set_shape(self._h, int(family), int(threads),
                                           int(mode)))
 

@@ -121,8 +121,8 @@ def test_updateBF_matches_oracle(oracle, nngp, n, m, cov, theta):
     assert abs(ld1 - ld0) / abs(ld0) < RTOL
 
 
-SHAPES = [(0, 64, 0), (0, 128, 0), (0, 256, 0), (1, 64, 0), (1, 128, 0),
-          (1, 256, 0)]
+SHAPES = [(0, 64, 0), (0, 128, 0), (0, 256, 0), (0, 64, 2), (0, 128, 2),
+          (0, 256, 2), (1, 64, 0), (1, 128, 0), (1, 256, 0)]
 
 
 @pytest.mark.gpu
@@ -178,6 +178,53 @@ def test_updateBF_with_a_non_canonical_layout(oracle, nngp):
     assert per_site_B_err(B1, B0, lu2, n) < RTOL
     assert rel_err(F1, F0) < RTOL
     assert abs(ld1 - ld0) < RTOL * abs(ld0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cov,m,p,theta", [
+    (0, 15, 3, [2.0, 0.2, 12.0]),
+    (0, 16, 2, [2.0, 0.2, 12.0]),
+    (1, 12, 1, [1.5, 0.5, 2.0]),
+    (3, 10, 0, [1.0, 0.3, 5.0]),
+    (2, 15, 1, [1.0, 0.3, 6.0, 1.5]),
+    (2, 12, 3, [1.0, 0.3, 6.0, 2.5]),
+])
+def test_lane_lds_staging_is_bitwise_the_plain_lane_kernel(oracle, nngp, cov,
+                                                           m, p, theta):
+    """Mode 2 of the per-lane kernel stages the covariate rows in LDS with
+    direct loads and adds the same terms in the same order: B, F, the log
+    determinant and every cross-product equal mode 0's bit for bit, for row
+    strides of 2 (p <= 1) and 4 doubles (p = 2, 3), at 64 and 128 threads,
+    on a ragged last block; both match the oracle."""
+    n = 4000 + 21
+    print(f"lane lds cov={cov} m={m} p={p}", flush=True)
+    c = uniform_coords(n, 500 + m + p)
+    rng = np.random.default_rng(m + 10 * p)
+    X = np.column_stack([np.ones(n), rng.normal(size=(n, p - 1))]) \
+        if p else None
+    beta = np.linspace(1.0, -1.0, p) if p else np.zeros(1)
+    y = (X @ beta if p else 0.0) + rng.normal(size=n)
+    nn = oracle.mkNNIndxCB(c, m)
+    B0, F0, ld0 = oracle.updateBF(c.T.reshape(-1).copy(), nn["nnIndx"],
+                                  nn["nnIndxLU"], n, m, theta, cov)
+    with nngp.Model(c, nn["nnIndx"], nn["nnIndxLU"], m, cov, X, y) as md:
+        for th in (64, 128):
+            res = []
+            for mode in (0, 2):
+                md.set_shape(0, th, mode)
+                out = md.update(theta, slot=mode // 2, beta=beta)
+                res.append((out, *md.get_bf(mode // 2)))
+            (o0, Ba, Fa), (o2, Bb, Fb) = res
+            assert np.array_equal(Ba, Bb) and np.array_equal(Fa, Fb)
+            assert o0["logDet"] == o2["logDet"]
+            assert o0["qResid"] == o2["qResid"]
+            if p:
+                assert np.array_equal(o0["XtViX"], o2["XtViX"])
+                assert np.array_equal(o0["XtViy"], o2["XtViy"])
+                assert o0["ytViy"] == o2["ytViy"]
+    assert per_site_B_err(Bb, B0, nn["nnIndxLU"], n) < RTOL
+    assert rel_err(Fb, F0) < RTOL
+    assert abs(o2["logDet"] - ld0) < RTOL * abs(ld0)
 
 
 @pytest.mark.gpu
