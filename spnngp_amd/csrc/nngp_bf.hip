@@ -162,11 +162,11 @@ __device__ __forceinline__ void lds_load16(const double *src, double2 *dst) {
 // neighbor count, offset and index slots (slots past the count are not its
 // neighbors; they are replaced by site 0 here, before any use).
 // ZH > 0 stages the rows of Z (zs = 2 ZH doubles) of the M neighbor slots
-// and of the site itself in LDS right after the coordinate gathers, so that
-// they arrive while the system is built and factored: chunk h of slot k
-// (slot M = the site) at stg[(k * ZH + h) * 64 + lane].  ZH is a constant
-// so that the waits before the factorization count the staged loads
-// exactly.  The residuals add the same terms in the same order either way.
+// and of the site itself in LDS with the coordinate gathers, instead of
+// gathering them into registers after the solve, where the wait for them
+// would stall the wavefront: chunk h of slot k (slot M = the site) at
+// stg[(k * ZH + h) * 64 + lane].  The residuals add the same terms in the
+// same order either way.
 template <int M, int COV, bool BATCH, int ZH = 0>
 __device__ __forceinline__ void lane_site(const BfArgs &A, const BfProblem &P,
                                           const CovArgs &c, int i, int cnt,
@@ -185,6 +185,21 @@ __device__ __forceinline__ void lane_site(const BfArgs &A, const BfProblem &P,
     for (int k = 0; k < M; k++)
       if (k >= cnt) nb[k] = 0;
   }
+  // The staged rows are requested before the coordinates: the hardware's
+  // load counter retires in order, so the one wait for the coordinates
+  // covers them too, and nothing waits on memory again until the residuals.
+  if constexpr (ZH > 0) {
+#pragma unroll
+    for (int k = 0; k <= M; k++) {
+      const double *zr = P.Z + (size_t)(k < M ? nb[k] : i) * (2 * ZH);
+#pragma unroll
+      for (int h = 0; h < ZH; h++)
+        lds_load16(zr + 2 * h, stg + (k * ZH + h) * 64);
+    }
+  }
+  // the site's own coordinates first, so that they arrive with the
+  // neighbors' (one wait for both)
+  const double2 s = P.xy[i];
   double px[M], py[M];
 #pragma unroll
   for (int k = 0; k < M; k++) {
@@ -199,16 +214,6 @@ __device__ __forceinline__ void lane_site(const BfArgs &A, const BfProblem &P,
         px[k] = 1e100 * (k + 1);
         py[k] = -1e100 * (k + 1);
       }
-    }
-  }
-  const double2 s = P.xy[i];
-  if constexpr (ZH > 0) {
-#pragma unroll
-    for (int k = 0; k <= M; k++) {
-      const double *zr = P.Z + (size_t)(k < M ? nb[k] : i) * (2 * ZH);
-#pragma unroll
-      for (int h = 0; h < ZH; h++)
-        lds_load16(zr + 2 * h, stg + (k * ZH + h) * 64);
     }
   }
 
