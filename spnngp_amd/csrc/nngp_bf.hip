@@ -149,6 +149,15 @@ __device__ inline void block_site_terms(int nz, double *rows, int rows_cap,
   }
 }
 
+// Covariance entry of the per-lane kernels from a squared distance in the
+// units of lane_site: scaled by tsc for the exponential and the gaussian.
+template <int COV>
+__device__ __forceinline__ double lane_corr(double d2, const CovArgs &c) {
+  if constexpr (COV == 0) return s2_exp2(-nn_sqrt(d2), c);
+  else if constexpr (COV == 3) return s2_exp2(-d2, c);
+  else return s2_corr<COV>(d2, c);
+}
+
 // Direct-to-LDS load of 16 bytes a lane: lane t's bytes land at dst + t
 // (dst the same for the whole wavefront).  The load counts in vmcnt like any
 // other; the data are in LDS once it has been waited for.
@@ -216,16 +225,30 @@ __device__ __forceinline__ void lane_site(const BfArgs &A, const BfProblem &P,
       }
     }
   }
+  // exponential and gaussian: neighbor offsets from the site, in units
+  // where the covariance is s2 2^-d (s2 2^-d^2), see s2_exp2.  Offsetting
+  // first keeps large coordinate values (projected systems) out of the
+  // rounding of the scaled coordinates.
+  constexpr bool kBase2 = COV == 0 || COV == 3;
+  if constexpr (kBase2) {
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+      px[k] = (px[k] - s.x) * c.tsc;
+      py[k] = (py[k] - s.y) * c.tsc;
+    }
+  }
 
   constexpr int T = M * (M + 1) / 2;
   double a[T];  // packed lower triangle
   double v[M];
 #pragma unroll
   for (int k = 0; k < M; k++) {
-    v[k] = s2_corr<COV>(sqdist(s.x, s.y, px[k], py[k]), c);
+    v[k] = lane_corr<COV>(kBase2 ? fma(px[k], px[k], py[k] * py[k])
+                                 : sqdist(s.x, s.y, px[k], py[k]),
+                          c);
 #pragma unroll
     for (int l = 0; l < k; l++)
-      a[tri(k, l)] = s2_corr<COV>(sqdist(px[k], py[k], px[l], py[l]), c);
+      a[tri(k, l)] = lane_corr<COV>(sqdist(px[k], py[k], px[l], py[l]), c);
     a[tri(k, k)] = c.diag;
   }
   double vv;

@@ -71,10 +71,17 @@ CovArgs make_cov(int cov, double s2, double t2, double phi, double nu) {
   c.ln2_lo = (double)(ln2 - (long double)hi);
   c.log2e = (double)(1.0L / ln2);
   double fact = 1.0;
+  long double l2k = 1.0L;  // (ln 2)^k
   for (int k = 0; k <= kExpDeg; k++) {
-    if (k > 0) fact *= k;
+    if (k > 0) {
+      fact *= k;
+      l2k *= ln2;
+    }
     c.ec[k] = s2 / fact;
+    c.ec2[k] = (double)((long double)s2 * l2k / (long double)fact);
   }
+  c.tsc = cov == 3 ? (double)((long double)phi / sqrtl(ln2))
+                   : (double)((long double)phi / ln2);
   if (cov == 2) {
     const int nl = (int)(nu + 0.5);
     const double mu = nu - nl;

@@ -228,6 +228,35 @@ def test_lane_lds_staging_is_bitwise_the_plain_lane_kernel(oracle, nngp, cov,
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("cov,theta", [(0, [2.0, 0.2, 12.0e-4]),
+                                       (3, [2.0, 0.2, 3.0e-4])])
+def test_projected_coordinates_keep_parity(oracle, nngp, cov, theta):
+    """Sites in a projected system: a 1e4 x 1e4 window at an offset of
+    (5e5, 4e6), where the coordinates carry 10 more significant digits than
+    the distances.  The per-lane kernels form neighbor offsets from the site
+    before scaling (the exponential and gaussian run in base-2 units), so B,
+    F and the log determinant keep the parity bound in both launch modes."""
+    n, m = 3000 + 5, 15
+    print(f"projected cov={cov}", flush=True)
+    c = uniform_coords(n, 808) * 1e4 + np.array([5e5, 4e6])
+    c = c[np.argsort(c[:, 0], kind="stable")]
+    nn = oracle.mkNNIndxCB(c, m)
+    B0, F0, ld0 = oracle.updateBF(c.T.reshape(-1).copy(), nn["nnIndx"],
+                                  nn["nnIndxLU"], n, m, theta, cov)
+    rng = np.random.default_rng(8)
+    X = np.column_stack([np.ones(n), rng.normal(size=(n, 2))])
+    y = X @ [1.0, 0.5, -1.0] + rng.normal(size=n)
+    with nngp.Model(c, nn["nnIndx"], nn["nnIndxLU"], m, cov, X, y) as md:
+        for mode in (0, 2):
+            md.set_shape(0, 64, mode)
+            out = md.update(theta, slot=0, beta=[1.0, 0.5, -1.0])
+            B, F = md.get_bf(0)
+            assert per_site_B_err(B, B0, nn["nnIndxLU"], n) < RTOL
+            assert rel_err(F, F0) < RTOL
+            assert abs(out["logDet"] - ld0) < RTOL * abs(ld0)
+
+
+@pytest.mark.gpu
 def test_set_shape_rejects_shapes_that_do_not_apply(nngp):
     n, m = 500, 20
     c = uniform_coords(n, 9)
