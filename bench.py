@@ -28,11 +28,12 @@ sys.path.insert(0, ROOT)
 METRIC = "MCMC iter/s @ n=1e6,m=15 (FP64 vs mixed) + kriging p50/p99 latency"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per launch of the FP64 updateBF pass at n=1e6, m=15: two
+# HBM bytes per launch of the FP64 updateBF pass at n=1e6, m=15 (the
+# per-lane kernel with covariate rows staged in LDS, 64-thread blocks): two
 # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this script, each scaled
 # by what the counter reads for a copy kernel of known size (committed
 # summary: profiles/updateBF_rNNGP_n1e6_m15_pmc_traffic.md)
-BF_TRAFFIC_BYTES = 127_247_134
+BF_TRAFFIC_BYTES = 119_147_922
 
 
 def f_alg(m):
@@ -698,7 +699,8 @@ def main():
                      "bound_note": "FP64 compute: the kernel runs on the "
                                    "vector FMA units, whose FP64 peak equals "
                                    "the FP64 matrix peak (78.6 TFLOP/s)",
-                     "kernel": "updateBF (k_bf_lane<15,exp>, %s %d threads)"
+                     "kernel": "updateBF (per-lane kernel, m=15, "
+                               "exponential, %s %d threads)"
                                % tuple(f64["tune"]["chosen"]),
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
