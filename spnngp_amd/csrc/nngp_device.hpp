@@ -15,6 +15,7 @@
 namespace nngp {
 
 constexpr int kExpDeg = 11;  // Taylor degree of exp on |r| <= ln2/2
+constexpr int kExp2Deg = 10; // degree of the 2^f polynomial on |f| <= 1/2
 
 struct CovArgs {
   double s2;       // sigma.sq (1 on the correlation scale)
@@ -28,8 +29,9 @@ struct CovArgs {
   double ln2_hi;   // ln 2 split so that k * ln2_hi is exact for |k| < 2^20
   double ln2_lo;
   double ec[kExpDeg + 1];  // s2 / k!: exp polynomial with the scale folded in
-  double ec2[kExpDeg + 1]; // s2 (ln 2)^k / k!: the same polynomial in f,
-                           // 2^f = exp(f ln 2)
+  double ec2[kExpDeg + 1]; // s2 times the coefficients of the degree-10
+                           // polynomial of 2^f on |f| <= 1/2 (Chebyshev-node
+                           // interpolant); entry 11 unused
   double tsc;      // coordinate scale of the per-lane updateBF kernels:
                    // phi / ln 2 (exponential), phi / sqrt(ln 2) (gaussian)
   // matern (Temme's method; see matern_s2 below)
@@ -81,19 +83,20 @@ __device__ __forceinline__ double s2_exp(double x, const CovArgs &a) {
 }
 
 // s2 * 2^t for t <= 0 (large negative t gives 0): t = k + f with k the
-// nearest integer, so |f| <= 1/2 and f = t - k is exact, then the same
-// degree-11 Taylor polynomial as s2_exp in r = f ln 2, its coefficients
-// s2 (ln 2)^j / j! (relative error ~1e-15).  With t = -phi d / ln 2 (or
-// -phi^2 d^2 / ln 2) formed from coordinates scaled by tsc, this is
-// s2_corr of the exponential (gaussian) covariance without the products
-// by phi and 1 / ln 2 and without the two-step reduction.
+// nearest integer, so |f| <= 1/2 and f = t - k is exact, then a degree-10
+// polynomial of 2^f, the interpolant at the Chebyshev nodes of [-1/2, 1/2]
+// with s2 folded into its coefficients (relative error 4.5e-16 before
+// rounding).  With t = -phi d / ln 2 (or -phi^2 d^2 / ln 2) formed from
+// coordinates scaled by tsc, this is s2_corr of the exponential (gaussian)
+// covariance without the products by phi and 1 / ln 2, without the
+// two-step reduction, and with one multiply-add fewer than s2_exp.
 __device__ __forceinline__ double s2_exp2(double t, const CovArgs &a) {
   t = fmax(t, -1500.0);
   const double k = __builtin_rint(t);
   const double f = t - k;
-  double p = a.ec2[kExpDeg];
+  double p = a.ec2[kExp2Deg];
 #pragma unroll
-  for (int j = kExpDeg - 1; j >= 0; j--) p = fma(p, f, a.ec2[j]);
+  for (int j = kExp2Deg - 1; j >= 0; j--) p = fma(p, f, a.ec2[j]);
   return ldexp(p, (int)k);
 }
 

@@ -50,6 +50,49 @@ static const double kRg[] = {0.5772156649015329,  -0.0420026350340952,
                              -0.0002152416741149, -0.0000201348547807,
                              0.0000011330272320};
 
+// Coefficients of the degree-10 polynomial that interpolates 2^x at the 11
+// Chebyshev nodes of [-1/2, 1/2] (Gaussian elimination on the Vandermonde
+// system in long double).  Interpolation at Chebyshev nodes is within a
+// small factor of the minimax polynomial: relative error 4.5e-16 on the
+// interval, against 8.7e-15 for the degree-11 Taylor polynomial of exp(f ln 2).
+static const long double *exp2_cheb() {
+  static long double c[kExp2Deg + 1];
+  static const bool done = [] {
+    constexpr int n = kExp2Deg + 1;
+    long double A[n][n], b[n];
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int k = 0; k < n; k++) {
+      const long double x = 0.5L * cosl((2 * k + 1) * pi / (2 * n));
+      long double t = 1.0L;
+      for (int j = 0; j < n; j++) {
+        A[k][j] = t;
+        t *= x;
+      }
+      b[k] = exp2l(x);
+    }
+    for (int i = 0; i < n; i++) {
+      int pv = i;
+      for (int r = i + 1; r < n; r++)
+        if (fabsl(A[r][i]) > fabsl(A[pv][i])) pv = r;
+      for (int j = 0; j < n; j++) std::swap(A[i][j], A[pv][j]);
+      std::swap(b[i], b[pv]);
+      for (int r = i + 1; r < n; r++) {
+        const long double f = A[r][i] / A[i][i];
+        for (int j = i; j < n; j++) A[r][j] -= f * A[i][j];
+        b[r] -= f * b[i];
+      }
+    }
+    for (int i = n - 1; i >= 0; i--) {
+      long double acc = b[i];
+      for (int j = i + 1; j < n; j++) acc -= A[i][j] * c[j];
+      c[i] = acc / A[i][i];
+    }
+    return true;
+  }();
+  (void)done;
+  return c;
+}
+
 CovArgs make_cov(int cov, double s2, double t2, double phi, double nu) {
   CovArgs c;
   std::memset(&c, 0, sizeof c);
@@ -71,14 +114,11 @@ CovArgs make_cov(int cov, double s2, double t2, double phi, double nu) {
   c.ln2_lo = (double)(ln2 - (long double)hi);
   c.log2e = (double)(1.0L / ln2);
   double fact = 1.0;
-  long double l2k = 1.0L;  // (ln 2)^k
+  const long double *ch = exp2_cheb();
   for (int k = 0; k <= kExpDeg; k++) {
-    if (k > 0) {
-      fact *= k;
-      l2k *= ln2;
-    }
+    if (k > 0) fact *= k;
     c.ec[k] = s2 / fact;
-    c.ec2[k] = (double)((long double)s2 * l2k / (long double)fact);
+    c.ec2[k] = k <= kExp2Deg ? (double)((long double)s2 * ch[k]) : 0.0;
   }
   c.tsc = cov == 3 ? (double)((long double)phi / sqrtl(ln2))
                    : (double)((long double)phi / ln2);

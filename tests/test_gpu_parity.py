@@ -342,10 +342,13 @@ def make_response_data(n, p, seed, cov=0):
     return c, X, y
 
 
-@pytest.mark.parametrize("cov,nu", [(0, 0.0), (2, 1.0)])
-def test_rNNGP_chain_identical_to_oracle(oracle, nngp, cov, nu):
-    n, m, p, S = 400, 10, 2, 200
-    print(f"rNNGP chain cov={cov}", flush=True)
+# Exponential (a short and a longer chain), gaussian and matern; the tape
+# positions check that the same number of normals and uniforms were drawn.
+@pytest.mark.parametrize("cov,nu,S", [(0, 0.0, 200), (0, 0.0, 600),
+                                      (3, 0.0, 300), (2, 1.0, 200)])
+def test_rNNGP_chain_identical_to_oracle(oracle, nngp, cov, nu, S):
+    n, m, p = 400, 10, 2
+    print(f"rNNGP chain cov={cov} S={S}", flush=True)
     c, X, y = make_response_data(n, p, 42)
     nn = oracle.mkNNIndx(c, m)
     rng = np.random.default_rng(99)
@@ -358,13 +361,14 @@ def test_rNNGP_chain_identical_to_oracle(oracle, nngp, cov, nu):
                 sigmaSqTuning=0.3, tauSqTuning=0.3, phiTuning=0.3,
                 nuTuning=0.2, nSamples=S)
     flat = c.T.reshape(-1).copy()
+    ref_tape = oracle.TapeRng(normals, unifs)
     ref = oracle.rNNGP(y, X, p, n, m, flat, cov, nn["nnIndx"],
-                       nn["nnIndxLU"], rng=oracle.TapeRng(normals, unifs),
-                       **args)
+                       nn["nnIndxLU"], rng=ref_tape, **args)
     tape = oracle.TapeRng(normals, unifs)
     got = nngp.rNNGP(y, X, p, n, m, c, cov, nn["nnIndx"], nn["nnIndxLU"],
                      rng=tape, **args)
     # same number of draws consumed, same accept count, same chain
+    assert (tape.i_n, tape.i_u) == (ref_tape.i_n, ref_tape.i_u)
     assert got["accept"] == ref["accept"]
     assert 0 < ref["accept"] < S
     th0, th1 = ref["p.theta.samples"], got["p.theta.samples"]
@@ -373,6 +377,47 @@ def test_rNNGP_chain_identical_to_oracle(oracle, nngp, cov, nu):
     assert np.array_equal(acc0, acc1)
     assert rel_err(th1, th0) < 1e-9
     assert rel_norm_err(got["p.beta.samples"], ref["p.beta.samples"]) < 1e-9
+
+
+def test_rNNGP_verbose_reports(oracle, nngp, capfd):
+    """The chain prints the reference's progress reports
+    (src/rNNGP.cpp:360-373): at s = 100, 200 of 300, the interval rate
+    counting iterations since the last report (the first interval has 101,
+    as in the reference), then the final lines; every rate agrees with the
+    accepted moves visible in the theta samples."""
+    n, m, p, S = 300, 8, 2, 300
+    c, X, y = make_response_data(n, p, 5)
+    nn = oracle.mkNNIndx(c, m)
+    beta0 = np.linalg.lstsq(X, y, rcond=None)[0]
+    got = nngp.rNNGP(y, X, p, n, m, c, 0, nn["nnIndx"], nn["nnIndxLU"],
+                     sigmaSqIG=[2.0, 5.0], tauSqIG=[2.0, 1.0],
+                     phiUnif=[3.0, 30.0], nuUnif=None, betaStarting=beta0,
+                     sigmaSqStarting=5.0, tauSqStarting=1.0, phiStarting=6.0,
+                     nuStarting=0.0, sigmaSqTuning=0.3, tauSqTuning=0.3,
+                     phiTuning=0.3, nuTuning=0.0, nSamples=S, verbose=1,
+                     nReport=100, seed=3)
+    out = capfd.readouterr().out
+    th = got["p.theta.samples"]
+    prev = np.r_[[5.0, 1.0, 6.0]][:, None]
+    moved = np.any(th != np.c_[prev, th[:, :-1]], axis=0)
+    cum = np.cumsum(moved)
+    assert cum[-1] == got["accept"]
+    want = []
+    last = 0
+    for s in (100, 200):
+        want += ["Sampled: %i of %i, %3.2f%%" % (s, S, 100.0 * s / S),
+                 "Report interval Metrop. Acceptance rate: %3.2f%%"
+                 % (100.0 * (cum[s] - last) / 100),
+                 "Overall Metrop. Acceptance rate: %3.2f%%"
+                 % (100.0 * cum[s] / s),
+                 "-------------------------------------------------"]
+        last = cum[s]
+    want += ["Sampled: %i of %i, %3.2f%%" % (S, S, 100.0),
+             "Overall Metrop. Acceptance rate: %3.2f%%"
+             % (100.0 * got["accept"] / S)]
+    lines = out.splitlines()
+    start = lines.index(want[0])
+    assert lines[start:start + len(want)] == want
 
 
 def test_rNNGP_fit_rep_matches_oracle(oracle, nngp):
