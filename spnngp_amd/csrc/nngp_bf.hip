@@ -153,9 +153,21 @@ __device__ inline void block_site_terms(int nz, double *rows, int rows_cap,
 // units of lane_site: scaled by tsc for the exponential and the gaussian.
 template <int COV>
 __device__ __forceinline__ double lane_corr(double d2, const CovArgs &c) {
-  if constexpr (COV == 0) return s2_exp2(-nn_sqrt(d2), c);
-  else if constexpr (COV == 3) return s2_exp2(-d2, c);
-  else return s2_corr<COV>(d2, c);
+  constexpr double kLn2 = 0.6931471805599453;  // phi d = d' ln 2
+  if constexpr (COV == 0) {
+    return s2_exp2(-nn_sqrt(d2), c);
+  } else if constexpr (COV == 3) {
+    return s2_exp2(-d2, c);
+  } else if constexpr (COV == kCovMatern15) {
+    const double dd = nn_sqrt(d2);
+    return s2_exp2(-dd, c) * (1.0 + dd * kLn2);
+  } else if constexpr (COV == kCovMatern25) {
+    const double dd = nn_sqrt(d2);
+    const double x = dd * kLn2;
+    return s2_exp2(-dd, c) * (1.0 + x + x * x * (1.0 / 3.0));
+  } else {
+    return s2_corr<COV>(d2, c);
+  }
 }
 
 // Direct-to-LDS load of 16 bytes a lane: lane t's bytes land at dst + t
@@ -225,11 +237,12 @@ __device__ __forceinline__ void lane_site(const BfArgs &A, const BfProblem &P,
       }
     }
   }
-  // exponential and gaussian: neighbor offsets from the site, in units
-  // where the covariance is s2 2^-d (s2 2^-d^2), see s2_exp2.  Offsetting
-  // first keeps large coordinate values (projected systems) out of the
-  // rounding of the scaled coordinates.
-  constexpr bool kBase2 = COV == 0 || COV == 3;
+  // exponential, gaussian and the closed-form materns: neighbor offsets
+  // from the site, in units where the exponential factor is 2^-d (2^-d^2),
+  // see s2_exp2.  Offsetting first keeps large coordinate values (projected
+  // systems) out of the rounding of the scaled coordinates.
+  constexpr bool kBase2 = COV == 0 || COV == 3 || COV == kCovMatern15 ||
+                          COV == kCovMatern25;
   if constexpr (kBase2) {
 #pragma unroll
     for (int k = 0; k < M; k++) {
@@ -323,18 +336,18 @@ __global__ __launch_bounds__(kLaneMaxThreads) void k_bf_lane(BfArgs A) {
   if (P.rows) block_site_terms(A.nz, P.rows, A.rows_cap, valid, Fi, e, b);
 }
 
-// k_bf_lane (one problem) with the covariate rows staged in LDS: the
-// block's dynamic LDS holds, per wavefront, (M + 1) x ZH chunks of 64 x 16
-// bytes.  The launcher uses it only when Z is set, zs = 2 ZH and Z is
-// 16-byte aligned.
-template <int M, int COV, int ZH>
+// k_bf_lane with the covariate rows staged in LDS: the block's dynamic LDS
+// holds, per wavefront, (M + 1) x ZH chunks of 64 x 16 bytes.  The launcher
+// uses it only when Z is set, zs = 2 ZH and every problem's Z is 16-byte
+// aligned.
+template <int M, int COV, int ZH, bool BATCH>
 __global__ __launch_bounds__(kLaneMaxThreads) void k_bf_lane_lds(BfArgs A) {
   extern __shared__ double2 zstage[];
   const int nt = (int)blockDim.x;
   const int b = xcd_block((A.n + nt - 1) / nt);
   if (b < 0) return;  // whole padding block
-  const BfProblem P = bf_problem<false>(A);
-  const CovArgs c = A.c;
+  const BfProblem P = bf_problem<BATCH>(A);
+  const CovArgs c = BATCH ? A.covs[(int)blockIdx.y % A.ng] : A.c;
   const int i = b * nt + (int)threadIdx.x;
   const bool valid = i < A.n;
   const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
@@ -355,8 +368,8 @@ __global__ __launch_bounds__(kLaneMaxThreads) void k_bf_lane_lds(BfArgs A) {
     int nb[M];
 #pragma unroll
     for (int k = 0; k < M; k++) nb[k] = P.nnIndx[off + k];
-    lane_site<M, COV, false, ZH>(A, P, c, i, cnt, off, nb, Fi, e, stg);
-    if (A.F) A.F[i] = Fi;
+    lane_site<M, COV, BATCH, ZH>(A, P, c, i, cnt, off, nb, Fi, e, stg);
+    if (!BATCH && A.F) A.F[i] = Fi;
   }
   if (P.rows) block_site_terms(A.nz, P.rows, A.rows_cap, valid, Fi, e, b);
 }
@@ -973,33 +986,35 @@ static size_t lane_lds_shm(int m, int zs, int threads) {
          sizeof(double2);
 }
 
-// Whether an FP64 single-problem pass can stage its covariate rows in LDS
-// (mode 2: m > 8, where the factorization is long enough to cover the
-// loads); where it cannot, mode 2 runs the plain per-lane kernel, whose
-// results are the same bit for bit.
+// Whether an FP64 pass can stage its covariate rows in LDS (mode 2: m > 8,
+// where the factorization is long enough to cover the loads; batched
+// passes with zs = 4 only); where it cannot, mode 2 runs the plain
+// per-lane kernel, whose results are the same bit for bit.
 static bool lane_lds_applies(const BfArgs &a, int threads) {
-  return !a.mixed && a.ng == 0 && a.Z && a.nz > 0 && a.m > 8 &&
-         (a.zs == 2 || a.zs == 4) && ((uintptr_t)a.Z & 15) == 0 &&
+  return !a.mixed && a.Z && a.nz > 0 && a.m > 8 &&
+         (a.zs == 4 || (a.zs == 2 && a.ng == 0)) &&
+         ((uintptr_t)a.Z & 15) == 0 && (a.ng == 0 || a.fs_Z % 2 == 0) &&
          lane_lds_shm(a.m, a.zs, threads) <= 65536;
 }
 
-template <int M, int ZH>
+template <int M, int ZH, bool BATCH>
 static void launch_lane_lds_mh(const BfArgs &a, int variant, const dim3 &grid,
                                const dim3 &blk, size_t shm, hipStream_t st) {
   switch (variant) {
-    case 0: hipLaunchKernelGGL((k_bf_lane_lds<M, 0, ZH>), grid, blk, shm, st, a); break;
-    case 1: hipLaunchKernelGGL((k_bf_lane_lds<M, 1, ZH>), grid, blk, shm, st, a); break;
-    case kCovMatern15: hipLaunchKernelGGL((k_bf_lane_lds<M, kCovMatern15, ZH>), grid, blk, shm, st, a); break;
-    case kCovMatern25: hipLaunchKernelGGL((k_bf_lane_lds<M, kCovMatern25, ZH>), grid, blk, shm, st, a); break;
-    default: hipLaunchKernelGGL((k_bf_lane_lds<M, 3, ZH>), grid, blk, shm, st, a); break;
+    case 0: hipLaunchKernelGGL((k_bf_lane_lds<M, 0, ZH, BATCH>), grid, blk, shm, st, a); break;
+    case 1: hipLaunchKernelGGL((k_bf_lane_lds<M, 1, ZH, BATCH>), grid, blk, shm, st, a); break;
+    case kCovMatern15: hipLaunchKernelGGL((k_bf_lane_lds<M, kCovMatern15, ZH, BATCH>), grid, blk, shm, st, a); break;
+    case kCovMatern25: hipLaunchKernelGGL((k_bf_lane_lds<M, kCovMatern25, ZH, BATCH>), grid, blk, shm, st, a); break;
+    default: hipLaunchKernelGGL((k_bf_lane_lds<M, 3, ZH, BATCH>), grid, blk, shm, st, a); break;
   }
 }
 
 template <int M>
 static void launch_lane_lds_m(const BfArgs &a, int variant, const dim3 &grid,
                               const dim3 &blk, size_t shm, hipStream_t st) {
-  if (a.zs == 2) launch_lane_lds_mh<M, 1>(a, variant, grid, blk, shm, st);
-  else launch_lane_lds_mh<M, 2>(a, variant, grid, blk, shm, st);
+  if (a.ng > 0) launch_lane_lds_mh<M, 2, true>(a, variant, grid, blk, shm, st);
+  else if (a.zs == 2) launch_lane_lds_mh<M, 1, false>(a, variant, grid, blk, shm, st);
+  else launch_lane_lds_mh<M, 2, false>(a, variant, grid, blk, shm, st);
 }
 
 static void launch_lane_lds(const BfArgs &a, int variant, const dim3 &grid,
@@ -1101,7 +1116,12 @@ int bf_launch(const BfArgs &a0, const BfConfig &cfg, int nprob,
   const dim3 grid(8 * ((nrows + 7) / 8), nprob);
   const dim3 blk(cfg.threads);
   if (cfg.family == 0) {
-    if (batch) launch_lane<true>(a, variant, grid, blk, st);
+    if (batch) {
+      if (cfg.mode == 2 && lane_lds_applies(a, cfg.threads))
+        launch_lane_lds(a, variant, grid, blk, st);
+      else
+        launch_lane<true>(a, variant, grid, blk, st);
+    }
     else if (a.mixed) launch_mixed(a, variant, cfg.mode == 1, grid, blk, st);
     else if (cfg.mode == 2 && lane_lds_applies(a, cfg.threads))
       launch_lane_lds(a, variant, grid, blk, st);

@@ -875,9 +875,12 @@ int nngp_cvgrid_create(nngp_cvgrid **out, const double *y, const double *X,
   }
   if (opts && opts->shape[1] > 0) {
     const BfConfig pin{opts->shape[0], opts->shape[1], opts->shape[2]};
-    // a matern grid takes the per-lane shapes at nu = 0.5, 1.5, 2.5
+    // a matern grid takes the per-lane shapes at nu = 0.5, 1.5, 2.5; the
+    // grid is FP64, so mode 2 (covariate rows staged in LDS) is the only
+    // per-lane mode besides 0
     const int variant = covModel == 2 ? kCovMatern15 : covModel;
-    if (!bf_config_ok(pin, m, variant) || (pin.family == 0 && pin.mode)) {
+    if (!bf_config_ok(pin, m, variant) ||
+        (pin.family == 0 && pin.mode != 0 && pin.mode != 2)) {
       delete c;
       return set_err(NNGP_ERR_ARG,
                      "launch shape (%d, %d, %d) does not apply to this grid",

@@ -88,7 +88,8 @@ def test_cvgrid_matches_oracle_per_fold(oracle, nngp, cov, nu, m):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("shape", [(0, 64, 0), (0, 256, 0), (1, 128, 0)])
+@pytest.mark.parametrize("shape", [(0, 64, 0), (0, 256, 0), (0, 64, 2),
+                                   (0, 128, 2), (1, 128, 0)])
 def test_cvgrid_every_pinned_shape_matches_oracle(oracle, nngp, shape):
     """A grid pinned to an updateBF launch shape (per lane, per wavefront)
     gives the oracle's per-fold solution and skips the autotuner."""
@@ -130,7 +131,9 @@ def test_cvgrid_autotuner_reports_candidates(nngp):
         r = cv.run(ta)
     tune = r["tune"]
     kinds = {t[0] for t in tune}
-    assert kinds == {"lane", "wave"} and len(tune) == 6
+    # per lane at 64/128/256 threads, staged at 64/128 (256 would need
+    # 128 KiB of LDS a block), per wavefront at 1/2/4 waves
+    assert kinds == {"lane", "lane+lds-z", "wave"} and len(tune) == 8
     best = min(tune, key=lambda t: t[2])
     assert r["bf_config"] == (best[0], best[1])
     assert r["graph_ms"] > 0
