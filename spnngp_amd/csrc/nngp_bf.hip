@@ -149,27 +149,6 @@ __device__ inline void block_site_terms(int nz, double *rows, int rows_cap,
   }
 }
 
-// Covariance entry of the per-lane kernels from a squared distance in the
-// units of lane_site: scaled by tsc for the exponential and the gaussian.
-template <int COV>
-__device__ __forceinline__ double lane_corr(double d2, const CovArgs &c) {
-  constexpr double kLn2 = 0.6931471805599453;  // phi d = d' ln 2
-  if constexpr (COV == 0) {
-    return s2_exp2(-nn_sqrt(d2), c);
-  } else if constexpr (COV == 3) {
-    return s2_exp2(-d2, c);
-  } else if constexpr (COV == kCovMatern15) {
-    const double dd = nn_sqrt(d2);
-    return s2_exp2(-dd, c) * (1.0 + dd * kLn2);
-  } else if constexpr (COV == kCovMatern25) {
-    const double dd = nn_sqrt(d2);
-    const double x = dd * kLn2;
-    return s2_exp2(-dd, c) * (1.0 + x + x * x * (1.0 / 3.0));
-  } else {
-    return s2_corr<COV>(d2, c);
-  }
-}
-
 // Direct-to-LDS load of 16 bytes a lane: lane t's bytes land at dst + t
 // (dst the same for the whole wavefront).  The load counts in vmcnt like any
 // other; the data are in LDS once it has been waited for.
@@ -241,8 +220,7 @@ __device__ __forceinline__ void lane_site(const BfArgs &A, const BfProblem &P,
   // from the site, in units where the exponential factor is 2^-d (2^-d^2),
   // see s2_exp2.  Offsetting first keeps large coordinate values (projected
   // systems) out of the rounding of the scaled coordinates.
-  constexpr bool kBase2 = COV == 0 || COV == 3 || COV == kCovMatern15 ||
-                          COV == kCovMatern25;
+  constexpr bool kBase2 = lane_base2<COV>();
   if constexpr (kBase2) {
 #pragma unroll
     for (int k = 0; k < M; k++) {

@@ -233,14 +233,25 @@ __global__ __launch_bounds__(kCvPredThreads) void k_cv_pred_lane(
       }
     }
     const double2 s0 = A.xy0[(size_t)f * A.nho + i];
+    // offsets to the held-out site in base-2 units (lane_corr)
+    constexpr bool kBase2 = lane_base2<COV>();
+    if constexpr (kBase2) {
+#pragma unroll
+      for (int k = 0; k < M; k++) {
+        px[k] = (px[k] - s0.x) * c.tsc;
+        py[k] = (py[k] - s0.y) * c.tsc;
+      }
+    }
     constexpr int T = M * (M + 1) / 2;
     double a[T], v[M];
 #pragma unroll
     for (int k = 0; k < M; k++) {
-      v[k] = s2_corr<COV>(sqdist(s0.x, s0.y, px[k], py[k]), c);
+      v[k] = lane_corr<COV>(kBase2 ? fma(px[k], px[k], py[k] * py[k])
+                                   : sqdist(s0.x, s0.y, px[k], py[k]),
+                            c);
 #pragma unroll
       for (int l = 0; l < k; l++)
-        a[tri(k, l)] = s2_corr<COV>(sqdist(px[k], py[k], px[l], py[l]), c);
+        a[tri(k, l)] = lane_corr<COV>(sqdist(px[k], py[k], px[l], py[l]), c);
       a[tri(k, k)] = c.diag;
     }
     double vv;

@@ -359,6 +359,35 @@ __device__ __forceinline__ double s2_corr(double d2, const CovArgs &a) {
   }
 }
 
+// The per-lane kernels build the exponential, gaussian and closed-form
+// matern covariances from neighbor offsets to the site scaled by tsc, so
+// that the exponential factor is 2^-d (2^-d^2) for s2_exp2.
+template <int COV>
+constexpr bool lane_base2() {
+  return COV == 0 || COV == 3 || COV == kCovMatern15 || COV == kCovMatern25;
+}
+
+// Covariance entry of the per-lane kernels from a squared distance in those
+// units (scaled by tsc where lane_base2<COV>(), as given otherwise).
+template <int COV>
+__device__ __forceinline__ double lane_corr(double d2, const CovArgs &c) {
+  constexpr double kLn2 = 0.6931471805599453;  // phi d = d' ln 2
+  if constexpr (COV == 0) {
+    return s2_exp2(-nn_sqrt(d2), c);
+  } else if constexpr (COV == 3) {
+    return s2_exp2(-d2, c);
+  } else if constexpr (COV == kCovMatern15) {
+    const double dd = nn_sqrt(d2);
+    return s2_exp2(-dd, c) * (1.0 + dd * kLn2);
+  } else if constexpr (COV == kCovMatern25) {
+    const double dd = nn_sqrt(d2);
+    const double x = dd * kLn2;
+    return s2_exp2(-dd, c) * (1.0 + x + x * x * (1.0 / 3.0));
+  } else {
+    return s2_corr<COV>(d2, c);
+  }
+}
+
 // Runtime-dispatched variant for the generic (non-templated) kernels.
 __device__ inline double s2_corr_rt(double d2, const CovArgs &a) {
   switch (a.cov) {

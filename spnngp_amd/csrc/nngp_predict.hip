@@ -27,19 +27,20 @@ constexpr int kPredBlock = 256;
 constexpr int kPredMaxM = 16;
 constexpr int kPredMaxP = kMaxP;
 
-// correlation from a distance (not squared) with unit scale; ip = 1/phi
+// correlation from a distance (not squared) with unit scale; ip = 1/phi,
+// phs = phi times the unit record's tsc (the base-2 scale of s2_exp2)
 template <int COV>
 __device__ __forceinline__ double corr_d(double D, double phi, double ip,
-                                         const CovArgs &u) {
+                                         double phs, const CovArgs &u) {
   if constexpr (COV == 0) {
-    return s2_exp(-phi * D, u);
+    return s2_exp2(-phs * D, u);  // exp(-phi D) = 2^(-phi D / ln 2)
   } else if constexpr (COV == 1) {
     const double t = phi * D;
     const double v = 1.0 - 1.5 * t + 0.5 * (t * t * t);
     return (D > 0.0 && D <= ip) ? v : (D >= ip ? 0.0 : 1.0);
   } else {
-    const double t = phi * D;
-    return s2_exp(-(t * t), u);
+    const double t = phs * D;  // phi D / sqrt(ln 2)
+    return s2_exp2(-(t * t), u);
   }
 }
 
@@ -145,17 +146,18 @@ __global__ __launch_bounds__(kPredBlock) void k_pred_lane(PredArgs A) {
   const double *th = A.theta + (size_t)s * A.nTheta;
   const double s2 = th[0], t2 = th[1], phi = th[2];
   const double ip = 1.0 / phi;
+  const double phs = phi * A.unit.tsc;  // the unit record's phi is 1
   const bool latent = A.wS != nullptr;
   const double nug = latent ? 0.0 : t2;
   constexpr int T = M * (M + 1) / 2;
   double a[T], v[M];
 #pragma unroll
   for (int k = 0; k < M; k++) {
-    v[k] = s2 * corr_d<COV>(sD0[k], phi, ip, A.unit);
+    v[k] = s2 * corr_d<COV>(sD0[k], phi, ip, phs, A.unit);
 #pragma unroll
     for (int l = 0; l < k; l++)
       a[tri(k, l)] =
-          s2 * corr_d<COV>(sD[k * (k - 1) / 2 + l], phi, ip, A.unit);
+          s2 * corr_d<COV>(sD[k * (k - 1) / 2 + l], phi, ip, phs, A.unit);
     a[tri(k, k)] = s2 * 1.0 + nug;
   }
   double vv;
@@ -333,15 +335,26 @@ __global__ __launch_bounds__(kPredBlock) void k_conj_pred_lane(ConjPredArgs A) {
     }
   }
   const double2 s0 = A.xy0[i];
+  // offsets to the new site in base-2 units (lane_corr)
+  constexpr bool kBase2 = lane_base2<COV>();
+  if constexpr (kBase2) {
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+      px[k] = (px[k] - s0.x) * c.tsc;
+      py[k] = (py[k] - s0.y) * c.tsc;
+    }
+  }
   constexpr int T = M * (M + 1) / 2;
   double a[T], v[M], c0[M];
 #pragma unroll
   for (int k = 0; k < M; k++) {
-    c0[k] = s2_corr<COV>(sqdist(s0.x, s0.y, px[k], py[k]), c);
+    c0[k] = lane_corr<COV>(kBase2 ? fma(px[k], px[k], py[k] * py[k])
+                                  : sqdist(s0.x, s0.y, px[k], py[k]),
+                           c);
     v[k] = c0[k];
 #pragma unroll
     for (int l = 0; l < k; l++)
-      a[tri(k, l)] = s2_corr<COV>(sqdist(px[k], py[k], px[l], py[l]), c);
+      a[tri(k, l)] = lane_corr<COV>(sqdist(px[k], py[k], px[l], py[l]), c);
     a[tri(k, k)] = c.diag;
   }
   double vv;
