@@ -33,7 +33,7 @@ PEAK_HBM_GBS = 8000.0
 # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this script, each scaled
 # by what the counter reads for a copy kernel of known size (committed
 # summary: profiles/updateBF_rNNGP_n1e6_m15_pmc_traffic.md)
-BF_TRAFFIC_BYTES = 119_147_922
+BF_TRAFFIC_BYTES = 119_137_416
 
 
 def f_alg(m):
