@@ -106,6 +106,9 @@ struct SweepArgs {
   int n, p, ws;             // w[i * ws]
   int ph0, ph1;             // phases [ph0, ph1)
   int nph;                  // phases in the schedule (phase[] has nph + 1)
+  // k_resid_init: F of the slot times this, the slot's F at the current
+  // sigma.sq (F is proportional to sigma.sq without a nugget)
+  double fscale = 1.0;
 };
 
 // sum_k B[o+k] w[nnIndx[o+k]] over k < cnt, skipping the entry equal to
@@ -244,8 +247,9 @@ __device__ void sweep_site(const SweepArgs &A, int4 mt, int lane) {
 __global__ __launch_bounds__(kSweepGrid) void k_resid_init(SweepArgs A) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (i >= A.n) return;
-  A.FR[i] = make_double2(A.F[i], A.w[(size_t)i * A.ws] -
-                                     gather_dot(A, A.lu[i], A.lu[A.n + i], -1));
+  A.FR[i] = make_double2(A.F[i] * A.fscale,
+                         A.w[(size_t)i * A.ws] -
+                             gather_dot(A, A.lu[i], A.lu[A.n + i], -1));
 }
 
 // One block walks phases [ph0, ph1) with a barrier between them.  The
@@ -893,7 +897,13 @@ extern "C" int nngp_sNNGP(
   };
 
   int cur = 0, accept = 0, batchAccept = 0, status = 0;
-  NNGP_TRY(md.pass(cov_of(th[0], th[2], th[3]), cur, true, nullptr));
+  // B and F of slot cur were computed at sigma.sq = sigF (log det ldF).
+  // Without a nugget B does not depend on sigma.sq and F is proportional to
+  // it, so the current theta's pass of the phi step (src/sNNGP.cpp:343-360)
+  // is a rescaling of these and of the sweep's maintained residual sums.
+  PassOut p0;
+  NNGP_TRY(md.pass(cov_of(th[0], th[2], th[3]), cur, true, &p0));
+  double sigF = th[0], ldF = p0.logDet;
   for (int s = 0; s < nSamples; s++) {
     // w | rest: n normals in site order (the reference draws them in its
     // sweep, which visits the sites in that order)
@@ -926,6 +936,7 @@ extern "C" int nngp_sNNGP(
     A.p = p;
     A.ws = md.zs;
     A.nph = nph;
+    A.fscale = th[0] == sigF ? 1.0 : th[0] / sigF;
     hipLaunchKernelGGL(k_resid_init, dim3((n + kSweepGrid - 1) / kSweepGrid),
                        dim3(kSweepGrid), 0, st, A);
     NNGP_HIP(hipGetLastError());
@@ -964,6 +975,7 @@ extern "C" int nngp_sNNGP(
     NNGP_TRY(reduce(1, cur));
     th[1] = 1.0 / gtau.draw(R, 1.0 / (tB + 0.5 * hsum.as<double>()[0]));
     // sigma.sq | rest (:322-341)
+    const double sigOld = th[0];
     th[0] = 1.0 / gsig.draw(R, 1.0 / (sB + 0.5 * aSig * th[0]));
     // phi [, nu]: Metropolis with B, F of the new sigma.sq (:343-420); the
     // candidate is drawn before both passes, the draw order is unchanged
@@ -972,10 +984,14 @@ extern "C" int nngp_sNNGP(
     const double nuC =
         matern ? logit_inv(R.rnorm(logit_ab(th[3], nA, nB), nuTuning), nA, nB)
                : 0.0;
-    PassOut pc, pn;
-    NNGP_TRY(md.pass(cov_of(th[0], th[2], th[3]), cur, true, &pc));
+    // the current theta at the new sigma.sq: F = F(sigF) th[0] / sigF, so
+    // log det = ldF + n log(th[0] / sigF); the quadratic form of the swept
+    // residuals (aSig, taken with F at sigOld) scales by sigOld / th[0]
+    const double ldCur = ldF + n * std::log(th[0] / sigF);
+    const double qCur = aSig * (sigOld / th[0]);
+    PassOut pn;
     NNGP_TRY(md.pass(cov_of(th[0], phiC, nuC), 1 - cur, true, &pn));
-    double lpCur = -0.5 * pc.logDet - 0.5 * gram_at(pc.gram, 0, 0);
+    double lpCur = -0.5 * ldCur - 0.5 * qCur;
     lpCur += std::log(th[2] - pA) + std::log(pB - th[2]);
     if (matern) lpCur += std::log(th[3] - nA) + std::log(nB - th[3]);
     double lpCand = -0.5 * pn.logDet - 0.5 * gram_at(pn.gram, 0, 0);
@@ -985,6 +1001,8 @@ extern "C" int nngp_sNNGP(
       cur = 1 - cur;
       th[2] = phiC;
       if (matern) th[3] = nuC;
+      sigF = th[0];
+      ldF = pn.logDet;
       accept++;
       batchAccept++;
     }
