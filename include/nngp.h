@@ -89,7 +89,10 @@ typedef struct nngp_opts {
                                a grid context; threads == 0: the default,
                                or the autotuner's choice                    */
   int sweep;                /* enum nngp_sweep, for nngp_sNNGP              */
-  int reserved[4];
+  int search;               /* a grid context's per-fold neighbor search:
+                               0 code book (mkNNIndxCB), 1 brute (mkNNIndx),
+                               R's search.type (R/spConjNNGP.R:301-305)    */
+  int reserved[3];
 } nngp_opts;
 
 /* Message for the last nonzero status returned on this host thread. */
@@ -114,9 +117,13 @@ int nngp_device_count(void);
 int nngp_mkNNIndxCB(int n, int m, const double *coords, int *nnIndx,
                     double *nnDist, int *nnIndxLU, int nThreads);
 
-/* mkNNIndx — replaces the brute-force twin src/nn.cpp:63-111.  Same output
- * as nngp_mkNNIndxCB (the two differ in the reference only in how exact
- * distance ties are broken). */
+/* mkNNIndx — replaces the brute-force twin src/nn.cpp:63-111: the same
+ * neighbor sets as nngp_mkNNIndxCB where distances do not tie.  Where they
+ * do, each search gives the reference's own result: the code-book walk's
+ * candidate order and R's shell sort (rsort_with_index) decide the order
+ * of tied neighbors and which of them is kept at the m-th place for
+ * mkNNIndxCB, the ascending scan and the same sort for mkNNIndx (which
+ * compares Euclidean, not squared, distances). */
 int nngp_mkNNIndx(int n, int m, const double *coords, int *nnIndx,
                   double *nnDist, int *nnIndxLU, int nThreads);
 
@@ -134,8 +141,11 @@ int nngp_mkUIndx(int n, int m, const int *nnIndx, int *uIndx, int *uIndxLU,
 
 /* nn2 replacement (RANN, called at R/generics.R:642, R/spConjNNGP.R:158,311):
  * exact k nearest fitted sites of each of q query sites, no ordering
- * constraint.  nnIndx0 is q x k column-major, 0-based (the R code's
- * `nn2(...)$nn.idx - 1`); nnDist0 (may be NULL) likewise. */
+ * constraint, ascending by squared distance fl(fl(dx*dx)+fl(dy*dy)) and,
+ * among equal distances, by ascending fitted row (RANN's own order among
+ * exact ties depends on its kd-tree and is not reproduced).  nnIndx0 is
+ * q x k column-major, 0-based (the R code's `nn2(...)$nn.idx - 1`);
+ * nnDist0 (may be NULL) likewise. */
 int nngp_nn2(int n, const double *coords, int q, const double *coords0,
              int k, int *nnIndx0, double *nnDist0);
 

@@ -58,7 +58,8 @@ class NngpOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("device", C.c_int),
                 ("seed", C.c_ulonglong), ("interrupt", _INT_FN),
                 ("interrupt_arg", C.c_void_p), ("shape", C.c_int * 3),
-                ("sweep", C.c_int), ("reserved", C.c_int * 4)]
+                ("sweep", C.c_int), ("search", C.c_int),
+                ("reserved", C.c_int * 3)]
 
 
 class NngpPassOut(C.Structure):

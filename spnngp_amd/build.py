@@ -41,11 +41,30 @@ def sources():
                   if f.endswith(".hip"))
 
 
-def build(verbose: bool = True, jobs: int | None = None) -> str:
+def _stale(src, obj):
+    """An object is stale when it is missing or older than its source or
+    any header of csrc/ or include/."""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [src] + [os.path.join(CSRC, f) for f in os.listdir(CSRC)
+                    if f.endswith(".hpp")]
+    deps.append(os.path.join(ROOT, "include", "nngp.h"))
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build(verbose: bool = True, jobs: int | None = None,
+          incremental: bool = False) -> str:
+    """Compile every source (incremental: only stale objects) and link."""
     hipcc = _hipcc()
     os.makedirs(OBJ, exist_ok=True)
     srcs = sources()
     objs = [os.path.join(OBJ, os.path.basename(s)[:-4] + ".o") for s in srcs]
+    only = [a[len("--only="):] for a in sys.argv[1:]
+            if a.startswith("--only=")] if __name__ == "__main__" else []
+    todo = [(s, o) for s, o in zip(srcs, objs)
+            if (not incremental or _stale(s, o)) and
+            (not only or os.path.basename(s)[:-4] in only)]
 
     def compile_one(pair):
         src, obj = pair
@@ -55,9 +74,9 @@ def build(verbose: bool = True, jobs: int | None = None) -> str:
             raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")
         return src
 
-    jobs = jobs or min(len(srcs), os.cpu_count() or 4, 8)
+    jobs = jobs or max(1, min(len(todo), os.cpu_count() or 4, 8))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        for src in ex.map(compile_one, zip(srcs, objs)):
+        for src in ex.map(compile_one, todo):
             if verbose:
                 print(f"[nngp build] compiled {os.path.basename(src)}",
                       flush=True)
@@ -75,5 +94,5 @@ def build(verbose: bool = True, jobs: int | None = None) -> str:
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv[1:])
     sys.exit(0)

@@ -42,8 +42,11 @@ static double logit_inv(double z, double a, double b) {
 // ---------------------------------------------------------------------------
 // neighbor search (strip index: nngp_strips.hpp)
 // ---------------------------------------------------------------------------
+// mode 0: the code-book search (src/nn.cpp:214-257), mode 1: the scan
+// (src/nn.cpp:63-111); tied distances are resolved in each one's own
+// candidate order (knn_build).
 static int knn_pred(int n, int m, const double *coords, int *nnIndx,
-                    double *nnDist, int *nnIndxLU) {
+                    double *nnDist, int *nnIndxLU, int mode) {
   NNGP_TRY(check_nm(n, m));
   if (!coords || !nnIndx || !nnDist || !nnIndxLU)
     return set_err(NNGP_ERR_ARG, "NULL buffer");
@@ -60,17 +63,8 @@ static int knn_pred(int n, int m, const double *coords, int *nnIndx,
   NNGP_TRY(dI.alloc(sizeof(int) * (size_t)ni));
   NNGP_TRY(dD.alloc(sizeof(double) * (size_t)ni));
   NNGP_TRY(dL.alloc(sizeof(int) * 2 * (size_t)n));
-  KnnArgs a;
-  a.s = ds.idx;
-  a.xy = xy.as<double2>();
-  a.strip_of = ds.strip_of.as<int>();
-  a.n = n;
-  a.m = m;
-  a.sorted = h.sorted ? 1 : 0;
-  a.nnIndx = dI.as<int>();
-  a.nnDist = dD.as<double>();
-  a.lu = dL.as<int>();
-  if (knn_launch(a, st.s)) return set_err(NNGP_ERR_HIP, "kNN launch failed");
+  NNGP_TRY(knn_build(coords, n, m, mode, h, ds, xy.as<double2>(), dI.as<int>(),
+                     dD.as<double>(), dL.as<int>(), st.s, nullptr));
   NNGP_HIP(hipMemcpyAsync(nnIndx, dI.p, dI.bytes, hipMemcpyDeviceToHost, st.s));
   NNGP_HIP(hipMemcpyAsync(nnDist, dD.p, dD.bytes, hipMemcpyDeviceToHost, st.s));
   NNGP_HIP(hipMemcpyAsync(nnIndxLU, dL.p, dL.bytes, hipMemcpyDeviceToHost, st.s));
@@ -287,14 +281,14 @@ int nngp_mkNNIndxCB(int n, int m, const double *coords, int *nnIndx,
                     double *nnDist, int *nnIndxLU, int nThreads) {
   (void)nThreads;
   clear_err();
-  return knn_pred(n, m, coords, nnIndx, nnDist, nnIndxLU);
+  return knn_pred(n, m, coords, nnIndx, nnDist, nnIndxLU, 0);
 }
 
 int nngp_mkNNIndx(int n, int m, const double *coords, int *nnIndx,
                   double *nnDist, int *nnIndxLU, int nThreads) {
   (void)nThreads;
   clear_err();
-  return knn_pred(n, m, coords, nnIndx, nnDist, nnIndxLU);
+  return knn_pred(n, m, coords, nnIndx, nnDist, nnIndxLU, 1);
 }
 
 int nngp_nn2(int n, const double *coords, int q, const double *coords0,

@@ -439,6 +439,7 @@ static int cv_pred_launch(CvPredArgs a, int variant, int nprob,
 // ---------------------------------------------------------------------------
 struct CvGrid {
   int n = 0, p = 0, m = 0, k = 0, nho = 0, nmod = 0, cov = 0, nz = 0, zs = 0;
+  int search = 0;  // per-fold neighbor search: 0 code book, 1 scan
   long long nIndx = 0, nn_stride = 0;
   double a0 = 0.0, b0 = 0.0;
   Stream st;
@@ -461,7 +462,7 @@ struct CvGrid {
 
   int create(const double *yv, const double *X, int p_, int n_,
              const double *coords, int m_, int k_, const int *ho, int nho_,
-             const double *ig, int cov_);
+             const double *ig, int cov_, int search_);
   int run(const double *thetaAlpha, int g, double *scores,
           nngp_cvgrid_detail *det);
   int record(int g, int variant, bool want_y0);
@@ -475,8 +476,9 @@ struct CvGrid {
 
 int CvGrid::create(const double *yv, const double *X, int p_, int n_,
                    const double *coords, int m_, int k_, const int *ho,
-                   int nho_, const double *ig, int cov_) {
+                   int nho_, const double *ig, int cov_, int search_) {
   n = n_;
+  search = search_;
   p = p_;
   m = m_;
   k = k_;
@@ -549,17 +551,9 @@ int CvGrid::create(const double *yv, const double *X, int p_, int n_,
     build_strips(nmod, cm.data(), cm.data() + nmod, m, hs);
     DevStrips ds;
     NNGP_TRY(ds.upload(hs, st.s));
-    KnnArgs ka;
-    ka.s = ds.idx;
-    ka.xy = dxy.as<double2>();
-    ka.strip_of = ds.strip_of.as<int>();
-    ka.n = nmod;
-    ka.m = m;
-    ka.sorted = hs.sorted ? 1 : 0;
-    ka.nnIndx = nnI.as<int>() + (size_t)f * nn_stride;
-    ka.nnDist = nullptr;
-    ka.lu = lu.as<int>() + 2 * (size_t)f * nmod;
-    if (knn_launch(ka, st.s)) return set_err(NNGP_ERR_HIP, "kNN launch failed");
+    NNGP_TRY(knn_build(cm.data(), nmod, m, search, hs, ds, dxy.as<double2>(),
+                       nnI.as<int>() + (size_t)f * nn_stride, nullptr,
+                       lu.as<int>() + 2 * (size_t)f * nmod, st.s, nullptr));
     // held-out sites and their m nearest training sites
     NNGP_TRY(upload_xy(c0.data(), nho, dxy0, st.s));
     NNGP_HIP(hipMemcpyAsync(xy0.as<double2>() + (size_t)f * nho, dxy0.p,
@@ -875,11 +869,16 @@ int nngp_cvgrid_create(nngp_cvgrid **out, const double *y, const double *X,
   if (m > 64) return set_err(NNGP_ERR_UNSUPPORTED, "n.neighbors = %d exceeds 64", m);
   if (!y || !X || !coords || !hoIndx || !sigmaSqIG)
     return set_err(NNGP_ERR_ARG, "NULL buffer");
+  const int search = opts ? opts->search : 0;
+  if (search != 0 && search != 1)
+    return set_err(NNGP_ERR_ARG,
+                   "search = %d (0: code book, 1: brute) is not defined",
+                   search);
   NNGP_TRY(use_device(opts));
   nngp_cvgrid *c = new (std::nothrow) nngp_cvgrid;
   if (!c) return set_err(NNGP_ERR_NOMEM, "host allocation failed");
   const int st = c->cv.create(y, X, p, n, coords, m, kFold, hoIndx, nHo,
-                              sigmaSqIG, covModel);
+                              sigmaSqIG, covModel, search);
   if (st != NNGP_OK) {
     delete c;
     return st;

@@ -144,8 +144,23 @@ struct KnnArgs {
   int *nnIndx;           // outputs in the reference layout
   double *nnDist;        // or nullptr: not stored
   int *lu;
+  int mode;              // 0: code-book walk (mkNNIndxCB), 1: scan (mkNNIndx)
+  double E;              // 2^-48 max(|x| + |y|): rounding scale of the walk
+  unsigned char *flag;   // n: 1 where ties need the reference's own order
 };
 int knn_launch(const KnnArgs &a, hipStream_t st);
+
+// The flagged sites redone in the reference's candidate order.
+struct KnnReplayArgs {
+  int n, m, mode, nflag;
+  const int *list;                 // flagged site ids
+  const double *ux, *uy;           // mode 0: coordinates in R's u order
+  const int *usid, *upos;          //   site at each position, and inverse
+  const double *x, *y;             // mode 1: coordinates by site
+  int *nnIndx;
+  double *nnDist;
+};
+int knn_replay_launch(const KnnReplayArgs &a, hipStream_t st);
 
 // nn2: k nearest fitted points of each query point.
 struct Nn2Args {

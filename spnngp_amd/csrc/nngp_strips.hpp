@@ -99,6 +99,23 @@ struct DevStrips {
   }
 };
 
+// Compacts the sites with flag[i] != 0 into list[0 .. *count) (device),
+// ascending; tmp receives the scan's workspace.
+int knn_flagged(const unsigned char *flag, int n, int *list, int *count,
+                DevBuf &tmp, hipStream_t st);
+
+// Neighbor lists of n sites (coords: column-major host copy; xy: the same
+// on the device, interleaved) into device buffers in the reference layout:
+// mode 0 the code-book search (src/nn.cpp:214-257), 1 the scan
+// (src/nn.cpp:63-111).  The strip search flags the sites whose tied
+// distances need the reference's own candidate order, and those are redone
+// in that order.  nnDist may be nullptr; *nflag_out (optional) receives the
+// number of redone sites.  Returns after the device work has finished.
+int knn_build(const double *coords, int n, int m, int mode,
+              const HostStrips &h, const DevStrips &ds, const double2 *xy,
+              int *nnIndx, double *nnDist, int *lu, hipStream_t st,
+              int *nflag_out);
+
 // Column-major n x 2 coordinates (host) to interleaved double2 (device).
 inline int upload_xy(const double *cm, int n, DevBuf &xy, hipStream_t st) {
   DevBuf tmp;

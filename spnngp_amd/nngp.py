@@ -47,10 +47,10 @@ def _cm(a):
 
 
 def _opts(precision: int = 0, device: int = 0, seed: int = 1, shape=None,
-          sweep: int = 0):
+          sweep: int = 0, search: int = 0):
     sh = (C.c_int * 3)(*(shape if shape is not None else (0, 0, 0)))
     return NngpOpts(precision, device, seed, _lib._INT_FN(), None, sh, sweep,
-                    (C.c_int * 4)())
+                    search, (C.c_int * 3)())
 
 
 def device_count() -> int:
@@ -607,7 +607,13 @@ class CvGrid:
     fixed at construction (ho: k x nHo held-out rows, 0-based), every run
     scores all folds x grid points in one graph launch."""
 
-    def __init__(self, y, X, coords, m, ho, sigmaSqIG, covModel, shape=None):
+    def __init__(self, y, X, coords, m, ho, sigmaSqIG, covModel, shape=None,
+                 search_type="cb"):
+        """search_type: each fold's neighbor search, "cb" (mkNNIndxCB) or
+        "brute" (mkNNIndx), R's search.type."""
+        if search_type not in ("cb", "brute"):
+            raise ValueError(f"search_type must be 'cb' or 'brute', "
+                             f"not {search_type!r}")
         X = np.asarray(X, dtype=np.float64)
         self.n, self.p = X.shape
         ho = np.asarray(ho)
@@ -615,7 +621,7 @@ class CvGrid:
         self.m = m
         self.cov = cov_index(covModel)
         self._h = C.c_void_p()
-        opts = _opts(shape=shape)
+        opts = _opts(shape=shape, search=int(search_type == "brute"))
         check(load().nngp_cvgrid_create(
             C.byref(self._h), _f64(y), _cm(X), self.p, self.n, _cm(coords),
             m, self.k, _i32(ho.reshape(-1)), self.nho, _f64(sigmaSqIG),

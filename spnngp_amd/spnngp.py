@@ -429,7 +429,7 @@ def spConjNNGP(y, X, coords, *, theta_alpha, sigma_sq_IG, n_neighbors=15,
             ho_indx = np.array([np.sort(rng.choice(n, nho, replace=False))
                                 for _ in range(k_fold)], dtype=np.int32)
         with _nn.CvGrid(y_o, X_o, coords_o, n_neighbors, ho_indx, ig,
-                        cov_model) as cv:
+                        cov_model, search_type=search_type) as cv:
             r = cv.run(grid)
         scores = r["scores"]  # g x 2: rmspe, crps
         best = grid[int(np.argmin(scores[:, 0 if score_rule == "rmspe"

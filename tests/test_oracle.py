@@ -105,6 +105,67 @@ def test_rsort_with_index_sorts_pairs(oracle):
     assert list(ix) == [2, 3, 0, 1]
 
 
+def test_rsort_with_index_shell_order_of_ties_known_answer(oracle):
+    # Traced by hand through R's shell sort (gaps 4 then 1 for n = 10): the
+    # h = 4 pass moves the 3.0 at position 5 past the 1.5 to position 9,
+    # and the stable h = 1 pass keeps it behind the other 3.0s, so the tie
+    # order is not the input order.
+    x = np.array([0.0, 1.0, 2.0, 3.0, 3.0, 3.0, 3.0, 3.0, 3.0, 1.5])
+    xs, ix = oracle.rsort_with_index(x, np.arange(10, dtype=np.int32))
+    assert list(ix) == [0, 1, 9, 2, 3, 4, 6, 7, 8, 5]
+    # below n = 9 there is only the h = 1 pass: a stable insertion sort
+    x = np.array([2.0, 1.0, 2.0, 1.0, 0.5, 2.0, 1.0, 0.5])
+    xs, ix = oracle.rsort_with_index(x, np.arange(8, dtype=np.int32))
+    assert list(ix) == list(np.argsort(x, kind="stable"))
+
+
+def test_knn_ties_on_a_2x2_lattice_known_answer(oracle):
+    # Sites (x order): s0 (0,0), s1 (0,1), s2 (1,0), s3 (1,1); m = 2.
+    # Traced by hand through src/nn.cpp.  Code book: u = (0, 1, 1, 2) keeps
+    # s1 before s2 (gap-1 sort); s3's walk goes down only (it is last in u),
+    # meets s2 first, then s1 at the same distance 1 (inserted: 1 < inf),
+    # then stops at s0 ((du)^2 = 4 > 2).  The stable sort leaves s2 first.
+    # The scan meets s1 before s2, so it lists s1 first.
+    c = np.array([[0.0, 0.0], [0.0, 1.0], [1.0, 0.0], [1.0, 1.0]])
+    cb = oracle.mkNNIndxCB(c, 2)
+    br = oracle.mkNNIndx(c, 2)
+    assert list(cb["nnIndxLU"]) == [0, 0, 1, 3, 0, 1, 2, 2]
+    assert list(cb["nnIndx"]) == [0, 0, 1, 2, 1]
+    assert list(br["nnIndx"]) == [0, 0, 1, 1, 2]
+    r2 = np.sqrt(2.0)
+    assert list(cb["nnDist"]) == [1.0, 1.0, r2, 1.0, 1.0]
+    assert list(br["nnDist"]) == [1.0, 1.0, r2, 1.0, 1.0]
+
+
+@pytest.mark.parametrize("kind", ["lattice", "rounded", "duplicates"])
+def test_knn_tied_inputs_keep_the_m_smallest_distances(oracle, kind):
+    # Whatever the tie order, each list holds the m smallest distances
+    # among the predecessors (the walk's pruning is exact up to ties).
+    rng = np.random.default_rng(7)
+    if kind == "lattice":
+        g = np.arange(25) / 25.0
+        c = np.array([(a, b) for a in g for b in g])
+    elif kind == "rounded":
+        c = np.round(rng.uniform(size=(700, 2)), 2)
+    else:
+        c = rng.uniform(size=(500, 2))
+        c = np.vstack([c, c[:80], c[:10]])
+    c = c[np.argsort(c[:, 0], kind="stable")]
+    n, m = c.shape[0], 10
+    for fn in (oracle.mkNNIndx, oracle.mkNNIndxCB):
+        r = fn(c, m)
+        lu = r["nnIndxLU"]
+        for i in range(1, n):
+            off, cnt = lu[i], lu[n + i]
+            dx = c[i, 0] - c[:i, 0]
+            dy = c[i, 1] - c[:i, 1]
+            best = np.sort(np.sqrt(dx * dx + dy * dy))[:cnt]
+            got = r["nnDist"][off:off + cnt]
+            assert np.array_equal(got, best), (fn.__name__, i)
+            j = r["nnIndx"][off:off + cnt]
+            assert len(set(j)) == cnt and np.all(j < i)
+
+
 def brute_pred_knn(coords, m):
     """m nearest predecessors by the reference's squared-distance arithmetic,
     written directly in numpy."""
@@ -157,6 +218,21 @@ def test_knn_custom_order(oracle):
     for i in range(1, 400):
         off, cnt = lu[i], lu[400 + i]
         assert np.array_equal(r["nnIndx"][off:off + cnt], ref[i - 1][0])
+
+
+def test_nn2_orders_ties_by_ascending_row(oracle):
+    g = np.arange(20) / 20.0
+    c = np.array([(a, b) for a in g for b in g])[::-1].copy()
+    c0 = np.vstack([c[::7] + 0.025, c[::11]])  # cell centres and sites
+    k = 9
+    idx, dist = oracle.nn2(c, c0, k)
+    for i in range(c0.shape[0]):
+        dx = c0[i, 0] - c[:, 0]
+        dy = c0[i, 1] - c[:, 1]
+        d2 = dx * dx + dy * dy
+        order = np.lexsort((np.arange(c.shape[0]), d2))[:k]
+        assert np.array_equal(idx[i], order)
+        assert np.array_equal(dist[i], np.sqrt(d2[order]))
 
 
 def test_nn2_matches_kdtree(oracle):
