@@ -175,6 +175,23 @@ int nngp_Q(const double *B, const double *F, const double *u,
            const double *v, int n, int m, const int *nnIndx,
            const int *nnIndxLU, double *q);
 
+/* Q on device-resident buffers, allocating nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * B, F, u, v, nnIndx, nnIndxLU are device pointers in nngp_Q's layouts
+ * (not validated: the caller guarantees them); *q, also on the device,
+ * receives the sum.  The kernels' partial sums live in a library-owned
+ * device array of 1024 doubles (8 KiB), held by every GPU the library runs
+ * on until the process ends; the reduction grid never has more than 1024
+ * blocks, so every n whose layout fits int32 (nIndx < 2^31) is accepted
+ * and larger ones are refused.  The work is enqueued on `stream` (a
+ * hipStream_t of the current device; NULL: the null stream) and not waited
+ * for.  Calls on one GPU share the array: calls that overlap on one GPU
+ * race, so order them (one stream, or events). */
+int nngp_Q_noalloc(const double *B, const double *F, const double *u,
+                   const double *v, int n, int m, const int *nnIndx,
+                   const int *nnIndxLU, double *q, void *stream);
+
 /* Sequential NNGP draw w_i = B_i . w_N(i) + sqrt(F_i) z_i over i = 0..n-1
  * — replaces updateRep, src/rNNGP.cpp:24-42 (also
  * src/rNNGPReplicated.cpp:183-199), with the n standard normals z supplied

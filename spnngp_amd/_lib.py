@@ -42,7 +42,7 @@ EXPORTS = (
     "nngp_cvgrid_create", "nngp_cvgrid_run", "nngp_cvgrid_destroy",
     "nngp_model_autotune", "nngp_model_set_shape", "nngp_rNNGPReplicated",
     "nngp_rng_r_create", "nngp_rng_r_destroy", "nngp_mkUIndx", "nngp_sNNGP",
-    "nngp_sNNGPPredict",
+    "nngp_sNNGPPredict", "nngp_Q_noalloc",
 )
 
 _RNG_FN = C.CFUNCTYPE(C.c_double, C.c_void_p)
@@ -146,6 +146,8 @@ def load():
                                 C.POINTER(_d), C.POINTER(NngpOpts)]
     L.nngp_updateConjBF.argtypes = [_dp, _dp, _dp, _ip, _ip, _i, _i, _d, _d,
                                     _d, _i, C.POINTER(NngpOpts)]
+    L.nngp_Q_noalloc.argtypes = [_vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp,
+                                 _vp]
     L.nngp_Q.argtypes = [_dp, _dp, _dp, _dp, _i, _i, _ip, _ip,
                          C.POINTER(_d)]
     L.nngp_rNNGP.argtypes = (

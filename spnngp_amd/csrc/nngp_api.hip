@@ -6,6 +6,7 @@
 #include <chrono>
 #include <climits>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 
 #include "nngp_host.hpp"
@@ -258,6 +259,37 @@ static void print_model(int n, int p, int m, int covModel, int nSamples,
   fflush(stdout);
 }
 
+// ---------------------------------------------------------------------------
+// Q on device buffers without allocating
+// ---------------------------------------------------------------------------
+// k_quadform's per-block partial sums go to this file-scope array instead of
+// a buffer allocated per call.  Every device holds its own copy, at its own
+// address: that address is looked up once per device (hipGetSymbolAddress,
+// with the device current) and only it is handed to kernels.  The array is
+// shared by every call on a device and keeps the last call's partial sums;
+// each launch writes every partial it then reads, so nothing needs clearing.
+static __device__ __align__(256) double scratch[kQPartialRows];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch[kMaxDevices] = {};
+
+static int scratch_on_current_device(double **out) {
+  int dev = 0;
+  NNGP_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return set_err(NNGP_ERR_UNSUPPORTED, "device ordinal %d above %d", dev,
+                   kMaxDevices - 1);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    NNGP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch[dev];
+  return NNGP_OK;
+}
+
 }  // namespace nngp
 
 using namespace nngp;
@@ -403,6 +435,31 @@ int nngp_Q(const double *B, const double *F, const double *u,
   if (q_launch(a, st.s)) return set_err(NNGP_ERR_HIP, "Q launch failed");
   NNGP_HIP(hipMemcpyAsync(q, out.p, sizeof(double), hipMemcpyDeviceToHost, st.s));
   NNGP_HIP(hipStreamSynchronize(st.s));
+  return NNGP_OK;
+}
+
+// Q on device buffers without allocating (scratch: nngp_api.hip's
+// file-scope array, see scratch_on_current_device)
+int nngp_Q_noalloc(const double *B, const double *F, const double *u,
+                   const double *v, int n, int m, const int *nnIndx,
+                   const int *nnIndxLU, double *q, void *stream) {
+  clear_err();
+  NNGP_TRY(check_nm(n, m));
+  if (!B || !F || !u || !v || !nnIndx || !nnIndxLU || !q)
+    return set_err(NNGP_ERR_ARG, "NULL buffer");
+  if (n_indx(n, m) > (long long)INT_MAX)
+    return set_err(NNGP_ERR_ARG, "n = %d, m = %d: nIndx exceeds 2^31 - 1", n,
+                   m);
+  const int rows = q_partial_rows(n);
+  if (rows > kQPartialRows)
+    return set_err(NNGP_ERR_UNSUPPORTED,
+                   "n = %d needs %d partial sums; the scratch holds %d", n,
+                   rows, kQPartialRows);
+  double *part = nullptr;
+  NNGP_TRY(scratch_on_current_device(&part));
+  QArgs a{B, F, u, v, nnIndx, nnIndxLU, n, part, q};
+  if (q_launch(a, static_cast<hipStream_t>(stream)))
+    return set_err(NNGP_ERR_HIP, "Q launch failed");
   return NNGP_OK;
 }
 

@@ -1246,6 +1246,11 @@ __global__ __launch_bounds__(kRedThreads) void k_quadform(QArgs A) {
   }
 }
 
+static_assert(kRedBlocks == kQPartialRows,
+              "k_quadform's grid bounds its partial sums");
+
+int q_partial_rows(int n) { return red_grid(n); }
+
 int q_launch(const QArgs &a, hipStream_t st) {
   const int nb = red_grid(a.n);
   hipLaunchKernelGGL(k_quadform, dim3(nb), dim3(kRedThreads), 0, st, a);

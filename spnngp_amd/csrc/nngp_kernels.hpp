@@ -88,6 +88,11 @@ struct QArgs {
   double *out;
 };
 
+// Partial sums k_quadform writes: one per block of its grid-stride launch,
+// never more than this many, whatever n.
+constexpr int kQPartialRows = 1024;
+int q_partial_rows(int n);
+
 // Rows a partial-row buffer needs for n sites under any launch shape.
 int rows_capacity(int n);
 bool bf_lane_ok(int m, int variant);
