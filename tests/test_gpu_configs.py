@@ -1,0 +1,262 @@
+"""The benchmark configurations at their own shapes wherever the oracle can
+run them, the GPU path against the oracle on the same inputs:
+
+  1. plain response NNGP, n = 2000, m = 10, exponential, 1000 MH iterations
+     at tuning 0.5 on exact-GP data: the whole chain (src/rNNGP.cpp:272-378);
+  2. the at-scale response model's systems (exponential, sigma.sq = 2,
+     phi = 12, tau.sq = 0.2, m = 15, data drawn from the NNGP prior) at
+     n = 1e6: FP64 within 1e-10, the mixed form's log-likelihood within
+     1e-8 (its FP32 entries carry ~6e-8 each; the per-site errors average
+     out in the sums over n, so the bound is a property of this size);
+  3. the conjugate grid, Matern nu = 1.5, m = 15, k = 5 folds, 20 grid
+     points, at n = 2e4: every fold x point against cNNGP
+     (src/cNNGP.cpp:215-316) and the scores of R/spConjNNGP.R:336-350;
+  5. the stress systems, 50 Gaussian clusters plus 10% uniform, x stretched
+     by 3, Matern with a general (Bessel K) nu, phi = 20, sigma.sq = 1,
+     tau.sq = 0.05, m = 32: FP64 within 1e-10 of the oracle at n = 2e4, and
+     at the full n = 4e6 the mixed form's log-likelihood within 1e-8 of the
+     FP64 pass (which the oracle pins at the smaller size).
+"""
+import numpy as np
+import pytest
+import scipy.stats as ss
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+def max_rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def per_site_rel(B1, B0, lu, n):
+    """max over sites i >= 1 of max|B1_i - B0_i| / max|B0_i|"""
+    off = lu[1:n]
+    d = np.maximum.reduceat(np.abs(B1 - B0)[:len(B0)], off)
+    s = np.maximum.reduceat(np.abs(B0), off)
+    return float(np.max(d / np.maximum(s, 1e-300)))
+
+
+def x_ordered(c, *more):
+    o = np.argsort(c[:, 0], kind="stable")
+    return (c[o],) + tuple(a[o] for a in more)
+
+
+# ---------------------------------------------------------------------------
+# 1. n = 2000, m = 10, 1000 MH iterations
+# ---------------------------------------------------------------------------
+def test_config1_chain_identical_to_oracle(oracle, nngp):
+    print("config 1 chain n=2000 m=10 S=1000", flush=True)
+    n, m, S = 2000, 10, 1000
+    rng = np.random.default_rng(1)
+    c = rng.uniform(size=(n, 2))
+    X = np.column_stack([np.ones(n), rng.normal(size=n)])
+    D = np.sqrt(((c[:, None] - c[None]) ** 2).sum(-1))
+    L = np.linalg.cholesky(5.0 * np.exp(-6.0 * D) + 1e-10 * np.eye(n))
+    y = X @ [1.0, 5.0] + L @ rng.normal(size=n) + rng.normal(size=n)
+    zs, us = rng.normal(size=S * 6 + 10), rng.uniform(size=S + 2)
+    priors = {"sigma.sq.IG": [2, 5], "tau.sq.IG": [2, 1],
+              "phi.Unif": [3, 300]}
+    starting = {"sigma.sq": 5, "tau.sq": 1, "phi": 6}
+    tuning = {"sigma.sq": 0.5, "tau.sq": 0.5, "phi": 0.5}
+    tape = oracle.TapeRng(zs, us)
+    fit = nngp.spNNGP(y, X, c, starting=starting, tuning=tuning,
+                      priors=priors, n_samples=S, n_neighbors=m,
+                      search_type="cb", rng=tape)
+    co, Xo, yo = x_ordered(c, X, y)
+    nn = oracle.mkNNIndxCB(co, m, nThreads=16)
+    b0 = np.linalg.lstsq(Xo, yo, rcond=None)[0]
+    ref_tape = oracle.TapeRng(zs, us)
+    ref = oracle.rNNGP(yo, Xo, 2, n, m, co.T.reshape(-1).copy(), 0,
+                       nn["nnIndx"], nn["nnIndxLU"], sigmaSqIG=[2, 5],
+                       tauSqIG=[2, 1], phiUnif=[3, 300], nuUnif=None,
+                       betaStarting=b0, sigmaSqStarting=5.0,
+                       tauSqStarting=1.0, phiStarting=6.0, nuStarting=0.0,
+                       sigmaSqTuning=0.5, tauSqTuning=0.5, phiTuning=0.5,
+                       nuTuning=0.0, nSamples=S, rng=ref_tape, nThreads=16)
+    assert (tape.i_n, tape.i_u) == (ref_tape.i_n, ref_tape.i_u)
+    th0 = ref["p.theta.samples"].T
+    th1 = fit["p.theta.samples"]
+    acc0 = np.r_[True, np.any(th0[1:] != th0[:-1], axis=1)]
+    acc1 = np.r_[True, np.any(th1[1:] != th1[:-1], axis=1)]
+    assert np.array_equal(acc0, acc1)
+    assert 0 < acc0.sum() < S  # tuning 0.5 accepts ~2% on these data
+    assert max_rel(th1, th0) < 1e-9
+    assert max_rel(fit["p.beta.samples"], ref["p.beta.samples"].T) < 1e-9
+    print(f"  accepts {acc0.sum()}, theta rel {max_rel(th1, th0):.1e}",
+          flush=True)
+
+
+# ---------------------------------------------------------------------------
+# 2. response model systems at n = 1e6 (m = 15, exponential)
+# ---------------------------------------------------------------------------
+def nngp_prior_data(oracle, c, m, sigma_sq, phi, tau_sq, beta, rng, cov=0,
+                    nu=0.0, jitter=0.0):
+    """X = [1, N(0,1), ...], w from the NNGP prior (w_i = B_i w_N(i) +
+    sqrt(F_i) z_i, nugget `jitter`), y = X beta + w + N(0, tau_sq)."""
+    n = c.shape[0]
+    X = np.column_stack([np.ones(n), rng.normal(size=(n, len(beta) - 1))])
+    nn = oracle.mkNNIndxCB(c, m, nThreads=16)
+    th = [sigma_sq, jitter, phi] + ([nu] if cov == 2 else [])
+    B, F, _ = oracle.updateBF(c.T.reshape(-1).copy(), nn["nnIndx"],
+                              nn["nnIndxLU"], n, m, th, cov, nThreads=16)
+    w = oracle.nngp_draw(B, F, nn["nnIndx"], nn["nnIndxLU"],
+                         rng.normal(size=n))
+    y = X @ beta + w + np.sqrt(tau_sq) * rng.normal(size=n)
+    return X, y, nn
+
+
+def loglik_pair(oracle, nngp, c, nn, X, y, m, cov, theta, beta):
+    n = c.shape[0]
+    B0, F0, ld0 = oracle.updateBF(c.T.reshape(-1).copy(), nn["nnIndx"],
+                                  nn["nnIndxLU"], n, m, theta, cov,
+                                  nThreads=16)
+    r = X @ beta - y
+    q0 = oracle.Q(B0, F0, r, r, nn["nnIndx"], nn["nnIndxLU"])
+    ll0 = -0.5 * ld0 - 0.5 * q0
+    out = {}
+    for prec in (0, 1):
+        with nngp.Model(c, nn["nnIndx"], nn["nnIndxLU"], m, cov, X, y,
+                        precision=prec) as md:
+            o = md.update(theta, slot=0, beta=beta)
+            B, F = md.get_bf(0)
+        ll = -0.5 * o["logDet"] - 0.5 * o["qResid"]
+        out[prec] = dict(ll=abs(ll - ll0) / abs(ll0),
+                         F=float(np.max(np.abs(F - F0) / F0)),
+                         B=float(np.max(np.abs(B - B0))),
+                         Bsite=per_site_rel(B, B0, nn["nnIndxLU"], n))
+    return out
+
+
+def test_config2_systems_fp64_and_mixed(oracle, nngp):
+    print("config 2 systems n=1e6 m=15", flush=True)
+    n, m = 1_000_000, 15
+    rng = np.random.default_rng(2)
+    c = x_ordered(rng.uniform(size=(n, 2)))[0]
+    beta = np.array([1.0, 2.0, -1.0])
+    X, y, nn = nngp_prior_data(oracle, c, m, 2.0, 12.0, 0.2, beta, rng)
+    e = loglik_pair(oracle, nngp, c, nn, X, y, m, 0, [2.0, 0.2, 12.0], beta)
+    print(f"  fp64 {e[0]}\n  mixed {e[1]}", flush=True)
+    assert e[0]["ll"] < RTOL and e[0]["F"] < RTOL and e[0]["Bsite"] < RTOL
+    assert e[1]["ll"] < 1e-8
+    assert e[1]["F"] < 1e-5 and e[1]["Bsite"] < 1e-4
+
+
+# ---------------------------------------------------------------------------
+# 3. conjugate grid: Matern 1.5, m = 15, k = 5, 20 grid points, n = 2e4
+# ---------------------------------------------------------------------------
+def r_crps(y, yhat, var):
+    sd = np.sqrt(var)
+    z = (y - yhat) / sd
+    return np.mean(-sd * (1 / np.sqrt(np.pi) - 2 * ss.norm.pdf(z)
+                          - z * (2 * ss.norm.cdf(z) - 1)))
+
+
+def test_config3_grid_every_fold_and_point(oracle, nngp):
+    print("config 3 grid n=2e4 m=15 k=5 g=20", flush=True)
+    n, p, m, k = 20_000, 3, 15, 5
+    rng = np.random.default_rng(3)
+    c = x_ordered(rng.uniform(size=(n, 2)))[0]
+    beta = np.array([1.0, 2.0, -1.0])
+    # y = X beta + w + e with X three N(0,1) columns (fitted as y ~ X - 1),
+    # w from the NNGP prior of Matern 1.5, phi 8, sigma.sq 1; tau.sq 0.1
+    X, y, _ = nngp_prior_data(oracle, c, m, 1.0, 8.0, 0.1, np.r_[0.0, beta],
+                              rng, cov=2, nu=1.5)
+    X = X[:, 1:].copy()
+    nho = n // k
+    ho = np.array([np.sort(rng.choice(n, nho, replace=False))
+                   for _ in range(k)], dtype=np.int32)
+    ig = [2.0, 1.0]
+    ta = np.array([[ph, al, 1.5] for ph in np.geomspace(1, 40, 5)
+                   for al in np.geomspace(0.01, 1, 4)])
+    g = ta.shape[0]
+    with nngp.CvGrid(y, X, c, m, ho, ig, "matern") as cv:
+        r = cv.run(ta, details=True, predictions=True)
+    want = np.zeros((g, 2))
+    worst = {}
+    for f in range(k):
+        keep = np.ones(n, bool)
+        keep[ho[f]] = False
+        cm, Xm, ym = c[keep], X[keep], y[keep]
+        c0, X0, y0 = c[ho[f]], X[ho[f]], y[ho[f]]
+        nn = oracle.mkNNIndxCB(cm, m, nThreads=16)
+        nn0, _ = oracle.nn2(cm, c0, m)
+        ref = oracle.cNNGP(ym, Xm.T.reshape(-1).copy(), p, len(ym),
+                           cm.T.reshape(-1).copy(), ta.reshape(-1).copy(),
+                           nn["nnIndx"], nn["nnIndxLU"],
+                           X0.T.reshape(-1).copy(), c0.T.reshape(-1).copy(),
+                           nho, nn0.T.reshape(-1).copy(), g, m, ig, 2,
+                           nThreads=16)
+        bref = ref["bB.inv"].T.reshape(g, p, p).transpose(0, 2, 1)
+        errs = dict(beta=max_rel(r["beta.hat"][f].T, ref["beta.hat"]),
+                    ab=max_rel(r["ab"][f].T, ref["ab"]),
+                    bBinv=max_rel(r["bB.inv"][f], bref),
+                    y0=max_rel(r["y.0.hat"][f].T, ref["y.0.hat"]),
+                    var=max_rel(r["y.0.hat.var"][f].T, ref["y.0.hat.var"]))
+        for key, v in errs.items():
+            worst[key] = max(worst.get(key, 0.0), v)
+        for j in range(g):
+            want[j, 0] += np.sqrt(np.mean((y0 - ref["y.0.hat"][:, j]) ** 2)) / k
+            want[j, 1] += r_crps(y0, ref["y.0.hat"][:, j],
+                                 ref["y.0.hat.var"][:, j]) / k
+    print(f"  worst over folds: {worst}", flush=True)
+    assert all(v < RTOL for v in worst.values())
+    assert max_rel(r["scores"], want) < RTOL
+
+
+# ---------------------------------------------------------------------------
+# 5. stress systems: clustered, x stretched by 3, m = 32, Bessel-K Matern
+# ---------------------------------------------------------------------------
+def clustered(n, rng):
+    nc = int(0.9 * n)
+    ce = rng.uniform(size=(50, 2))
+    c = np.vstack([ce[rng.integers(0, 50, nc)]
+                   + rng.normal(scale=0.02, size=(nc, 2)),
+                   rng.uniform(size=(n - nc, 2))])
+    c[:, 0] *= 3.0
+    return x_ordered(c)[0]
+
+
+@pytest.mark.parametrize("nu", [2.5, 2.37])
+def test_config5_systems_fp64(oracle, nngp, nu):
+    print(f"config 5 systems n=2e4 m=32 nu={nu}", flush=True)
+    n, m = 20_000, 32
+    rng = np.random.default_rng(5)
+    c = clustered(n, rng)
+    beta = np.array([1.0, 5.0])
+    # the draw carries a 1e-6 nugget: the nugget-free nu = 2.5 systems of
+    # sites ~1e-4 apart are singular to working precision
+    X, y, nn = nngp_prior_data(oracle, c, m, 1.0, 20.0, 0.05, beta, rng,
+                               cov=2, nu=nu, jitter=1e-6)
+    e = loglik_pair(oracle, nngp, c, nn, X, y, m, 2, [1.0, 0.05, 20.0, nu],
+                    beta)
+    print(f"  fp64 {e[0]}\n  mixed {e[1]}", flush=True)
+    assert e[0]["ll"] < RTOL and e[0]["F"] < RTOL and e[0]["Bsite"] < RTOL
+    assert e[1]["F"] < 5e-5 and e[1]["Bsite"] < 1e-3
+
+
+def test_config5_mixed_loglik_at_full_size(nngp):
+    print("config 5 mixed vs fp64 n=4e6 m=32", flush=True)
+    n, m = 4_000_000, 32
+    rng = np.random.default_rng(55)
+    c = clustered(n, rng)
+    X = np.column_stack([np.ones(n), rng.normal(size=n)])
+    idx = nngp.mkNNIndxCB(c, m)
+    B, F, _ = nngp.updateBF(c, idx["nnIndx"], idx["nnIndxLU"], m,
+                            [1.0, 1e-6, 20.0, 2.5], "matern")
+    w = nngp.prior_draw(B, F, idx["nnIndx"], idx["nnIndxLU"],
+                        rng.normal(size=n))
+    del B, F
+    y = X @ [1.0, 5.0] + w + rng.normal(scale=np.sqrt(0.05), size=n)
+    beta = np.array([1.0, 5.0])
+    ll = {}
+    for prec in (0, 1):
+        with nngp.Model(c, idx["nnIndx"], idx["nnIndxLU"], m, "matern", X, y,
+                        precision=prec) as md:
+            o = md.update([1.0, 0.05, 20.0, 2.5], slot=0, beta=beta)
+        ll[prec] = -0.5 * o["logDet"] - 0.5 * o["qResid"]
+    err = abs(ll[1] - ll[0]) / abs(ll[0])
+    print(f"  mixed loglik rel err {err:.2e}", flush=True)
+    assert err < 1e-8
