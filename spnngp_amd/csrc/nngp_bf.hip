@@ -64,16 +64,29 @@ __device__ __forceinline__ double dpp_mov(double v) {
   return __hiloint2double(hi, lo);
 }
 
+// The same where lanes outside ROWMASK may receive anything (no value is
+// prepared for them): for sums whose result is read in lanes the move
+// reaches.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_mov_any(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, ROWMASK,
+                                          0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, ROWMASK,
+                                          0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
 // Sum over the 64 lanes with DPP moves only (no LDS round trips); the
-// total is valid in lane 63.  Fixed order: pairs, quads, half rows, rows,
-// then rows 0+1 and 2+3, then all four.
+// total is valid in lane 63 (other lanes hold partial or meaningless
+// sums).  Fixed order: pairs, quads, half rows, rows, then rows 0+1 and
+// 2+3, then all four.
 __device__ __forceinline__ double wave_sum_to63(double x) {
-  x += dpp_mov<0xB1, 0xF>(x);   // quad_perm [1,0,3,2]
-  x += dpp_mov<0x4E, 0xF>(x);   // quad_perm [2,3,0,1]
-  x += dpp_mov<0x141, 0xF>(x);  // row_half_mirror
-  x += dpp_mov<0x140, 0xF>(x);  // row_mirror
-  x += dpp_mov<0x142, 0xA>(x);  // row_bcast:15 into rows 1 and 3
-  x += dpp_mov<0x143, 0xC>(x);  // row_bcast:31 into rows 2 and 3
+  x += dpp_mov_any<0xB1, 0xF>(x);   // quad_perm [1,0,3,2]
+  x += dpp_mov_any<0x4E, 0xF>(x);   // quad_perm [2,3,0,1]
+  x += dpp_mov_any<0x141, 0xF>(x);  // row_half_mirror
+  x += dpp_mov_any<0x140, 0xF>(x);  // row_mirror
+  x += dpp_mov_any<0x142, 0xA>(x);  // row_bcast:15 into rows 1 and 3
+  x += dpp_mov_any<0x143, 0xC>(x);  // row_bcast:31 into rows 2 and 3
   return x;
 }
 
@@ -145,6 +158,80 @@ __device__ inline void block_site_terms(int nz, double *rows, int rows_cap,
     double s = 0.0;
     if (k != 1)
       for (int ww = 0; ww < nw; ww++) s += red[ww][k];
+    rows[(size_t)k * rows_cap + row] = s;
+  }
+}
+
+// Number of per-site terms of a partial row (column 1, always 0, aside):
+// log F, then the nz(nz+1)/2 cross-products.
+__host__ __device__ constexpr int site_terms(int nz) {
+  return 1 + nz * (nz + 1) / 2;
+}
+
+// block_site_terms through LDS.  Each lane writes its terms to its wave's
+// area `w` (term v of lane t at w[v * 64 + t], site_terms(nz) x 64 doubles),
+// each quad of lanes then sums one term over 16 lanes apiece in lane order
+// and combines its four partials (pairs, then the two pairs), and with more
+// than one wave per block the wave sums meet in LDS and are added in wave
+// order.  The order is fixed, so a launch shape's sums are reproducible.
+// `w` may be memory the wave has finished reading (the staged covariate
+// rows); the barrier before the writes also keeps those reads first.
+__device__ inline void block_site_terms_lds(int nz, double *rows,
+                                            int rows_cap, bool valid,
+                                            double F,
+                                            const double (&e)[kMaxNZ],
+                                            int row, double *w) {
+  __shared__ double wsum[kLaneMaxThreads / 64][kStatsMax];
+  const int nw = (int)blockDim.x >> 6;
+  const int wv = (int)threadIdx.x >> 6, t = (int)threadIdx.x & 63;
+  const int nt = site_terms(nz);
+  const double rf = valid ? 1.0 / F : 0.0;
+  __syncthreads();
+  w[t] = valid ? log(F) : 0.0;
+  int g = 1;
+#pragma unroll
+  for (int p = 0; p < kMaxNZ; p++) {
+    if (p < nz) {
+#pragma unroll
+      for (int q = 0; q <= p; q++) w[(g + q) * 64 + t] = e[p] * e[q] * rf;
+    }
+    g += p + 1;
+  }
+  __syncthreads();
+  for (int v0 = 0; v0 < nt; v0 += 16) {
+    const int v = v0 + (t >> 2), qd = t & 3;
+    double s = 0.0;
+    if (v < nt) {
+      const double2 *src = reinterpret_cast<const double2 *>(w + v * 64 +
+                                                             qd * 16);
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const double2 x = src[u];
+        s += x.x;
+        s += x.y;
+      }
+    }
+    s += dpp_mov_any<0xB1, 0xF>(s);  // quad_perm [1,0,3,2]
+    s += dpp_mov_any<0x4E, 0xF>(s);  // quad_perm [2,3,0,1]
+    if (qd == 0 && v < nt) {
+      if (nw == 1)
+        rows[(size_t)(v == 0 ? 0 : v + 1) * rows_cap + row] = s;
+      else
+        wsum[wv][v] = s;
+    }
+  }
+  if (nw == 1) {
+    if (t == 0) rows[(size_t)rows_cap + row] = 0.0;
+    return;
+  }
+  __syncthreads();
+  const int k = (int)threadIdx.x;
+  if (k <= nt) {  // column k of the row: term k - 1 beyond column 1
+    double s = 0.0;
+    if (k != 1) {
+      const int v = k == 0 ? 0 : k - 1;
+      for (int ww = 0; ww < nw; ww++) s += wsum[ww][v];
+    }
     rows[(size_t)k * rows_cap + row] = s;
   }
 }
@@ -284,6 +371,7 @@ __device__ __forceinline__ void lane_site(const BfArgs &A, const BfProblem &P,
 
 template <int M, int COV, bool BATCH>
 __global__ __launch_bounds__(kLaneMaxThreads) void k_bf_lane(BfArgs A) {
+  extern __shared__ double lane_red[];  // site_terms(nz) x 64 per wave
   const int nt = (int)blockDim.x;
   const int b = xcd_block((A.n + nt - 1) / nt);
   if (b < 0) return;  // whole padding block
@@ -311,7 +399,11 @@ __global__ __launch_bounds__(kLaneMaxThreads) void k_bf_lane(BfArgs A) {
     lane_site<M, COV, BATCH>(A, P, c, i, cnt, off, nb, Fi, e);
     if (!BATCH && A.F) A.F[i] = Fi;
   }
-  if (P.rows) block_site_terms(A.nz, P.rows, A.rows_cap, valid, Fi, e, b);
+  if (P.rows) {
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    block_site_terms_lds(A.nz, P.rows, A.rows_cap, valid, Fi, e, b,
+                         lane_red + (size_t)w * site_terms(A.nz) * 64);
+  }
 }
 
 // k_bf_lane with the covariate rows staged in LDS: the block's dynamic LDS
@@ -349,7 +441,11 @@ __global__ __launch_bounds__(kLaneMaxThreads) void k_bf_lane_lds(BfArgs A) {
     lane_site<M, COV, BATCH, ZH>(A, P, c, i, cnt, off, nb, Fi, e, stg);
     if (!BATCH && A.F) A.F[i] = Fi;
   }
-  if (P.rows) block_site_terms(A.nz, P.rows, A.rows_cap, valid, Fi, e, b);
+  // the sums reuse the wave's staging area (the residuals have read it);
+  // site_terms(nz) <= 11 doubles a lane fit in (M + 1) x ZH x 16 bytes
+  if (P.rows)
+    block_site_terms_lds(A.nz, P.rows, A.rows_cap, valid, Fi, e, b,
+                         reinterpret_cast<double *>(stg));
 }
 
 // ---------------------------------------------------------------------------
@@ -943,12 +1039,15 @@ int bf_rows(const BfArgs &a, const BfConfig &cfg) {
 template <int M, bool BATCH>
 static void launch_lane_m(const BfArgs &a, int variant, const dim3 &grid,
                           const dim3 &blk, hipStream_t st) {
+  // LDS of the block sums: site_terms(nz) doubles a lane
+  const size_t shm =
+      a.rows ? (size_t)blk.x * site_terms(a.nz) * sizeof(double) : 0;
   switch (variant) {
-    case 0: hipLaunchKernelGGL((k_bf_lane<M, 0, BATCH>), grid, blk, 0, st, a); break;
-    case 1: hipLaunchKernelGGL((k_bf_lane<M, 1, BATCH>), grid, blk, 0, st, a); break;
-    case kCovMatern15: hipLaunchKernelGGL((k_bf_lane<M, kCovMatern15, BATCH>), grid, blk, 0, st, a); break;
-    case kCovMatern25: hipLaunchKernelGGL((k_bf_lane<M, kCovMatern25, BATCH>), grid, blk, 0, st, a); break;
-    default: hipLaunchKernelGGL((k_bf_lane<M, 3, BATCH>), grid, blk, 0, st, a); break;
+    case 0: hipLaunchKernelGGL((k_bf_lane<M, 0, BATCH>), grid, blk, shm, st, a); break;
+    case 1: hipLaunchKernelGGL((k_bf_lane<M, 1, BATCH>), grid, blk, shm, st, a); break;
+    case kCovMatern15: hipLaunchKernelGGL((k_bf_lane<M, kCovMatern15, BATCH>), grid, blk, shm, st, a); break;
+    case kCovMatern25: hipLaunchKernelGGL((k_bf_lane<M, kCovMatern25, BATCH>), grid, blk, shm, st, a); break;
+    default: hipLaunchKernelGGL((k_bf_lane<M, 3, BATCH>), grid, blk, shm, st, a); break;
   }
 }
 
