@@ -660,7 +660,8 @@ __host__ __device__ inline int coop_wave_doubles(int m) {
 template <int MM>
 __device__ inline bool coop_site(const BfArgs &A, const BfProblem &P,
                                  const CovArgs &c, const BesselTab &bt,
-                                 double *Am, int i, double &F,
+                                 const MaternTab &mt, double *Am, int i,
+                                 double &F,
                                  double (&e)[kMaxNZ], double &bcoef,
                                  int &cnt_out, int &off_out) {
   const int t = lane_opaque((int)threadIdx.x & 63);
@@ -701,7 +702,7 @@ __device__ inline bool coop_site(const BfArgs &A, const BfProblem &P,
     } else {
       const double ax = k < cnt ? px[k] : s.x;
       const double ay = k < cnt ? py[k] : s.y;
-      const double val = s2_corr_rt(sqdist(ax, ay, px[l], py[l]), c, bt);
+      const double val = s2_corr_rt(sqdist(ax, ay, px[l], py[l]), c, bt, mt);
       if (k < cnt)
         Am[k * ld + l] = val;
       else
@@ -855,12 +856,23 @@ __device__ inline bool coop_site_mixed(const BfArgs &A, const BfProblem &P,
 // Logical block b covers sites [b*chunk, (b+1)*chunk); its waves take every
 // nw-th site of that run.  A.coop_blocks logical blocks per problem.
 // MIXED: the sites are solved by coop_site_mixed (single problem only).
+// The block's matern tables: FP64 passes the Bessel-K loop divisors and the
+// pass's correlation table, mixed passes the FP32 divisors.
+template <bool MIXED>
+struct CoopTabs {
+  BesselTab bt;
+  MaternTab mt;
+};
+template <>
+struct CoopTabs<true> {
+  BesselTab32 bt32;
+};
+
 template <bool BATCH, int MM, bool MIXED>
 __global__ __launch_bounds__(64 * kCoopMaxWaves) void k_bf_coop(BfArgs A) {
   extern __shared__ double smem[];
   __shared__ double red[kCoopMaxWaves][kStatsMax];
-  __shared__ BesselTab bt;
-  __shared__ BesselTab32 bt32;
+  __shared__ CoopTabs<MIXED> tabs;
   const int nw = (int)blockDim.x >> 6;
   const int w = (int)threadIdx.x >> 6;
   const int t = (int)threadIdx.x & 63;
@@ -869,11 +881,16 @@ __global__ __launch_bounds__(64 * kCoopMaxWaves) void k_bf_coop(BfArgs A) {
   const BfProblem P = bf_problem<BATCH>(A);
   const CovArgs c = BATCH ? A.covs[(int)blockIdx.y % A.ng] : A.c;
   if (c.cov == 2) {  // the matern's loop divisors for this block's mu
-    if constexpr (MIXED)
-      bessel_tab_fill32(bt32, c);
-    else
-      bessel_tab_fill(bt, c);
-    __syncthreads();
+    if constexpr (MIXED) {
+      bessel_tab_fill32(tabs.bt32, c);
+      __syncthreads();
+    } else {
+      bessel_tab_fill(tabs.bt, c);
+      __syncthreads();
+      // and the pass's correlation table, built in the waves' system areas
+      // (at least kMatScratch doubles, see the launcher) before their use
+      matern_tab_fill(tabs.mt, c, tabs.bt, smem);
+    }
   }
   double *Am = smem + (size_t)w * coop_wave_doubles(A.m);
   float *Wf = reinterpret_cast<float *>(smem) +
@@ -892,10 +909,11 @@ __global__ __launch_bounds__(64 * kCoopMaxWaves) void k_bf_coop(BfArgs A) {
     int cnt, off;
     bool ok;
     if constexpr (MIXED)
-      ok = coop_site_mixed<MM>(A, P, c, c32, bt32, Wf, i, F, e, bcoef, cnt,
-                               off);
+      ok = coop_site_mixed<MM>(A, P, c, c32, tabs.bt32, Wf, i, F, e, bcoef,
+                               cnt, off);
     else
-      ok = coop_site<MM>(A, P, c, bt, Am, i, F, e, bcoef, cnt, off);
+      ok = coop_site<MM>(A, P, c, tabs.bt, tabs.mt, Am, i, F, e, bcoef, cnt,
+                         off);
     if (!ok && t == 0) atomicMin(P.err, i);
     if (!BATCH) {
       if (A.B && ok && t < cnt) A.B[off + t] = bcoef;
@@ -995,9 +1013,14 @@ bool bf_lane_ok(int m, int variant) { return m <= 16 && variant >= 0; }
 static int lane_M(int m);
 static size_t mixed_copy_shm(int m, int threads);
 
-// LDS per block of the wavefront kernel with `waves` waves.
+// Scratch of matern_tab_fill, in doubles.
+constexpr int kMatScratch = 2 * kMatPieces * kMatN + kMatN * kMatN;
+
+// LDS per block of the wavefront kernel with `waves` waves (FP64; its
+// system areas also hold the matern table's build).
 static size_t coop_shm(int m, int waves) {
-  return (size_t)waves * coop_wave_doubles(m) * sizeof(double);
+  const size_t d = (size_t)waves * coop_wave_doubles(m);
+  return (d > (size_t)kMatScratch ? d : (size_t)kMatScratch) * sizeof(double);
 }
 
 bool bf_config_ok(const BfConfig &cfg, int m, int variant) {

@@ -319,6 +319,113 @@ __device__ inline double matern_s2_tab(double d, const CovArgs &a,
   return a.mnorm * xnu_bessel_k(x, a, t);
 }
 
+// ---------------------------------------------------------------------------
+// the general-nu matern as a table per pass
+// ---------------------------------------------------------------------------
+// s2 * rho(x) = mnorm * x^nu K_nu(x) for x in [2^-30, 2^6) as polynomials,
+// one per half octave: x = m 2^e with m in [1, sqrt 2) or [sqrt 2, 2), and
+// on each piece a degree-15 polynomial in u = (m - c) / h in [-1, 1], the
+// interpolant at the 16 Chebyshev points (built by matern_tab_fill from
+// xnu_bessel_k).  x^nu K_nu(x) = A(x^2) + x^(2 nu) B(x^2) with A, B entire,
+// so its only singularity is x = 0, three half-widths from the nearest
+// piece's centre on this grid; the interpolants converge like 11.6^-n and
+// agree with the series to ~1e-14 relative (~5e-15 absolute).  Outside the
+// range (near-coincident sites, or x >= 64 where the entry is below 1e-25)
+// the series itself is used.
+constexpr int kMatOct0 = -30;             // first octave 2^-30
+constexpr int kMatOct = 36;               // octaves, up to 2^6
+constexpr int kMatDeg = 15;
+constexpr int kMatN = kMatDeg + 1;        // nodes and coefficients a piece
+constexpr int kMatPieces = 2 * kMatOct;
+constexpr double kSqrt2 = 1.4142135623730951;
+
+struct MaternTab {
+  double a[kMatPieces][kMatN];  // monomial coefficients in u, s2 folded in
+};
+
+// Piece of mantissa m in [1, 2): lo [1, sqrt 2), hi [sqrt 2, 2); u = m / h
+// - c / h with centre c and half-width h.
+__host__ __device__ constexpr double mat_c(int hi) {
+  return hi ? (kSqrt2 + 2.0) * 0.5 : (1.0 + kSqrt2) * 0.5;
+}
+__host__ __device__ constexpr double mat_h(int hi) {
+  return hi ? (2.0 - kSqrt2) * 0.5 : (kSqrt2 - 1.0) * 0.5;
+}
+
+// Monomial coefficients of the Chebyshev polynomials T_0 .. T_15 (exact
+// integers): T_{k+1} = 2u T_k - T_{k-1}.
+struct ChebMonomials {
+  double t[kMatN][kMatN];
+  constexpr ChebMonomials() : t() {
+    t[0][0] = 1.0;
+    t[1][1] = 1.0;
+    for (int k = 2; k < kMatN; k++)
+      for (int i = 0; i < kMatN; i++)
+        t[k][i] = (i > 0 ? 2.0 * t[k - 1][i - 1] : 0.0) - t[k - 2][i];
+  }
+};
+__device__ constexpr ChebMonomials kChebMono{};
+
+// Fills T for the pass's matern (every thread of the block calls it, after
+// the BesselTab bt is filled and visible).  scratch: 2 kMatPieces kMatN +
+// kMatN^2 doubles of LDS that are free until the call returns.
+__device__ inline void matern_tab_fill(MaternTab &T, const CovArgs &a,
+                                       const BesselTab &bt, double *scratch) {
+  constexpr double kPi = 3.14159265358979323846;
+  double *fv = scratch;                             // node values
+  double *cc = scratch + kMatPieces * kMatN;        // Chebyshev coefficients
+  double *cs = cc + kMatPieces * kMatN;             // cos(k theta_j)
+  const int nt = (int)blockDim.x, t0 = (int)threadIdx.x;
+  for (int q = t0; q < kMatN * kMatN; q += nt) {
+    const int k = q / kMatN, j = q % kMatN;
+    cs[q] = cos(kPi * k * (j + 0.5) / kMatN);
+  }
+  for (int q = t0; q < kMatPieces * kMatN; q += nt) {
+    const int pc = q / kMatN, j = q % kMatN, hi = pc & 1;
+    const double u = cos(kPi * (j + 0.5) / kMatN);
+    const double x = ldexp(mat_c(hi) + mat_h(hi) * u, kMatOct0 + pc / 2);
+    fv[q] = a.mnorm * xnu_bessel_k(x, a, bt);
+  }
+  __syncthreads();
+  for (int q = t0; q < kMatPieces * kMatN; q += nt) {
+    const int pc = q / kMatN, k = q % kMatN;
+    double s = 0.0;
+    for (int j = 0; j < kMatN; j++) s = fma(fv[pc * kMatN + j], cs[k * kMatN + j], s);
+    cc[q] = s * (k == 0 ? 1.0 / kMatN : 2.0 / kMatN);
+  }
+  __syncthreads();
+  for (int q = t0; q < kMatPieces * kMatN; q += nt) {
+    const int pc = q / kMatN, i = q % kMatN;
+    double s = 0.0;
+    for (int k = i; k < kMatN; k++) s = fma(cc[pc * kMatN + k], kChebMono.t[k][i], s);
+    T.a[pc][i] = s;
+  }
+  __syncthreads();
+}
+
+// s2 * matern correlation at distance d from the table (the series outside
+// its range).
+__device__ __forceinline__ double matern_s2_fast(double d, const CovArgs &a,
+                                                 const BesselTab &bt,
+                                                 const MaternTab &T) {
+  const double x = d * a.phi;
+  if (!(x > 0.0)) return a.s2;
+  const int o = __builtin_amdgcn_frexp_exp(x) - 1 - kMatOct0;
+  if ((unsigned)o >= (unsigned)kMatOct) {
+    if (x > 700.0) return 0.0;
+    return a.mnorm * xnu_bessel_k(x, a, bt);
+  }
+  const double m = 2.0 * __builtin_amdgcn_frexp_mant(x);  // in [1, 2)
+  const int hi = m >= kSqrt2 ? 1 : 0;
+  const double u = hi ? fma(m, 1.0 / mat_h(1), -mat_c(1) / mat_h(1))
+                      : fma(m, 1.0 / mat_h(0), -mat_c(0) / mat_h(0));
+  const double *p = T.a[2 * o + hi];
+  double r = p[kMatDeg];
+#pragma unroll
+  for (int k = kMatDeg - 1; k >= 0; k--) r = fma(r, u, p[k]);
+  return r;
+}
+
 // Covariance variants of the templated kernels: the four covModels, plus
 // the matern at nu = 1.5 and nu = 2.5 in closed form, (1 + x) e^-x and
 // (1 + x + x^2/3) e^-x with x = phi D (the Bessel-K expression of
@@ -402,6 +509,13 @@ __device__ inline double s2_corr_rt(double d2, const CovArgs &a) {
 __device__ inline double s2_corr_rt(double d2, const CovArgs &a,
                                     const BesselTab &t) {
   if (a.cov == 2) return matern_s2_tab(nn_sqrt(d2), a, t);
+  return s2_corr_rt(d2, a);
+}
+
+// The same, with the matern from the block's table of the pass.
+__device__ inline double s2_corr_rt(double d2, const CovArgs &a,
+                                    const BesselTab &t, const MaternTab &mt) {
+  if (a.cov == 2) return matern_s2_fast(nn_sqrt(d2), a, t, mt);
   return s2_corr_rt(d2, a);
 }
 
