@@ -474,6 +474,9 @@ typedef struct nngp_server_stats {
   double site_samples;        /* q * nSamples summed over finished requests */
   unsigned long long device_bytes_used, device_bytes_peak;
   unsigned long long host_bytes_used, host_bytes_peak;
+  double kernel_ms;           /* device time of the retired kriging launches */
+  double kernel_site_samples; /* their sites x nSamples                      */
+  long long done_unreported;  /* finished requests no wait has handed out   */
 } nngp_server_stats;
 
 /* X n x p and coords n x 2 column-major (the fitted rows in any fixed

@@ -87,7 +87,10 @@ class NngpServerStats(C.Structure):
                 ("device_bytes_used", C.c_ulonglong),
                 ("device_bytes_peak", C.c_ulonglong),
                 ("host_bytes_used", C.c_ulonglong),
-                ("host_bytes_peak", C.c_ulonglong)]
+                ("host_bytes_peak", C.c_ulonglong),
+                ("kernel_ms", C.c_double),
+                ("kernel_site_samples", C.c_double),
+                ("done_unreported", C.c_longlong)]
 
 
 class NngpCvGridDetail(C.Structure):

@@ -125,7 +125,8 @@ static int mcmc_loop(Model &md, const McmcSpec &S, Rng &R,
   PassOut pc;  // current slot's logDet and cross-products
   NNGP_TRY(md.pass(cov_of(theta), cur, false, &pc));
   std::vector<double> tmp_pp((size_t)p * p, 0.0), tmp_p(p), tmp_p2(p);
-  std::vector<double> hB, hF, wrep;
+  RepDrawer rep;  // fit.rep: the replicate draw on the device
+  if (S.nRep > 0) NNGP_TRY(rep.init(n, S.nnIndx, S.nnIndxLU));
   bool thetaUpdate = true;
   int accept = 0, batchAccept = 0, status = 0, repCnt = 0;
 
@@ -155,23 +156,16 @@ static int mcmc_loop(Model &md, const McmcSpec &S, Rng &R,
       for (int i = 0; i < n; i++) yr[i] = 0.0;
       for (int j = 0; j < p; j++)
         for (int i = 0; i < n; i++) yr[i] += beta[j] * S.X[(size_t)j * n + i];
+      // B of the current theta (the slot's passes do not store it), the n
+      // normals in site order from the chain's stream, then the draw on the
+      // device; only the n-vector comes back
       NNGP_TRY(md.pass(cov_of(theta), cur, true, nullptr));
-      hB.resize((size_t)md.nIndx);
-      hF.resize((size_t)n);
-      NNGP_HIP(hipMemcpy(hB.data(), md.B[cur].p, sizeof(double) * hB.size(),
-                         hipMemcpyDeviceToHost));
-      NNGP_HIP(hipMemcpy(hF.data(), md.F[cur].p, sizeof(double) * n,
-                         hipMemcpyDeviceToHost));
-      wrep.assign(n, 0.0);
-      for (int i = 0; i < n; i++) {
-        const double z = R.rnorm(0.0, 1.0);
-        double acc = 0.0;
-        const int off = S.nnIndxLU[i], cnt = S.nnIndxLU[(size_t)n + i];
-        for (int j = 0; j < cnt; j++)
-          acc += hB[off + j] * wrep[S.nnIndx[off + j]];
-        wrep[i] = acc + std::sqrt(hF[i]) * z;
-      }
-      for (int i = 0; i < n; i++) yr[i] += wrep[i];
+      double *z = rep.z();
+      for (int i = 0; i < n; i++) z[i] = R.rnorm(0.0, 1.0);
+      NNGP_TRY(rep.draw(md.B[cur].as<double>(), md.F[cur].as<double>(),
+                        md.nnIndx.as<int>(), md.lu.as<int>(), md.st.s));
+      const double *w = rep.v();
+      for (int i = 0; i < n; i++) yr[i] += w[i];
       repCnt++;
     }
 

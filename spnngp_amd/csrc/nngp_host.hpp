@@ -260,6 +260,22 @@ double gram_at(const double *raw, int a, int b);
 // Q(r, r) for r = X beta - y: beta' G beta - 2 beta' h + y' V^-1 y
 double quad_resid(const PassOut &po, const double *beta, int p);
 
+// One replicate draw v_i = B_i . v_N(i) + sqrt(F_i) z_i (i = 0 .. n-1, the
+// updateRep of src/rNNGP.cpp:24-42) on the device from device B and F,
+// with the caller's n normals: the caller fills z() (pinned) in site order,
+// draw() uploads them, walks the neighbor DAG level by level (the level
+// schedule built once by init from the host index) and leaves v in v().
+struct RepDrawer {
+  int n = 0, nlev = 0;
+  DevBuf dorder, dloff, dz, dv, dy;
+  PinnedBuf hz, hv;
+  int init(int n, const int *nnIndx, const int *nnIndxLU);
+  double *z() { return hz.as<double>(); }
+  const double *v() const { return hv.as<double>(); }
+  int draw(const double *B, const double *F, const int *nnIndx,
+           const int *lu, hipStream_t st);
+};
+
 struct Model {
   int n = 0, m = 0, p = 0, nz = 0, zs = 0, cov = 0, precision = 0;
   long long nIndx = 0;

@@ -103,6 +103,56 @@ static int build_levels(int n, const int *nnIndx, const int *lu,
   return NNGP_OK;
 }
 
+int RepDrawer::init(int n_, const int *nnIndx, const int *lu) {
+  n = n_;
+  std::vector<int> order, loff;
+  NNGP_TRY(build_levels(n, nnIndx, lu, order, loff));
+  nlev = (int)loff.size() - 1;
+  NNGP_TRY(dorder.alloc(sizeof(int) * (size_t)n));
+  NNGP_TRY(dloff.alloc(sizeof(int) * loff.size()));
+  NNGP_HIP(hipMemcpy(dorder.p, order.data(), dorder.bytes,
+                     hipMemcpyHostToDevice));
+  NNGP_HIP(hipMemcpy(dloff.p, loff.data(), sizeof(int) * loff.size(),
+                     hipMemcpyHostToDevice));
+  NNGP_TRY(dz.alloc(sizeof(double) * (size_t)n));
+  NNGP_TRY(dv.alloc(sizeof(double) * (size_t)n));
+  NNGP_TRY(dy.alloc(sizeof(double) * (size_t)n));
+  NNGP_TRY(hz.alloc(sizeof(double) * (size_t)n));
+  NNGP_TRY(hv.alloc(sizeof(double) * (size_t)n));
+  return NNGP_OK;
+}
+
+int RepDrawer::draw(const double *B, const double *F, const int *nnIndx,
+                    const int *lu, hipStream_t st) {
+  NNGP_HIP(hipMemcpyAsync(dz.p, hz.p, sizeof(double) * (size_t)n,
+                          hipMemcpyHostToDevice, st));
+  RepArgs A;
+  A.B = B;
+  A.F = F;
+  A.z = dz.as<double>();
+  A.X = nullptr;
+  A.beta = nullptr;
+  A.nnIndx = nnIndx;
+  A.lu = lu;
+  A.order = dorder.as<int>();
+  A.loff = dloff.as<int>();
+  A.v = dv.as<double>();
+  A.y = dy.as<double>();
+  A.bstride = 0;
+  A.n = n;
+  A.nlev = nlev;
+  A.p = 0;  // y = v
+  A.nS = 1;
+  A.s0 = 0;
+  A.seed = 0;
+  hipLaunchKernelGGL(k_rep_draw, dim3(1), dim3(kRepThreads), 0, st, A);
+  NNGP_HIP(hipGetLastError());
+  NNGP_HIP(hipMemcpyAsync(hv.p, dy.p, sizeof(double) * (size_t)n,
+                          hipMemcpyDeviceToHost, st));
+  NNGP_HIP(hipStreamSynchronize(st));
+  return NNGP_OK;
+}
+
 }  // namespace nngp
 
 using namespace nngp;

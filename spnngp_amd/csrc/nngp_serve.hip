@@ -21,6 +21,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <list>
 #include <map>
 #include <mutex>
 #include <thread>
@@ -113,6 +114,8 @@ struct Req {
   int state = kPending;
   int status = NNGP_OK;
   bool reported = false;
+  bool in_done_q = false;             // finished, not yet reported
+  std::list<long long>::iterator dq;  // its place in done_q
 };
 
 struct Part {
@@ -123,7 +126,7 @@ struct Part {
 struct Slot {
   PinnedBuf hseg;
   DevBuf dseg;
-  hipEvent_t kev = nullptr, oev = nullptr;
+  hipEvent_t ks = nullptr, kev = nullptr, oev = nullptr;  // ks..kev: kernel
   std::vector<Part> parts;
 };
 
@@ -146,7 +149,9 @@ struct Server {
   std::deque<Req *> pending;
   std::vector<Req *> active;  // scheduler thread only
   std::map<long long, Req *> reqs;
-  std::deque<long long> done_q;
+  // finished requests not yet handed out by a wait, in completion order
+  // (a request leaves it when a wait reports it or when it is released)
+  std::list<long long> done_q;
   long long next_ticket = 0;
   bool stop = false;
   std::thread worker;
@@ -154,6 +159,14 @@ struct Server {
   // statistics
   long long n_submitted = 0, n_completed = 0, n_launches = 0;
   double site_samples = 0.0;
+  double kernel_ms = 0.0, kernel_site_samples = 0.0;  // retired launches
+
+  void unlist_locked(Req *r) {
+    if (r->in_done_q) {
+      done_q.erase(r->dq);
+      r->in_done_q = false;
+    }
+  }
 
   int create(const double *Xh, const double *yh, const double *coords, int n_,
              int p_, int m_, const double *betaS, const double *thetaS,
@@ -225,7 +238,8 @@ int Server::create(const double *Xh, const double *yh, const double *coords,
   for (int k = 0; k < depth; k++) {
     NNGP_TRY(slots[k].hseg.alloc(sizeof(PredSeg) * kMaxSegs));
     NNGP_TRY(slots[k].dseg.alloc(sizeof(PredSeg) * kMaxSegs));
-    NNGP_HIP(hipEventCreateWithFlags(&slots[k].kev, hipEventDisableTiming));
+    NNGP_HIP(hipEventCreate(&slots[k].ks));
+    NNGP_HIP(hipEventCreate(&slots[k].kev));
     NNGP_HIP(hipEventCreateWithFlags(&slots[k].oev, hipEventDisableTiming));
     free_slots.push_back(depth - 1 - k);
   }
@@ -321,6 +335,7 @@ int Server::launch(int k) {
   a.nseg = (int)sl.parts.size();
   a.site0 = 0;
   a.nsite = blocks;
+  NNGP_HIP(hipEventRecord(sl.ks, s_k.s));
   if (pred_launch(a, s_k.s)) return set_err(NNGP_ERR_HIP, "kriging launch failed");
   NNGP_HIP(hipEventRecord(sl.kev, s_k.s));
   NNGP_HIP(hipStreamWaitEvent(s_out.s, sl.kev, 0));
@@ -348,7 +363,8 @@ void Server::complete_locked(Req *r, int status) {
     r->d_held = false;
   }
   active.erase(std::remove(active.begin(), active.end(), r), active.end());
-  done_q.push_back(r->ticket);
+  r->dq = done_q.insert(done_q.end(), r->ticket);
+  r->in_done_q = true;
   n_completed++;
   if (status == NNGP_OK) site_samples += (double)r->q * S;
   cv_done.notify_all();
@@ -360,6 +376,14 @@ void Server::retire_locked() {
     const hipError_t e = hipEventQuery(slots[k].oev);
     if (e == hipErrorNotReady) break;
     inflight.pop_front();
+    if (e == hipSuccess) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, slots[k].ks, slots[k].kev) == hipSuccess)
+        kernel_ms += ms;
+      long long sites = 0;
+      for (const Part &pt : slots[k].parts) sites += pt.count;
+      kernel_site_samples += (double)sites * S;
+    }
     for (const Part &pt : slots[k].parts) {
       Req *r = pt.r;
       r->copied += pt.count;
@@ -476,6 +500,7 @@ Server::~Server() {
     delete kv.second;
   }
   for (Slot &sl : slots) {
+    if (sl.ks) (void)hipEventDestroy(sl.ks);
     if (sl.kev) (void)hipEventDestroy(sl.kev);
     if (sl.oev) (void)hipEventDestroy(sl.oev);
   }
@@ -585,11 +610,7 @@ int nngp_server_wait(nngp_server *srv, long long *ticket, double timeout_s,
       auto it = sv.reqs.find(want);
       return it == sv.reqs.end() || it->second->state == kDone;
     }
-    for (long long t : sv.done_q) {
-      auto it = sv.reqs.find(t);
-      if (it != sv.reqs.end() && !it->second->reported) return true;
-    }
-    return false;
+    return !sv.done_q.empty();
   };
   if (timeout_s < 0) {
     sv.cv_done.wait(lk, ready);
@@ -604,17 +625,9 @@ int nngp_server_wait(nngp_server *srv, long long *ticket, double timeout_s,
       return set_err(NNGP_ERR_ARG, "ticket %lld was released meanwhile", want);
     r = it->second;
   } else {
-    while (!sv.done_q.empty()) {
-      const long long t = sv.done_q.front();
-      sv.done_q.pop_front();
-      auto it = sv.reqs.find(t);
-      if (it != sv.reqs.end() && !it->second->reported) {
-        r = it->second;
-        break;
-      }
-    }
-    if (!r) return set_err(NNGP_ERR_ARG, "no finished request");
+    r = sv.reqs.at(sv.done_q.front());
   }
+  sv.unlist_locked(r);
   r->reported = true;
   *ticket = r->ticket;
   if (info) {
@@ -639,6 +652,7 @@ int nngp_server_release(nngp_server *srv, long long ticket) {
   Req *r = it->second;
   if (r->state != kDone || r->parts_out > 0)
     return set_err(NNGP_ERR_ARG, "request %lld is still running", ticket);
+  sv.unlist_locked(r);
   if (r->h_err) sv.halloc.release(r->h_off, r->h_bytes);
   if (r->ready) (void)hipEventDestroy(r->ready);
   sv.reqs.erase(it);
@@ -684,5 +698,8 @@ int nngp_server_get_stats(nngp_server *srv, nngp_server_stats *st) {
   st->device_bytes_peak = sv.dalloc.peak();
   st->host_bytes_used = sv.halloc.used();
   st->host_bytes_peak = sv.halloc.peak();
+  st->kernel_ms = sv.kernel_ms;
+  st->kernel_site_samples = sv.kernel_site_samples;
+  st->done_unreported = (long long)sv.done_q.size();
   return NNGP_OK;
 }

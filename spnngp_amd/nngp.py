@@ -47,10 +47,17 @@ def _cm(a):
 
 
 def _opts(precision: int = 0, device: int = 0, seed: int = 1, shape=None,
-          sweep: int = 0, search: int = 0):
+          sweep: int = 0, search: int = 0, interrupt=None):
+    """nngp_opts.  interrupt: a callable polled once per MCMC iteration or
+    grid point (nonzero stops the call with NNGP_ERR_INTERRUPT); the
+    returned structure keeps its C wrapper alive."""
     sh = (C.c_int * 3)(*(shape if shape is not None else (0, 0, 0)))
-    return NngpOpts(precision, device, seed, _lib._INT_FN(), None, sh, sweep,
-                    search, (C.c_int * 3)())
+    fn = _lib._INT_FN(lambda _arg: int(bool(interrupt()))) if interrupt \
+        else _lib._INT_FN()
+    o = NngpOpts(precision, device, seed, fn, None, sh, sweep, search,
+                 (C.c_int * 3)())
+    o._keep = fn
+    return o
 
 
 def device_count() -> int:
@@ -204,11 +211,13 @@ def rNNGP(y, X, p, n, m, coords, covModel, nnIndx, nnIndxLU, sigmaSqIG,
           tauSqIG, phiUnif, nuUnif, betaStarting, sigmaSqStarting,
           tauSqStarting, phiStarting, nuStarting, sigmaSqTuning, tauSqTuning,
           phiTuning, nuTuning, nSamples, nThreads=1, verbose=0, nReport=100,
-          nRep=0, repIndx=None, rng=None, precision=0, seed=1):
+          nRep=0, repIndx=None, rng=None, precision=0, seed=1,
+          interrupt=None):
     """rNNGP (src/rNNGP.cpp:105-420), arguments in the .Call order of
     R/spNNGP.R:401-405.  rng: object with .struct(cls) giving the
     {state, norm_rand, unif_rand} callbacks, or None for the built-in
-    generator seeded by `seed`."""
+    generator seeded by `seed`.  interrupt: optional callable polled once
+    per iteration (R_CheckUserInterrupt, src/rNNGP.cpp:377)."""
     cm = cov_index(covModel)
     nTheta = 4 if cm == 2 else 3
     beta = np.zeros(p * nSamples)
@@ -216,7 +225,7 @@ def rNNGP(y, X, p, n, m, coords, covModel, nnIndx, nnIndxLU, sigmaSqIG,
     acc = C.c_int()
     rep = np.zeros(max(n * nRep, 1))
     ri = _i32(repIndx if repIndx is not None else np.zeros(max(nSamples, 1)))
-    opts = _opts(precision, 0, seed)
+    opts = _opts(precision, 0, seed, interrupt=interrupt)
     nu_u = _f64(nuUnif if nuUnif is not None else [0.0, 0.0])
     check(load().nngp_rNNGP(
         _f64(y), _cm(X), p, n, m, _cm(coords), cm, _i32(nnIndx),
@@ -288,9 +297,10 @@ def sNNGP(y, X, p, n, m, coords, covModel, nnIndx, nnIndxLU, uIndx, uIndxLU,
 
 def cNNGP(y, X, p, n, coords, thetaAlpha, nnIndx, nnIndxLU, X0, coords0, n0,
           nnIndx0, g, m, sigmaSqIG, covModel, nThreads=1, verbose=0,
-          precision=0):
+          precision=0, interrupt=None):
     """cNNGP (src/cNNGP.cpp:73-358), .Call order of R/spConjNNGP.R:336.
-    thetaAlpha: nTheta x g (rows phi, alpha[, nu]); nnIndx0: n0 x m."""
+    thetaAlpha: nTheta x g (rows phi, alpha[, nu]); nnIndx0: n0 x m.
+    interrupt: optional callable polled once per grid point."""
     cm = cov_index(covModel)
     betaHat = np.zeros(p * g)
     ab = np.zeros(2 * g)
@@ -298,7 +308,7 @@ def cNNGP(y, X, p, n, coords, thetaAlpha, nnIndx, nnIndxLU, X0, coords0, n0,
     bb = np.zeros(p * g)
     y0 = np.zeros(max(n0 * g, 1))
     y0v = np.zeros(max(n0 * g, 1))
-    opts = _opts(precision)
+    opts = _opts(precision, interrupt=interrupt)
     keep = []
 
     def vp(a, conv):

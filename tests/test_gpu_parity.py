@@ -444,6 +444,38 @@ def test_rNNGP_fit_rep_matches_oracle(oracle, nngp):
     assert rel_norm_err(got["y.rep.samples"], ref["y.rep.samples"]) < 1e-9
 
 
+def test_rNNGP_fit_rep_at_1e5_matches_oracle(oracle, nngp):
+    """fit.rep inside the chain at n = 1e5: the replicate draws run on the
+    device from the chain's B, F and the host stream's normals
+    (src/rNNGP.cpp:299-313); equal to the oracle's to 1e-10."""
+    print("rNNGP fit.rep n=1e5", flush=True)
+    n, m, p, S = 100_000, 15, 2, 4
+    c = uniform_coords(n, 17)
+    rng = np.random.default_rng(17)
+    X = np.column_stack([np.ones(n), rng.normal(size=n)])
+    y = X @ [1.0, 2.0] + np.sin(7 * c[:, 0]) * np.cos(5 * c[:, 1]) \
+        + 0.3 * rng.normal(size=n)
+    nn = oracle.mkNNIndxCB(c, m, nThreads=16)
+    normals = rng.normal(size=S * (n + 10) + 10)
+    unifs = rng.uniform(size=S + 5)
+    rep = np.array([0, 1, 0, 1], np.int32)
+    args = dict(sigmaSqIG=[2.0, 1.0], tauSqIG=[2.0, 0.1], phiUnif=[3.0, 30.0],
+                nuUnif=None, betaStarting=[1.0, 2.0], sigmaSqStarting=1.0,
+                tauSqStarting=0.1, phiStarting=8.0, nuStarting=0.0,
+                sigmaSqTuning=0.05, tauSqTuning=0.05, phiTuning=0.05,
+                nuTuning=0.0, nSamples=S, nRep=int(rep.sum()), repIndx=rep)
+    ref = oracle.rNNGP(y, X, p, n, m, c.T.reshape(-1).copy(), 0,
+                       nn["nnIndx"], nn["nnIndxLU"], nThreads=16,
+                       rng=oracle.TapeRng(normals, unifs), **args)
+    got = nngp.rNNGP(y, X, p, n, m, c, 0, nn["nnIndx"], nn["nnIndxLU"],
+                     rng=oracle.TapeRng(normals, unifs), **args)
+    assert got["accept"] == ref["accept"]
+    assert got["y.rep.samples"].shape == ref["y.rep.samples"].shape
+    err = rel_norm_err(got["y.rep.samples"], ref["y.rep.samples"])
+    print(f"  y.rep rel err {err:.1e}", flush=True)
+    assert err < 1e-10
+
+
 # ---------------------------------------------------------------------------
 # kriging and the conjugate model
 # ---------------------------------------------------------------------------
@@ -648,3 +680,51 @@ def test_mixed_chain_matches_fp64_chain(oracle, nngp):
         a, b = res[0][key], res[1][key]
         se = a.std(1) / np.sqrt(S)
         assert np.all(np.abs(a.mean(1) - b.mean(1)) < 0.01 * se + 1e-9)
+
+
+# ---------------------------------------------------------------------------
+# interrupts (R_CheckUserInterrupt, src/rNNGP.cpp:377, src/cNNGP.cpp:314)
+# ---------------------------------------------------------------------------
+def test_interrupt_stops_the_chain_cleanly(oracle, nngp):
+    n, m, p, S = 300, 8, 2, 50
+    c, X, y = make_response_data(n, p, 21)
+    nn = oracle.mkNNIndx(c, m)
+    calls = []
+
+    def stop_after_5():
+        calls.append(1)
+        return len(calls) >= 5
+
+    args = dict(sigmaSqIG=[2.0, 5.0], tauSqIG=[2.0, 1.0], phiUnif=[3.0, 30.0],
+                nuUnif=None, betaStarting=np.zeros(p), sigmaSqStarting=5.0,
+                tauSqStarting=1.0, phiStarting=6.0, nuStarting=0.0,
+                sigmaSqTuning=0.3, tauSqTuning=0.3, phiTuning=0.3,
+                nuTuning=0.0, nSamples=S, seed=4)
+    with pytest.raises(nngp.NngpError) as e:
+        nngp.rNNGP(y, X, p, n, m, c, 0, nn["nnIndx"], nn["nnIndxLU"],
+                   interrupt=stop_after_5, **args)
+    assert e.value.status == nngp.NNGP_ERR_INTERRUPT
+    assert "interrupted after 5 samples" in str(e.value)
+    assert len(calls) == 5
+    # the library is usable afterwards: the same chain runs to the end
+    out = nngp.rNNGP(y, X, p, n, m, c, 0, nn["nnIndx"], nn["nnIndxLU"],
+                     interrupt=lambda: False, **args)
+    assert out["p.theta.samples"].shape == (3, S)
+
+
+def test_interrupt_stops_the_grid_cleanly(oracle, nngp):
+    n, m, p, q = 400, 6, 2, 30
+    c, X, y = make_response_data(n, p, 22)
+    nn = oracle.mkNNIndx(c, m)
+    rng = np.random.default_rng(22)
+    c0 = rng.uniform(size=(q, 2))
+    X0 = np.column_stack([np.ones(q), rng.normal(size=q)])
+    nn0, _ = oracle.nn2(c, c0, m)
+    ta = np.array([[ph, al] for ph in (2.0, 5.0, 9.0) for al in (0.1, 0.5)]).T
+    seen = []
+    with pytest.raises(nngp.NngpError) as e:
+        nngp.cNNGP(y, X, p, n, c, ta, nn["nnIndx"], nn["nnIndxLU"], X0, c0,
+                   q, nn0, ta.shape[1], m, [2.0, 1.0], 0,
+                   interrupt=lambda: seen.append(1) or len(seen) >= 2)
+    assert e.value.status == nngp.NNGP_ERR_INTERRUPT
+    assert "interrupted after 2 of 6" in str(e.value)

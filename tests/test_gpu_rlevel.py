@@ -269,3 +269,79 @@ def test_spNNGP_chain_from_r_set_seed_stream(oracle, nngp):
                        nuTuning=0.0, nSamples=S, rng=nngp.RRng(2024))
     assert max_rel(fit["p.beta.samples"], ref["p.beta.samples"].T) < RTOL
     assert max_rel(fit["p.theta.samples"], ref["p.theta.samples"].T) < RTOL
+
+
+def r_get_n_indx(i, m):
+    """get.n.indx of R/util.R:257-270, 1-based as in R: the positions of
+    site i's neighbors in nn.indx (None for the first site)."""
+    i = i - 1
+    if i == 0:
+        return None
+    if i < m:
+        start = i / 2 * (i - 1)
+        return np.arange(start + 1, start + 1 + i).astype(int)
+    start = m / 2 * (m - 1) + (i - m) * m
+    return np.arange(start + 1, start + 1 + m).astype(int)
+
+
+@pytest.mark.gpu
+def test_neighbor_info_and_sample_layout_match_the_reference(oracle, nngp,
+                                                             tmp_path):
+    """The fit's list layout of R/spNNGP.R:439-456: p.theta.samples and
+    p.beta.samples transposed to n.samples rows (mcmc(t(...))) with the
+    reference's column names; neighbor.info with n.neighbors, the
+    mk.n.indx.list of R/util.R:272-279 (NA, then 1-based neighbor sets
+    through get.n.indx), nn.indx / nn.indx.lu of mkNNIndxCB on the x-ordered
+    sites, and ord = order(coords[, 1]); coords, y and X back in the
+    caller's order.  save_fit / load_fit keep every element."""
+    print("neighbor.info layout", flush=True)
+    n, m, S = 500, 8, 30
+    c, X, y = gp_data(n, 31)
+    c[::50, 0] = c[1::50, 0]  # tied x values: order() is stable
+    rng = np.random.default_rng(31)
+    zs, us = rng.normal(size=S * 8), rng.uniform(size=S)
+    fit = nngp.spNNGP(y, X, c, starting={"sigma.sq": 5, "tau.sq": 1,
+                                         "phi": 6},
+                      tuning={"sigma.sq": 0.5, "tau.sq": 0.5, "phi": 0.5},
+                      priors={"sigma.sq.IG": [2, 5], "tau.sq.IG": [2, 1],
+                              "phi.Unif": [3, 300]},
+                      n_samples=S, n_neighbors=m,
+                      rng=oracle.TapeRng(zs, us), return_neighbor_info=True)
+    ord_r = np.argsort(c[:, 0], kind="stable")  # order(coords[,1])
+    co, Xo, yo = c[ord_r], X[ord_r], y[ord_r]
+    nn = oracle.mkNNIndxCB(co, m)
+    ni = fit["neighbor.info"]
+    assert ni["n.neighbors"] == m
+    assert np.array_equal(ni["ord"], ord_r)
+    assert np.array_equal(ni["nn.indx"], nn["nnIndx"])
+    assert np.array_equal(ni["nn.indx.lu"], nn["nnIndxLU"])
+    assert len(ni["n.indx"]) == n and ni["n.indx"][0] is None
+    for i in range(2, n + 1):
+        pos = r_get_n_indx(i, m)
+        assert np.array_equal(ni["n.indx"][i - 1], nn["nnIndx"][pos - 1] + 1)
+    ref = oracle.rNNGP(yo, Xo, 2, n, m, co.T.reshape(-1).copy(), 0,
+                       nn["nnIndx"], nn["nnIndxLU"], sigmaSqIG=[2, 5],
+                       tauSqIG=[2, 1], phiUnif=[3, 300], nuUnif=None,
+                       betaStarting=np.linalg.lstsq(Xo, yo, rcond=None)[0],
+                       sigmaSqStarting=5.0, tauSqStarting=1.0,
+                       phiStarting=6.0, nuStarting=0.0, sigmaSqTuning=0.5,
+                       tauSqTuning=0.5, phiTuning=0.5, nuTuning=0.0,
+                       nSamples=S, rng=oracle.TapeRng(zs, us))
+    assert fit["p.theta.samples"].shape == (S, 3)
+    assert fit["p.beta.samples"].shape == (S, 2)
+    assert fit["theta.names"] == ["sigma.sq", "tau.sq", "phi"]
+    assert max_rel(fit["p.theta.samples"], ref["p.theta.samples"].T) < RTOL
+    assert max_rel(fit["p.beta.samples"], ref["p.beta.samples"].T) < RTOL
+    assert np.array_equal(fit["coords"], c) and np.array_equal(fit["y"], y)
+    assert np.array_equal(fit["X"], X)
+    path = tmp_path / "fit.npz"
+    nngp.save_fit(fit, path)
+    back = nngp.load_fit(path)
+    for k in ("p.theta.samples", "p.beta.samples", "coords", "y", "X"):
+        assert np.array_equal(back[k], fit[k]), k
+    bni = back["neighbor.info"]
+    for k in ("nn.indx", "nn.indx.lu", "ord"):
+        assert np.array_equal(bni[k], ni[k]), k
+    assert bni["n.indx"][0] is None
+    assert all(np.array_equal(a, b) for a, b in
+               zip(bni["n.indx"][1:], ni["n.indx"][1:]))

@@ -158,3 +158,39 @@ def test_server_wait_timeout_and_unknown_ticket(nngp):
         srv.release(t)
         with pytest.raises(nngp.NngpError):
             srv.release(t)
+
+
+@pytest.mark.gpu
+def test_server_done_queue_stays_bounded(nngp):
+    """Requests waited for by ticket (or served by predict) leave the queue
+    of finished requests: after many of them it is empty, and a wait for
+    any request then times out instead of returning a stale ticket."""
+    print("server done queue", flush=True)
+    c, X, y, beta, th = fitted(0, 800, 2, 8, seed=12)
+    with nngp.Server(X, y, c, 6, beta, th, 0, device_bytes=16 << 20,
+                     host_bytes=16 << 20) as srv:
+        tickets = []
+        for k in range(200):
+            X0, c0 = request(1 + k % 7, 2, seed=100 + k)
+            tickets.append(srv.submit(X0, c0, k))
+        for t in tickets:
+            srv.wait(t, timeout=30.0, copy=False)
+            srv.release(t)
+        for k in range(50):
+            X0, c0 = request(3, 2, seed=400 + k)
+            srv.predict(X0, c0, k)
+        # finished, released before any wait: also leaves the queue
+        X0, c0 = request(2, 2, seed=999)
+        t = srv.submit(X0, c0, 9)
+        st = srv.stats()
+        while st["completed"] < 251:
+            st = srv.stats()
+        srv.release(t)
+        st = srv.stats()
+        assert st["done_unreported"] == 0
+        assert st["completed"] == 251
+        assert st["kernel_ms"] > 0.0
+        assert st["kernel_site_samples"] == st["site_samples"]
+        with pytest.raises(nngp.NngpError) as e:
+            srv.wait(-1, timeout=0.05)
+        assert e.value.status == 8  # NNGP_ERR_TIMEOUT
