@@ -28,12 +28,50 @@ sys.path.insert(0, ROOT)
 METRIC = "MCMC iter/s @ n=1e6,m=15 (FP64 vs mixed) + kriging p50/p99 latency"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per launch of the FP64 updateBF pass at n=1e6, m=15 (the
-# per-lane kernel with covariate rows staged in LDS, 64-thread blocks): two
-# rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this script, each scaled
-# by what the counter reads for a copy kernel of known size (committed
-# summary: profiles/updateBF_rNNGP_n1e6_m15_pmc_traffic.md)
-BF_TRAFFIC_BYTES = 119_137_416
+# HBM bytes per launch of the dominant kernels, from two rocprofv3 --pmc
+# passes (FETCH_SIZE, WRITE_SIZE) of this script, each scaled by what the
+# counter reads for a copy kernel of known size: the committed summary's
+# table rows "| key | kernel | workload | bytes |".
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_per_launch.md")
+
+
+def traffic(key, kernel):
+    """(bytes, note) of the summary row `key` when it measured `kernel`
+    (the launch this run used), else (None, why)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                cells = [x.strip() for x in line.strip().strip("|").split("|")]
+                if len(cells) >= 4 and cells[0] == key:
+                    if cells[1].strip("`") != kernel:
+                        return None, (f"summary row {key} measured "
+                                      f"{cells[1]}, this run used {kernel}")
+                    return int(cells[3].replace(",", "")), \
+                        f"profiles/hbm_traffic_per_launch.md row {key}"
+    except OSError:
+        return None, "no traffic summary"
+    return None, f"no summary row {key}"
+
+
+def f_krig(m, p):
+    """Algorithmic flops of one (new site, posterior sample) of kriging: the
+    m x m covariance from the site's shared distances (2 flops and one exp
+    per pair), Cholesky m^3/3, two solves 2m^2, w.c and the mean and
+    variance terms 2pm + 2p + 4m + 3, and the one sqrt."""
+    pairs = m * (m + 1) / 2
+    return m ** 3 / 3 + 2 * m * m + 2 * pairs + 2 * p * m + 2 * p + 4 * m \
+        + 3 + pairs + 1
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def f_alg(m):
@@ -138,7 +176,7 @@ def cpu_baseline(c, X, y, idx, m, iters, start):
     O.rNNGP(y, X, p, n, m, c.T.reshape(-1).copy(), 0, idx["nnIndx"],
             idx["nnIndxLU"], rng=tape, nThreads=nt, nSamples=iters, **start)
     dt = time.perf_counter() - t0
-    return {"value": iters / dt, "unit": "iter/s", "cores": nt,
+    return {"value": iters / dt, "unit": "iter/s", "cores": nt, "cpu_model": cpu_model(),
             "kind": "port",
             "sample": f"{iters} MH iterations of the same n={n}, m={m} "
                       f"chain (oracle rNNGP, OpenMP, {nt} threads), "
@@ -195,7 +233,7 @@ def replicates_bench(c, X, idx, m, S, seed, cpu_samples):
                           cpu_samples, O.TapeRng(z, []), nThreads=nt)
         cdt = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": cpu_samples / cdt,
-                               "unit": "replicates/s", "cores": nt,
+                               "unit": "replicates/s", "cores": nt, "cpu_model": cpu_model(),
                                "kind": "port",
                                "sample": f"{cpu_samples} replicate(s) of the "
                                          f"same n={n}, m={m} model (oracle "
@@ -226,7 +264,7 @@ def knn_bench(c, idx, m, gpu_s, cpu):
             and np.array_equal(ref["nnIndxLU"], idx["nnIndxLU"])
             and np.array_equal(ref["nnDist"], idx["nnDist"]))
         out["cpu_baseline"] = {
-            "value": cdt, "unit": "s", "cores": nt, "kind": "port",
+            "value": cdt, "unit": "s", "cores": nt, "cpu_model": cpu_model(), "kind": "port",
             "sample": f"the whole n={c.shape[0]}, m={m} search (oracle "
                       f"mkNNIndxCB, OpenMP)"}
     return out
@@ -283,7 +321,7 @@ def latent_bench(c, X, y, idx, m, iters, site_iters, cpu_iters, seed):
         cdt = time.perf_counter() - t0
         rr.close()
         out["cpu_baseline"] = {
-            "value": cpu_iters / cdt, "unit": "iter/s", "cores": nt,
+            "value": cpu_iters / cdt, "unit": "iter/s", "cores": nt, "cpu_model": cpu_model(),
             "kind": "port", "mkUIndx_s": cu,
             "sample": f"{cpu_iters} iteration(s) of the same n={n}, m={m} "
                       f"latent chain (oracle sNNGP: OpenMP updateBF, serial "
@@ -370,6 +408,19 @@ def kriging_serving(c, X, y, m, seconds, rate, S, seed, cpu_sites=0):
         ok += info["status"] == 0
         lat[k] = (info["t_done"] - (base + arrivals[k])) * 1e3
     mid = (sizes >= 5_000) & (sizes <= 20_000)
+    # the kriging kernel's own rate: device time of the retired launches
+    # (HIP events around each launch on the server's kernel stream)
+    kflops = st["kernel_site_samples"] * f_krig(m, p)
+    kachieved = kflops / (st["kernel_ms"] * 1e-3) / 1e12 \
+        if st["kernel_ms"] > 0 else 0.0
+    roof = roofline_obj(
+        "kriging (per-lane kernel, m=15, exponential, batched server "
+        "launches of <= 16384 sites x %d samples)" % S, kachieved,
+        *traffic("kriging_server_launch_n1e6_m15_S500",
+                 "nngp::k_pred_lane<15, 0>"),
+        flops_per_site_sample=f_krig(m, p),
+        kernel_ms_total=st["kernel_ms"], launches=int(st["launches"]),
+        mean_launch_ms=st["kernel_ms"] / max(st["launches"], 1))
     cpu = None
     if cpu_sites > 0:
         # the oracle's rNNGPPredict (src/rNNGPPredict.cpp:25-222, OpenMP over
@@ -387,7 +438,7 @@ def kriging_serving(c, X, y, m, seconds, rate, S, seed, cpu_sites=0):
                        0, z, nThreads=nt)
         cdt = time.perf_counter() - t0
         cpu = {"value": cpu_sites * S / cdt, "unit": "site_samples/s",
-               "cores": nt, "kind": "port",
+               "cores": nt, "cpu_model": cpu_model(), "kind": "port",
                "q1e4_request_ms": cdt * 1e3 * 10_000 / cpu_sites,
                "sample": f"{cpu_sites} new sites x {S} samples of the same "
                          f"model (oracle rNNGPPredict, OpenMP), {cdt:.2f} s; "
@@ -406,6 +457,7 @@ def kriging_serving(c, X, y, m, seconds, rate, S, seed, cpu_sites=0):
             "alone_q1e4_p50_ms": float(np.percentile(alone, 50)),
             "alone_q1e4_p99_ms": float(np.percentile(alone, 99)),
             "setup_s": setup_s,
+            "roofline": roof,
             "cpu_baseline": cpu,
             "note": "resident model (n=1e6, m=15, exponential), nn2 + "
                     "kriging on the GPU, results delivered to pinned host "
@@ -493,7 +545,7 @@ def conj_grid_bench(n, m, k, seed, cpu_points):
                 np.abs(det["ab"][0, :cpu_points] - abo) / np.abs(abo)))}
         res["cpu_baseline"] = {
             "value": cpu_points * nmod / dt, "unit": "site-updates/s",
-            "cores": nt, "kind": "port",
+            "cores": nt, "cpu_model": cpu_model(), "kind": "port",
             "sample": f"oracle cNNGP, fold 1 x first {cpu_points} grid "
                       f"points ({nmod} training + {nho} held-out sites), "
                       f"{dt:.1f} s; the full grid would take "
@@ -580,6 +632,15 @@ def stress_bench(n, m, iters, seed):
         legs[name] = leg
     del ref
     fp, mx = legs["fp64"], legs["mixed"]
+    kname = "nngp::k_bf_coop<false, 32, false> (%d threads)" \
+        % fp["bf_tune"]["chosen"][1]
+    fp["roofline"] = roofline_obj(
+        "updateBF (wavefront kernel, m=32, Bessel-K matern, %s %d threads)"
+        % tuple(fp["bf_tune"]["chosen"]), fp["flops_per_s"] / 1e12,
+        *(traffic("updateBF_stress_n4e6_m32", kname)
+          if (n, m) == (4_000_000, 32) else (None, "other workload")),
+        algorithmic_bytes=(24 + 4 * m) * n,
+        flops_per_launch=n * f_alg(m), launch_ms=fp["updateBF_ms"])
     return {"n": n, "m": m, "iters": iters, "knn_s": knn_s,
             # FP64 leg at the top level, the mixed leg beside it
             **fp, "mixed": mx,
@@ -587,6 +648,29 @@ def stress_bench(n, m, iters, seed):
             "precision": "fp64 and mixed (FP32 build incl. Bessel K, FP32 "
                          "factor and solves, one FP64 refinement), each "
                          "autotuned"}
+
+
+def roofline_obj(kernel, achieved_tflops, traffic_bytes, traffic_note,
+                 **extra):
+    """The roofline record of an FP64 compute-bound kernel: the kernels run
+    on the FP64 vector ALUs (no matrix cores), whose peak is 78.6 TFLOP/s."""
+    return {"bound": "fp64_valu",
+            "bound_note": "FP64 vector FMA issue (no MFMA); peak 78.6 TFLOP/s",
+            "kernel": kernel, "achieved": achieved_tflops,
+            "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved_tflops / PEAK_FP64_TFLOPS,
+            "traffic": traffic_bytes, "traffic_note": traffic_note, **extra}
+
+
+def chain_kernel(chosen, m):
+    """Name of the chain's FP64 updateBF kernel for an autotuned shape."""
+    fam, threads = chosen
+    M = 15 if m == 15 else m
+    if fam == "lane+lds-z":
+        return f"nngp::k_bf_lane_lds<{M}, 0, 2, false> ({threads} threads)"
+    if fam == "lane":
+        return f"nngp::k_bf_lane<{M}, 0, false> ({threads} threads)"
+    return f"{fam} ({threads} threads)"
 
 
 def main():
@@ -598,7 +682,7 @@ def main():
     ap.add_argument("--m", type=int, default=15)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-iters", type=int, default=40)
-    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--grid-n", type=int, default=500_000)
     ap.add_argument("--grid-cpu-points", type=int, default=4)
@@ -695,24 +779,14 @@ def main():
                    "n": n, "m": m, "p": p, "cov_model": "exponential",
                    "precision": "fp64", "tuning": 0.02},
         "acceptance": out["accept"] / args.steps,
-        "roofline": {"bound": "mfma",
-                     "bound_note": "FP64 compute: the kernel runs on the "
-                                   "vector FMA units, whose FP64 peak equals "
-                                   "the FP64 matrix peak (78.6 TFLOP/s)",
-                     "kernel": "updateBF (per-lane kernel, m=15, "
-                               "exponential, %s %d threads)"
-                               % tuple(f64["tune"]["chosen"]),
-                     "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": BF_TRAFFIC_BYTES if (n, m) == (1_000_000, 15)
-                                else None,
-                     "traffic_note": "HBM bytes per launch from rocprofv3 "
-                                     "FETCH_SIZE + WRITE_SIZE, calibrated "
-                                     "on a copy kernel; algorithmic bytes "
-                                     "84 per site (84 MB per launch)",
-                     "flops_per_launch": flops,
-                     "launch_ms": bf_ms,
-                     "launches": tm["bf_launches"]},
+        "roofline": roofline_obj(
+            "updateBF (per-lane kernel, m=15, exponential, %s %d threads)"
+            % tuple(f64["tune"]["chosen"]), achieved,
+            *(traffic("updateBF_chain_n1e6_m15",
+                      chain_kernel(f64["tune"]["chosen"], m))
+              if (n, m) == (1_000_000, 15) else (None, "other workload")),
+            algorithmic_bytes=(24 + 4 * m) * n, flops_per_launch=flops,
+            launch_ms=bf_ms, launches=tm["bf_launches"]),
         "breakdown_ms_per_step": {
             "updateBF": tm["bf_ms"] / args.steps,
             "sums": tm["sum_ms"] / args.steps,
