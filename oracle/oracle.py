@@ -15,7 +15,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# NNGP_ORACLE_LIB names the sanitized build (`make -C oracle sanitize`)
+_LIB_PATH = os.environ.get("NNGP_ORACLE_LIB") or os.path.join(
+    _HERE, "liboracle.so")
 _lib = None
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")

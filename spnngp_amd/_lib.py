@@ -10,8 +10,11 @@ import os
 
 import numpy as np
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
-                        "libnngp_hip.so")
+# NNGP_LIB names another build of the same library (the host-sanitized
+# copy of `python -m spnngp_amd.build --sanitize`); by default the in-tree
+# libnngp_hip.so
+LIB_PATH = os.environ.get("NNGP_LIB") or os.path.join(
+    os.path.dirname(os.path.abspath(__file__)), "libnngp_hip.so")
 
 NNGP_OK = 0
 NNGP_ERR_ARG = 1

@@ -1,9 +1,16 @@
 """Build libnngp_hip.so (gfx950) in-tree with hipcc.
 
-    python -m spnngp_amd.build
+    python -m spnngp_amd.build              # spnngp_amd/libnngp_hip.so
+    python -m spnngp_amd.build --sanitize   # spnngp_amd/_san/libnngp_hip.so
 
 Each .hip file of spnngp_amd/csrc is compiled to an object in parallel,
 then linked into spnngp_amd/libnngp_hip.so.  No GPU is needed to build.
+
+--sanitize builds a second copy whose host code (only: the device code is
+compiled as usual) carries AddressSanitizer and UndefinedBehaviorSanitizer,
+for running the CPU tests against it (tests/run_sanitized.sh).  It needs
+the clang ASan runtime preloaded, and it is never the library the package
+loads by default.
 """
 from __future__ import annotations
 
@@ -18,6 +25,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libnngp_hip.so")
+SAN_OBJ = os.path.join(HERE, "_obj_san")
+SAN_LIB = os.path.join(HERE, "_san", "libnngp_hip.so")
 ARCH = os.environ.get("NNGP_OFFLOAD_ARCH", "gfx950")
 
 # Packed-FP32 instructions are disabled (they have given wrong results on
@@ -27,6 +36,13 @@ HIPFLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(ROOT, "include"), "-Wall", "-Wno-unused-function",
 ]
+
+# host-side sanitizers; each flag behind -Xarch_host so the device code is
+# built exactly as in the plain library
+SANFLAGS = [f for x in ("-fsanitize=address", "-fsanitize=undefined",
+                        "-fno-sanitize-recover=undefined",
+                        "-fno-omit-frame-pointer", "-g")
+            for f in ("-Xarch_host", x)]
 
 
 def _hipcc() -> str:
@@ -54,12 +70,16 @@ def _stale(src, obj):
 
 
 def build(verbose: bool = True, jobs: int | None = None,
-          incremental: bool = False) -> str:
+          incremental: bool = False, sanitize: bool = False) -> str:
     """Compile every source (incremental: only stale objects) and link."""
     hipcc = _hipcc()
-    os.makedirs(OBJ, exist_ok=True)
+    obj_dir, lib = (SAN_OBJ, SAN_LIB) if sanitize else (OBJ, LIB)
+    extra = SANFLAGS if sanitize else []
+    os.makedirs(obj_dir, exist_ok=True)
+    os.makedirs(os.path.dirname(lib), exist_ok=True)
     srcs = sources()
-    objs = [os.path.join(OBJ, os.path.basename(s)[:-4] + ".o") for s in srcs]
+    objs = [os.path.join(obj_dir, os.path.basename(s)[:-4] + ".o")
+            for s in srcs]
     only = [a[len("--only="):] for a in sys.argv[1:]
             if a.startswith("--only=")] if __name__ == "__main__" else []
     todo = [(s, o) for s, o in zip(srcs, objs)
@@ -68,7 +88,7 @@ def build(verbose: bool = True, jobs: int | None = None,
 
     def compile_one(pair):
         src, obj = pair
-        cmd = [hipcc, *HIPFLAGS, "-c", src, "-o", obj]
+        cmd = [hipcc, *HIPFLAGS, *extra, "-c", src, "-o", obj]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")
@@ -80,19 +100,22 @@ def build(verbose: bool = True, jobs: int | None = None,
             if verbose:
                 print(f"[nngp build] compiled {os.path.basename(src)}",
                       flush=True)
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
+    link_san = (["-Xarch_host", "-fsanitize=address", "-Xarch_host",
+                 "-fsanitize=undefined", "-Xarch_host", "-shared-libasan"]
+                if sanitize else [])
     cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread",
-           "-o", tmp,
-           *objs]
+           *link_san, "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     if verbose:
-        print(f"[nngp build] wrote {LIB}", flush=True)
-    return LIB
+        print(f"[nngp build] wrote {lib}", flush=True)
+    return lib
 
 
 if __name__ == "__main__":
-    build(incremental="--incremental" in sys.argv[1:])
+    build(incremental="--incremental" in sys.argv[1:],
+          sanitize="--sanitize" in sys.argv[1:])
     sys.exit(0)

@@ -853,8 +853,8 @@ __device__ inline bool coop_site_mixed(const BfArgs &A, const BfProblem &P,
   return true;
 }
 
-// Logical block b covers sites [b*chunk, (b+1)*chunk); its waves take every
-// nw-th site of that run.  A.coop_blocks logical blocks per problem.
+// The waves of the logical blocks of one XCD take that XCD's run of sites
+// round robin (xcd_site_run).  A.coop_blocks logical blocks per problem.
 // MIXED: the sites are solved by coop_site_mixed (single problem only).
 // The block's matern tables: FP64 passes the Bessel-K loop divisors and the
 // pass's correlation table, mixed passes the FP32 divisors.
@@ -896,15 +896,13 @@ __global__ __launch_bounds__(64 * kCoopMaxWaves) void k_bf_coop(BfArgs A) {
   float *Wf = reinterpret_cast<float *>(smem) +
               (size_t)w * coop_mixed_wave_floats(A.m);
   const CovArgs32 c32 = cov32(c);
-  const int chunk = (A.n + A.coop_blocks - 1) / A.coop_blocks;
-  const int i0 = b * chunk;
-  const int i1 = min(A.n, i0 + chunk);
+  const SiteRun run = xcd_site_run(A.coop_blocks, b, A.n, nw, w);
   // the wave's running sums live in LDS (lane 0 updates them), which keeps
   // this kernel's registers low enough for several waves per SIMD
   const int nv = 2 + A.nz * (A.nz + 1) / 2;
   for (int k = t; k < nv; k += 64) red[w][k] = 0.0;
   wave_sync();
-  for (int i = i0 + w; i < i1; i += nw) {
+  for (int i = run.first; i < run.end; i += run.stride) {
     double F, e[kMaxNZ], bcoef;
     int cnt, off;
     bool ok;

@@ -291,8 +291,8 @@ __global__ __launch_bounds__(kCvPredThreads) void k_cv_pred_lane(
 }
 
 // One held-out site per wavefront (any m <= MM <= 64, any covariance);
-// logical block b walks a contiguous run of sites, its waves every nw-th
-// one.
+// the waves of one XCD's logical blocks take its run of sites round robin
+// (xcd_site_run).
 template <int MM>
 __global__ __launch_bounds__(64 * kCvCoopWaves) void k_cv_pred_coop(
     CvPredArgs A) {
@@ -313,10 +313,9 @@ __global__ __launch_bounds__(64 * kCvCoopWaves) void k_cv_pred_coop(
   const double *Z = A.Z + (size_t)f * A.nmod * A.zs;
   const int *nn0 = A.nn0 + (size_t)f * A.nho * m;
   const double *X0 = A.X0 + (size_t)f * A.nho * p;
-  const int chunk = (A.nho + A.coop_blocks - 1) / A.coop_blocks;
-  const int i0 = b * chunk, i1 = min(A.nho, i0 + chunk);
+  const SiteRun run = xcd_site_run(A.coop_blocks, b, A.nho, nw, w);
   double crps_acc = 0.0, sq_acc = 0.0;
-  for (int i = i0 + w; i < i1; i += nw) {
+  for (int i = run.first; i < run.end; i += run.stride) {
     int id = 0;
     if (t < m) {
       id = nn0[(size_t)t * A.nho + i];
@@ -417,8 +416,8 @@ static int cv_pred_launch(CvPredArgs a, int variant, int nprob,
     int waves = kCvCoopWaves;
     const size_t per = (size_t)(a.m * (a.m + 1) + 2 * a.m) * sizeof(double);
     while (waves > 1 && waves * per > 65536) waves--;
-    // chunking assumed kCvCoopWaves waves per logical block; fewer waves
-    // per block only lengthen each wave's walk over the same run
+    // fewer waves per block only lengthen each wave's walk over its XCD's
+    // run of sites
     const dim3 blk(64 * waves);
     switch (wave_rows_for(a.m)) {
       case 16:

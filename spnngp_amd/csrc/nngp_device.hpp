@@ -864,4 +864,29 @@ __device__ __forceinline__ int xcd_block(int nlogical) {
   return b < nlogical ? b : -1;
 }
 
+// The sites of wave w of logical block b (from xcd_block) in a launch whose
+// nlogical blocks of nw waves each stay resident while they loop over n
+// sites.  XCD x owns the contiguous run of its logical blocks' shares, and
+// the waves of those blocks take the run's sites round robin: at any moment
+// they work on neighboring sites, whose neighbor gathers then share the
+// XCD's L2.  (A contiguous share per block would spread the XCD's waves
+// over its whole run.)  The wave takes first, first + stride, ... < end.
+struct SiteRun {
+  int first, stride, end;
+};
+__device__ __forceinline__ SiteRun xcd_site_run(int nlogical, int b, int n,
+                                                int nw, int w) {
+  const int per = (nlogical + 7) >> 3;
+  const int x = b / per, j = b - x * per;
+  const int nb = min(per, nlogical - x * per);  // logical blocks on XCD x
+  const long long chunk = ((long long)n + nlogical - 1) / nlogical;
+  const long long s0 = (long long)x * per * chunk;
+  const long long s1 = min((long long)n, s0 + (long long)nb * chunk);
+  SiteRun r;
+  r.first = (int)min(s0 + (long long)j * nw + w, s1);
+  r.stride = nb * nw;
+  r.end = (int)s1;
+  return r;
+}
+
 }  // namespace nngp
