@@ -632,11 +632,13 @@ def stress_bench(n, m, iters, seed):
         legs[name] = leg
     del ref
     fp, mx = legs["fp64"], legs["mixed"]
-    kname = "nngp::k_bf_coop<false, 32, false> (%d threads)" \
-        % fp["bf_tune"]["chosen"][1]
+    fam, th = fp["bf_tune"]["chosen"]
+    kname = (f"nngp::k_bf_grp<32, true> ({th} threads)" if fam == "row" else
+             f"nngp::k_bf_coop<false, 32, false> ({th} threads)")
     fp["roofline"] = roofline_obj(
-        "updateBF (wavefront kernel, m=32, Bessel-K matern, %s %d threads)"
-        % tuple(fp["bf_tune"]["chosen"]), fp["flops_per_s"] / 1e12,
+        "updateBF (%s kernel, m=32, Bessel-K matern, %d threads)"
+        % ("16-lane-row" if fam == "row" else "wavefront", th),
+        fp["flops_per_s"] / 1e12,
         *(traffic("updateBF_stress_n4e6_m32", kname)
           if (n, m) == (4_000_000, 32) else (None, "other workload")),
         algorithmic_bytes=(24 + 4 * m) * n,

@@ -394,7 +394,9 @@ int nngp_model_timing(nngp_model *model, double *t);
 #define NNGP_TUNE_MAX 12
 typedef struct nngp_tune_report {
   int n;                          /* candidates timed                     */
-  int family[NNGP_TUNE_MAX];      /* 0: one site per lane, 1: per wavefront */
+  int family[NNGP_TUNE_MAX];      /* 0: one site per lane, 1: per wavefront,
+                                     2: per 16-lane row, four sites a
+                                     wavefront (m <= 32, FP64)            */
   int threads[NNGP_TUNE_MAX];     /* threads per block                    */
   int mode[NNGP_TUNE_MAX];        /* family 0, NNGP_MIXED: 1 = the
                                      covariance copy kept in LDS for the
@@ -408,7 +410,7 @@ typedef struct nngp_tune_report {
 } nngp_tune_report;
 
 /* Time every launch shape of this model's updateBF pass at theta (sites per
- * wavefront 64 or 1; 64/128/256-thread blocks or 1/2/4-system LDS tiles)
+ * wavefront 64, 4 or 1; 64/128/256-thread blocks or 1/2/4-system LDS tiles)
  * and use the fastest for every later pass.  Opt-in: the shape fixes the
  * order of the per-block sums, so a chain is bit-reproducible for a fixed
  * shape; without this call the shape is fixed (256-thread blocks where the

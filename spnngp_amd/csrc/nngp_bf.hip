@@ -42,6 +42,7 @@
 #include <vector>
 
 #include "nngp_kernels.hpp"
+#include "nngp_group.hpp"
 #include "nngp_wave.hpp"
 
 namespace nngp {
@@ -50,6 +51,7 @@ constexpr int kLaneMaxThreads = 256;
 constexpr int kCoopMaxWaves = 4;
 constexpr int kCoopMaxM = 64;
 constexpr int kCoopBlocks = 2048;  // logical blocks of the wavefront kernel
+constexpr int kGrpMaxM = 32;       // the row kernel's largest system
 constexpr int kRedThreads = 256;   // threads per block of the sum kernels
 constexpr int kRowsThreads = 1024; // threads per column of k_rows_sum
 constexpr int kRedBlocks = 1024;   // grid of the sum kernels (at most)
@@ -943,6 +945,205 @@ __global__ __launch_bounds__(64 * kCoopMaxWaves) void k_bf_coop(BfArgs A) {
 }
 
 // ---------------------------------------------------------------------------
+// one site per 16-lane row (four sites per wavefront), m <= 32
+// ---------------------------------------------------------------------------
+constexpr int kGrpSlots = 4 * kCoopMaxWaves;  // sites a block works on at once
+
+// Per-wave LDS of the row kernel, in doubles: each of its four rows'
+// neighbor coordinates, x then y.
+__host__ __device__ inline int grp_wave_doubles(int mm) { return 4 * 2 * mm; }
+
+// Covariance entries of the row kernel.  A lane evaluates about fifty per
+// site in fully unrolled code, so only the common path is inline: for the
+// matern (MAT) the pass's table, with near-coincident sites and x >= 64
+// (outside the table) out of line; otherwise the exponential, spherical and
+// gaussian.
+__device__ __noinline__ double grp_matern_outside(double x, const CovArgs &a,
+                                                  const BesselTab &bt) {
+  if (x > 700.0) return 0.0;
+  return a.mnorm * xnu_bessel_k(x, a, bt);
+}
+
+template <bool MAT>
+__device__ __forceinline__ double grp_corr(double d2, const CovArgs &a,
+                                           const BesselTab &bt,
+                                           const MaternTab &T) {
+  if constexpr (MAT) {
+    const double x = nn_sqrt(d2) * a.phi;
+    if (!(x > 0.0)) return a.s2;
+    const int o = matern_tab_octave(x);
+    if ((unsigned)o >= (unsigned)kMatOct) return grp_matern_outside(x, a, bt);
+    return matern_tab_at(x, o, T);
+  } else {
+    switch (a.cov) {
+      case 0: return s2_corr<0>(d2, a);
+      case 1: return s2_corr<1>(d2, a);
+      default: return s2_corr<3>(d2, a);
+    }
+  }
+}
+
+// Site i on the 16-lane row of lane u (valid: i < n).  The lane builds rows
+// u and u + 16 of the site's system in registers from the neighbor
+// coordinates the row stages in px/py (rows past the site's count are
+// identity rows with a zero right-hand side, so B and F are those of the
+// cnt x cnt system), then factor, forward and back substitution
+// (nngp_group.hpp).  Every lane of the row returns F and the residuals e;
+// false when a pivot or F is not positive.
+template <int MM, bool MAT>
+__device__ inline bool grp_site(const BfArgs &A, const BfProblem &P,
+                                const CovArgs &c, const BesselTab &bt,
+                                const MaternTab &mt, double *px, double *py,
+                                int i, bool valid, int u, double &F,
+                                double (&e)[kMaxNZ]) {
+  constexpr int R = GrpRows<MM>::R;
+  // opaque per site: the compares of u with every row number below would
+  // otherwise be hoisted out of the caller's site loop as lane masks, more
+  // than the SGPR file holds
+  u = lane_opaque(u);
+  int cnt = 0, off = 0;
+  if (valid) {
+    cnt = P.lu[A.n + i];
+    off = P.lu[i];
+  }
+  const bool ina = u < cnt, inb = R == 2 && u + 16 < cnt;
+  int ida = 0, idb = 0;
+  double2 qa = make_double2(0.0, 0.0), qb = qa;
+  if (ina) {
+    ida = P.nnIndx[off + u];
+    qa = P.xy[ida];
+  }
+  if (inb) {
+    idb = P.nnIndx[off + u + 16];
+    qb = P.xy[idb];
+  }
+  const double2 s = valid ? P.xy[i] : make_double2(0.0, 0.0);
+  px[u] = qa.x;
+  py[u] = qa.y;
+  if constexpr (R == 2) {
+    px[u + 16] = qb.x;
+    py[u + 16] = qb.y;
+  }
+  wave_sync();
+  GrpRows<MM> S;
+  // row u: entries l < u, the diagonal, zeros above it
+  static_for<0, 16>([&](auto lc) {
+    constexpr int l = decltype(lc)::value;
+    S.ra[l] = l == u ? (ina ? c.diag : 1.0) : 0.0;
+    if (l < u && ina)
+      S.ra[l] = grp_corr<MAT>(sqdist(qa.x, qa.y, px[l], py[l]), c, bt, mt);
+  });
+  if constexpr (R == 2) {  // row u + 16
+    static_for<0, 32>([&](auto lc) {
+      constexpr int l = decltype(lc)::value;
+      S.rb[l] = l == u + 16 ? (inb ? c.diag : 1.0) : 0.0;
+      if (l < u + 16 && inb)
+        S.rb[l] = grp_corr<MAT>(sqdist(qb.x, qb.y, px[l], py[l]), c, bt, mt);
+    });
+  }
+  const double ca =
+      ina ? grp_corr<MAT>(sqdist(s.x, s.y, qa.x, qa.y), c, bt, mt) : 0.0;
+  const double cb =
+      inb ? grp_corr<MAT>(sqdist(s.x, s.y, qb.x, qb.y), c, bt, mt) : 0.0;
+  wave_sync();  // the row's coordinates are read; the next site rewrites them
+  bool ok = grp_factor<MM>(S, u);
+  double ya, yb, ba, bb;
+  grp_forward<MM>(S, u, ca, cb, ya, yb);
+  F = c.diag - row_sum16(ya * ya + yb * yb);
+  grp_backward<MM>(S, u, ya, yb, ba, bb);
+  ok = ok && F > 0.0;
+  if (valid && A.B) {
+    if (ina) A.B[off + u] = ba;
+    if (inb) A.B[off + u + 16] = bb;
+  }
+#pragma unroll
+  for (int j = 0; j < kMaxNZ; j++) {
+    e[j] = 0.0;
+    if (j < A.nz) {
+      double part = 0.0;
+      if (ina) part = ba * P.Z[(size_t)ida * A.zs + j];
+      if (inb) part = fma(bb, P.Z[(size_t)idb * A.zs + j], part);
+      part = row_sum16(part);
+      e[j] = valid ? P.Z[(size_t)i * A.zs + j] - part : 0.0;
+    }
+  }
+  if (!ok) {
+    F = 1.0;  // keeps the sums finite; the pass reports the failure
+#pragma unroll
+    for (int j = 0; j < kMaxNZ; j++) e[j] = 0.0;
+  }
+  return ok;
+}
+
+// Logical block b (xcd_block) takes groups of four consecutive sites from
+// its XCD's run (xcd_site_run over the groups); each wave works on one
+// group, row g of the wave on site 4 q + g.  Each row's running sums live
+// in its own LDS slot (lane 0 of the row updates them); the block's row of
+// partial sums adds the slots in a fixed order.  Single problem, FP64.
+// MAT: the matern (c.cov == 2), by the pass's table.
+template <int MM, bool MAT>
+__global__ __launch_bounds__(64 * kCoopMaxWaves) void k_bf_grp(BfArgs A) {
+  extern __shared__ double smem[];
+  __shared__ double red[kGrpSlots][kStatsMax];
+  __shared__ CoopTabs<false> tabs;
+  const int nw = (int)blockDim.x >> 6;
+  const int w = (int)threadIdx.x >> 6;
+  const int t = (int)threadIdx.x & 63;
+  const int g = t >> 4;
+  const int u = t & 15;
+  const int b = xcd_block(A.coop_blocks);
+  if (b < 0) return;
+  const BfProblem P = bf_problem<false>(A);
+  const CovArgs c = A.c;
+  if (MAT) {  // the block's Bessel-K divisors and the pass's table
+    bessel_tab_fill(tabs.bt, c);
+    __syncthreads();
+    matern_tab_fill(tabs.mt, c, tabs.bt, smem);
+    __syncthreads();  // its scratch becomes the waves' coordinate areas
+  }
+  double *px = smem + (size_t)w * grp_wave_doubles(MM) + (size_t)g * 2 * MM;
+  double *py = px + MM;
+  const int slot = 4 * w + g;
+  const int nv = 2 + A.nz * (A.nz + 1) / 2;
+  for (int k = u; k < nv; k += 16) red[slot][k] = 0.0;
+  wave_sync();
+  const SiteRun run = xcd_site_run(A.coop_blocks, b, (A.n + 3) >> 2, nw, w);
+  for (int q = run.first; q < run.end; q += run.stride) {
+    const int i = 4 * q + g;
+    const bool valid = i < A.n;
+    double F, e[kMaxNZ];
+    const bool ok =
+        grp_site<MM, MAT>(A, P, c, tabs.bt, tabs.mt, px, py, i, valid, u, F,
+                          e);
+    if (valid && u == 0) {
+      if (A.F) A.F[i] = F;
+      if (!ok) atomicMin(P.err, i);
+      const double rf = 1.0 / F;
+      red[slot][0] += log(F);
+      int gg = 2;
+#pragma unroll
+      for (int p = 0; p < kMaxNZ; p++) {
+        if (p < A.nz) {
+#pragma unroll
+          for (int qq = 0; qq <= p; qq++)
+            red[slot][gg + qq] += e[p] * e[qq] * rf;
+        }
+        gg += p + 1;
+      }
+    }
+  }
+  if (!P.rows) return;
+  __syncthreads();
+  const int k = (int)threadIdx.x;
+  if (k < nv) {
+    double sum = 0.0;
+    if (k != 1)
+      for (int sl = 0; sl < 4 * nw; sl++) sum += red[sl][k];
+    P.rows[(size_t)k * A.rows_cap + b] = sum;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // sums
 // ---------------------------------------------------------------------------
 // Block (k, j): out[j*out_stride + k] = sum over rows r < nrows of column k
@@ -1027,6 +1228,10 @@ bool bf_config_ok(const BfConfig &cfg, int m, int variant) {
            (cfg.threads == 64 || cfg.threads == 128 || cfg.threads == 256) &&
            (cfg.mode == 0 || cfg.mode == 2 ||
             (cfg.mode == 1 && mixed_copy_shm(m, cfg.threads) <= 65536));
+  if (cfg.family == 2)
+    return m <= kGrpMaxM && cfg.mode == 0 &&
+           (cfg.threads == 64 || cfg.threads == 128 || cfg.threads == 256);
+  if (cfg.family != 1) return false;
   const int waves = cfg.threads / 64;
   return m <= kCoopMaxM && cfg.threads % 64 == 0 && waves >= 1 &&
          waves <= kCoopMaxWaves && coop_shm(m, waves) <= 65536;
@@ -1053,7 +1258,9 @@ BfConfig bf_default(int m, int variant) {
 int bf_rows(const BfArgs &a, const BfConfig &cfg) {
   if (cfg.family == 0) return (a.n + cfg.threads - 1) / cfg.threads;
   const int waves = cfg.threads / 64;
-  const int need = (a.n + waves - 1) / waves;
+  // the row kernel's waves take groups of four sites
+  const int units = cfg.family == 2 ? (a.n + 3) / 4 : a.n;
+  const int need = (units + waves - 1) / waves;
   return need < kCoopBlocks ? need : kCoopBlocks;
 }
 
@@ -1201,6 +1408,30 @@ static void launch_coop(const BfArgs &a, const dim3 &grid, const dim3 &blk,
   }
 }
 
+// LDS of the row kernel's block: the matern table's build, then the
+// waves' coordinate areas.
+static size_t grp_shm(const BfArgs &a, int waves, int mm) {
+  const size_t d = (size_t)waves * grp_wave_doubles(mm);
+  const size_t need =
+      a.c.cov == 2 && d < (size_t)kMatScratch ? (size_t)kMatScratch : d;
+  return need * sizeof(double);
+}
+
+static void launch_grp(const BfArgs &a, const dim3 &grid, const dim3 &blk,
+                       hipStream_t st) {
+  const int waves = (int)blk.x / 64;
+  const int mm = a.m <= 16 ? 16 : 32;
+  const size_t shm = grp_shm(a, waves, mm);
+  const bool mat = a.c.cov == 2;
+  if (mm == 16) {
+    if (mat) hipLaunchKernelGGL((k_bf_grp<16, true>), grid, blk, shm, st, a);
+    else hipLaunchKernelGGL((k_bf_grp<16, false>), grid, blk, shm, st, a);
+  } else {
+    if (mat) hipLaunchKernelGGL((k_bf_grp<32, true>), grid, blk, shm, st, a);
+    else hipLaunchKernelGGL((k_bf_grp<32, false>), grid, blk, shm, st, a);
+  }
+}
+
 int bf_launch(const BfArgs &a0, const BfConfig &cfg, int nprob,
               hipStream_t st) {
   if (a0.n <= 0 || nprob <= 0) return 0;
@@ -1224,6 +1455,10 @@ int bf_launch(const BfArgs &a0, const BfConfig &cfg, int nprob,
     else if (cfg.mode == 2 && lane_lds_applies(a, cfg.threads))
       launch_lane_lds(a, variant, grid, blk, st);
     else launch_lane<false>(a, variant, grid, blk, st);
+  } else if (cfg.family == 2) {
+    if (batch || a.mixed) return -1;  // single FP64 problems only
+    a.coop_blocks = nrows;
+    launch_grp(a, grid, blk, st);
   } else {
     a.coop_blocks = nrows;
     const int waves = cfg.threads / 64;
@@ -1287,6 +1522,8 @@ int bf_autotune(const BfArgs &a, int nprob, hipStream_t st, BfConfig *best,
     }
   }
   for (int w : {1, 2, 4}) cand.push_back(BfConfig{1, 64 * w, 0});
+  if (!a.mixed && a.ng == 0)  // four sites per wavefront, one per row
+    for (int th : {64, 128, 256}) cand.push_back(BfConfig{2, th, 0});
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess) return -1;
   if (hipEventCreate(&e1) != hipSuccess) {

@@ -888,7 +888,7 @@ int nngp_cvgrid_create(nngp_cvgrid **out, const double *y, const double *X,
     // grid is FP64, so mode 2 (covariate rows staged in LDS) is the only
     // per-lane mode besides 0
     const int variant = covModel == 2 ? kCovMatern15 : covModel;
-    if (!bf_config_ok(pin, m, variant) ||
+    if (!bf_config_ok(pin, m, variant) || pin.family == 2 ||  // batched
         (pin.family == 0 && pin.mode != 0 && pin.mode != 2)) {
       delete c;
       return set_err(NNGP_ERR_ARG,

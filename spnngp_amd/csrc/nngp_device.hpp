@@ -405,16 +405,14 @@ __device__ inline void matern_tab_fill(MaternTab &T, const CovArgs &a,
 
 // s2 * matern correlation at distance d from the table (the series outside
 // its range).
-__device__ __forceinline__ double matern_s2_fast(double d, const CovArgs &a,
-                                                 const BesselTab &bt,
-                                                 const MaternTab &T) {
-  const double x = d * a.phi;
-  if (!(x > 0.0)) return a.s2;
-  const int o = __builtin_amdgcn_frexp_exp(x) - 1 - kMatOct0;
-  if ((unsigned)o >= (unsigned)kMatOct) {
-    if (x > 700.0) return 0.0;
-    return a.mnorm * xnu_bessel_k(x, a, bt);
-  }
+// The table's octave of x > 0: inside the table when 0 <= o < kMatOct.
+__device__ __forceinline__ int matern_tab_octave(double x) {
+  return __builtin_amdgcn_frexp_exp(x) - 1 - kMatOct0;
+}
+
+// The table's s2 * rho at x, for x > 0 in octave o of the table.
+__device__ __forceinline__ double matern_tab_at(double x, int o,
+                                                const MaternTab &T) {
   const double m = 2.0 * __builtin_amdgcn_frexp_mant(x);  // in [1, 2)
   const int hi = m >= kSqrt2 ? 1 : 0;
   const double u = hi ? fma(m, 1.0 / mat_h(1), -mat_c(1) / mat_h(1))
@@ -424,6 +422,19 @@ __device__ __forceinline__ double matern_s2_fast(double d, const CovArgs &a,
 #pragma unroll
   for (int k = kMatDeg - 1; k >= 0; k--) r = fma(r, u, p[k]);
   return r;
+}
+
+__device__ __forceinline__ double matern_s2_fast(double d, const CovArgs &a,
+                                                 const BesselTab &bt,
+                                                 const MaternTab &T) {
+  const double x = d * a.phi;
+  if (!(x > 0.0)) return a.s2;
+  const int o = matern_tab_octave(x);
+  if ((unsigned)o >= (unsigned)kMatOct) {
+    if (x > 700.0) return 0.0;
+    return a.mnorm * xnu_bessel_k(x, a, bt);
+  }
+  return matern_tab_at(x, o, T);
 }
 
 // Covariance variants of the templated kernels: the four covModels, plus

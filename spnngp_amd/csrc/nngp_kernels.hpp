@@ -62,7 +62,9 @@ __host__ __device__ inline void nn_closed(int i, int m, int &off, int &cnt) {
 
 // Launch shape of a pass: family 0 = one site per lane (threads 64, 128 or
 // 256 per block), family 1 = one site per wavefront (threads = 64 x waves,
-// waves 1, 2 or 4, each wave an LDS tile of one system).
+// waves 1, 2 or 4, each wave an LDS tile of one system), family 2 = one
+// site per 16-lane row, four per wavefront (m <= 32, single FP64 problems;
+// threads 64, 128 or 256).
 struct BfConfig {
   int family = 0;
   int threads = 256;

@@ -108,7 +108,8 @@ def nngp_prior_data(oracle, c, m, sigma_sq, phi, tau_sq, beta, rng, cov=0,
     return X, y, nn
 
 
-def loglik_pair(oracle, nngp, c, nn, X, y, m, cov, theta, beta):
+def loglik_pair(oracle, nngp, c, nn, X, y, m, cov, theta, beta,
+                fp64_shape=None):
     n = c.shape[0]
     B0, F0, ld0 = oracle.updateBF(c.T.reshape(-1).copy(), nn["nnIndx"],
                                   nn["nnIndxLU"], n, m, theta, cov,
@@ -119,7 +120,8 @@ def loglik_pair(oracle, nngp, c, nn, X, y, m, cov, theta, beta):
     out = {}
     for prec in (0, 1):
         with nngp.Model(c, nn["nnIndx"], nn["nnIndxLU"], m, cov, X, y,
-                        precision=prec) as md:
+                        precision=prec,
+                        shape=fp64_shape if prec == 0 else None) as md:
             o = md.update(theta, slot=0, beta=beta)
             B, F = md.get_bf(0)
         ll = -0.5 * o["logDet"] - 0.5 * o["qResid"]
@@ -235,6 +237,11 @@ def test_config5_systems_fp64(oracle, nngp, nu):
     print(f"  fp64 {e[0]}\n  mixed {e[1]}", flush=True)
     assert e[0]["ll"] < RTOL and e[0]["F"] < RTOL and e[0]["Bsite"] < RTOL
     assert e[1]["F"] < 5e-5 and e[1]["Bsite"] < 1e-3
+    # the FP64 pass on the 16-lane-row kernel (four sites a wavefront)
+    e = loglik_pair(oracle, nngp, c, nn, X, y, m, 2, [1.0, 0.05, 20.0, nu],
+                    beta, fp64_shape=(2, 256, 0))
+    print(f"  fp64 row kernel {e[0]}", flush=True)
+    assert e[0]["ll"] < RTOL and e[0]["F"] < RTOL and e[0]["Bsite"] < RTOL
 
 
 def test_config5_mixed_loglik_at_full_size(nngp):

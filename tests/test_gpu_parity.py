@@ -122,7 +122,8 @@ def test_updateBF_matches_oracle(oracle, nngp, n, m, cov, theta):
 
 
 SHAPES = [(0, 64, 0), (0, 128, 0), (0, 256, 0), (0, 64, 2), (0, 128, 2),
-          (0, 256, 2), (1, 64, 0), (1, 128, 0), (1, 256, 0)]
+          (0, 256, 2), (1, 64, 0), (1, 128, 0), (1, 256, 0), (2, 64, 0),
+          (2, 128, 0), (2, 256, 0)]
 
 
 @pytest.mark.gpu
@@ -131,9 +132,10 @@ SHAPES = [(0, 64, 0), (0, 128, 0), (0, 256, 0), (0, 64, 2), (0, 128, 2),
                                          (1, 6, [1.5, 0.5, 2.0])])
 def test_every_launch_shape_matches_oracle(oracle, nngp, cov, m, theta):
     """Each updateBF launch shape the autotuner may pick (one site per lane
-    in 64/128/256-thread blocks, one per wavefront in 1/2/4-wave blocks)
-    gives B, F, the log-determinant and Q within the parity bound; n is not
-    a multiple of 64, so the last block is ragged."""
+    in 64/128/256-thread blocks, one per wavefront in 1/2/4-wave blocks, one
+    per 16-lane row in 64/128/256-thread blocks) gives B, F, the
+    log-determinant and Q within the parity bound; n is not a multiple of
+    64, so the last block is ragged."""
     n = 5000 + 37
     print(f"shapes cov={cov} m={m}", flush=True)
     c = uniform_coords(n, 300 + m)
@@ -155,6 +157,74 @@ def test_every_launch_shape_matches_oracle(oracle, nngp, cov, m, theta):
             assert rel_err(F, F0) < RTOL
             assert abs(out["logDet"] - ld0) < RTOL * abs(ld0)
             assert abs(out["qResid"] - q0) < RTOL * abs(q0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cov,m,p,theta", [
+    (2, 32, 3, [1.0, 0.05, 20.0, 0.7]),
+    (2, 24, 1, [2.0, 0.2, 8.0, 1.2]),
+    (2, 17, 0, [1.0, 0.3, 6.0, 2.2]),
+    (2, 16, 2, [1.0, 0.1, 9.0, 0.35]),
+    (0, 32, 2, [1.0, 0.05, 10.0]),
+    (3, 20, 3, [1.0, 0.1, 5.0]),
+    (1, 28, 1, [1.5, 0.5, 2.0]),
+])
+def test_row_kernel_matches_oracle_and_the_wavefront_kernel(oracle, nngp,
+                                                            cov, m, p, theta):
+    """The 16-lane-row kernel (four sites a wavefront; lane u of a site's
+    row holds rows u and u + 16 of its system, DPP broadcasts within the
+    row) at 64, 128 and 256 threads: B, F, the log determinant and Q within
+    the parity bound of the oracle, and the cross-products within it of the
+    wavefront kernel's.  n = 4003 leaves a ragged last group of four, and
+    the first m sites have fewer than m neighbors (identity rows)."""
+    n = 4003
+    print(f"row kernel cov={cov} m={m} p={p}", flush=True)
+    c = uniform_coords(n, 700 + m + p)
+    rng = np.random.default_rng(m + 100 * p)
+    X = np.column_stack([np.ones(n), rng.normal(size=(n, p - 1))]) \
+        if p else None
+    beta = np.linspace(1.0, -1.0, p) if p else np.zeros(1)
+    y = (X @ beta if p else 0.0) + rng.normal(size=n)
+    nn = oracle.mkNNIndx(c, m)
+    B0, F0, ld0 = oracle.updateBF(c.T.reshape(-1).copy(), nn["nnIndx"],
+                                  nn["nnIndxLU"], n, m, theta, cov)
+    if p:
+        r = X @ beta - y
+        q0 = oracle.Q(B0, F0, r, r, nn["nnIndx"], nn["nnIndxLU"])
+    with nngp.Model(c, nn["nnIndx"], nn["nnIndxLU"], m, cov, X, y) as md:
+        md.set_shape(1, 256, 0)
+        ref = md.update(theta, slot=0, beta=beta)
+        for th in (64, 128, 256):
+            md.set_shape(2, th, 0)
+            out = md.update(theta, slot=1, beta=beta)
+            B, F = md.get_bf(1)
+            assert per_site_B_err(B, B0, nn["nnIndxLU"], n) < RTOL
+            assert rel_err(F, F0) < RTOL
+            assert abs(out["logDet"] - ld0) < RTOL * abs(ld0)
+            if p:
+                assert abs(out["qResid"] - q0) < RTOL * abs(q0)
+                for k in ("XtViX", "XtViy"):
+                    assert np.max(np.abs(out[k] - ref[k])) \
+                        < RTOL * np.max(np.abs(ref[k]))
+
+
+@pytest.mark.gpu
+def test_row_kernel_shapes_are_checked(nngp):
+    """The row kernel takes m <= 32 and FP64 passes only."""
+    c = uniform_coords(600, 10)
+    for m, ok in ((32, True), (33, False)):
+        nn = nngp.mkNNIndxCB(c, m)
+        with nngp.Model(c, nn["nnIndx"], nn["nnIndxLU"], m, 0) as md:
+            if ok:
+                md.set_shape(2, 128, 0)
+            else:
+                with pytest.raises(nngp.NngpError):
+                    md.set_shape(2, 128, 0)
+    nn = nngp.mkNNIndxCB(c, 15)
+    with nngp.Model(c, nn["nnIndx"], nn["nnIndxLU"], 15, 2,
+                    precision=1) as md:
+        with pytest.raises(nngp.NngpError):
+            md.set_shape(2, 64, 0)
 
 
 def test_updateBF_with_a_non_canonical_layout(oracle, nngp):
