@@ -413,8 +413,10 @@ typedef struct nngp_tune_report {
  * wavefront 64, 4 or 1; 64/128/256-thread blocks or 1/2/4-system LDS tiles)
  * and use the fastest for every later pass.  Opt-in: the shape fixes the
  * order of the per-block sums, so a chain is bit-reproducible for a fixed
- * shape; without this call the shape is fixed (256-thread blocks where the
- * per-lane kernel applies). */
+ * shape; without this call the shape is fixed: the per-lane kernel in
+ * 64-thread blocks with the covariate rows staged where it applies, else
+ * for FP64 passes with m <= 32 the 16-lane-row kernel in 256-thread blocks,
+ * else the wavefront kernel in its widest LDS tile. */
 int nngp_model_autotune(nngp_model *model, const double *theta,
                         nngp_tune_report *report);
 

@@ -957,22 +957,25 @@ __host__ __device__ inline int grp_wave_doubles(int mm) { return 4 * 2 * mm; }
 // site in fully unrolled code, so only the common path is inline: for the
 // matern (MAT) the pass's table, with near-coincident sites and x >= 64
 // (outside the table) out of line; otherwise the exponential, spherical and
-// gaussian.
-__device__ __noinline__ double grp_matern_outside(double x, const CovArgs &a,
+// gaussian.  The out-of-line series reads the parameters from the block's
+// LDS copy `al` (a reference to the kernel argument would make the kernel
+// keep a stack copy of it and read the inline path's fields from there).
+__device__ __noinline__ double grp_matern_outside(double x, const CovArgs &al,
                                                   const BesselTab &bt) {
   if (x > 700.0) return 0.0;
-  return a.mnorm * xnu_bessel_k(x, a, bt);
+  return al.mnorm * xnu_bessel_k(x, al, bt);
 }
 
 template <bool MAT>
 __device__ __forceinline__ double grp_corr(double d2, const CovArgs &a,
+                                           const CovArgs &al,
                                            const BesselTab &bt,
                                            const MaternTab &T) {
   if constexpr (MAT) {
     const double x = nn_sqrt(d2) * a.phi;
     if (!(x > 0.0)) return a.s2;
     const int o = matern_tab_octave(x);
-    if ((unsigned)o >= (unsigned)kMatOct) return grp_matern_outside(x, a, bt);
+    if ((unsigned)o >= (unsigned)kMatOct) return grp_matern_outside(x, al, bt);
     return matern_tab_at(x, o, T);
   } else {
     switch (a.cov) {
@@ -983,41 +986,70 @@ __device__ __forceinline__ double grp_corr(double d2, const CovArgs &a,
   }
 }
 
-// Site i on the 16-lane row of lane u (valid: i < n).  The lane builds rows
-// u and u + 16 of the site's system in registers from the neighbor
-// coordinates the row stages in px/py (rows past the site's count are
-// identity rows with a zero right-hand side, so B and F are those of the
-// cnt x cnt system), then factor, forward and back substitution
-// (nngp_group.hpp).  Every lane of the row returns F and the residuals e;
-// false when a pivot or F is not positive.
+// A site's inputs on its row: neighbor count and offset, the lane's two
+// neighbor ids and coordinates, the site's coordinates.  The three loads
+// depend on each other, so the row kernel fetches the next site's in three
+// stages spread over the current site's work.
+struct GrpIn {
+  int cnt, off, ida, idb;
+  double2 qa, qb, s;
+};
+
+__device__ __forceinline__ void grp_in_lu(const BfArgs &A, const BfProblem &P,
+                                          int i, bool valid, GrpIn &in) {
+  in.cnt = 0;
+  in.off = 0;
+  if (valid) {
+    in.cnt = P.lu[A.n + i];
+    in.off = P.lu[i];
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void grp_in_ids(const BfProblem &P, int u,
+                                           GrpIn &in) {
+  in.ida = 0;
+  in.idb = 0;
+  if (u < in.cnt) in.ida = P.nnIndx[in.off + u];
+  if (R == 2 && u + 16 < in.cnt) in.idb = P.nnIndx[in.off + u + 16];
+}
+
+template <int R>
+__device__ __forceinline__ void grp_in_xy(const BfProblem &P, int u, int i,
+                                          bool valid, GrpIn &in) {
+  in.qa = make_double2(0.0, 0.0);
+  in.qb = in.qa;
+  in.s = in.qa;
+  if (u < in.cnt) in.qa = P.xy[in.ida];
+  if (R == 2 && u + 16 < in.cnt) in.qb = P.xy[in.idb];
+  if (valid) in.s = P.xy[i];
+}
+
+// Site i on the 16-lane row of lane u (valid: i < n), its inputs in `in`.
+// The lane builds rows u and u + 16 of the site's system in registers from
+// the neighbor coordinates the row stages in px/py (rows past the site's
+// count are identity rows with a zero right-hand side, so B and F are those
+// of the cnt x cnt system), then factor, forward and back substitution
+// (nngp_group.hpp).  Meanwhile it fetches the inputs of the row's next site
+// in2 (site i2, valid2).  Every lane of the row returns F and the residuals
+// e; false when a pivot or F is not positive.
 template <int MM, bool MAT>
 __device__ inline bool grp_site(const BfArgs &A, const BfProblem &P,
-                                const CovArgs &c, const BesselTab &bt,
+                                const CovArgs &c, const CovArgs &al,
+                                const BesselTab &bt,
                                 const MaternTab &mt, double *px, double *py,
-                                int i, bool valid, int u, double &F,
+                                int i, bool valid, const GrpIn &in, int i2,
+                                bool valid2, GrpIn &in2, int u, double &F,
                                 double (&e)[kMaxNZ]) {
   constexpr int R = GrpRows<MM>::R;
   // opaque per site: the compares of u with every row number below would
   // otherwise be hoisted out of the caller's site loop as lane masks, more
   // than the SGPR file holds
   u = lane_opaque(u);
-  int cnt = 0, off = 0;
-  if (valid) {
-    cnt = P.lu[A.n + i];
-    off = P.lu[i];
-  }
+  const int cnt = in.cnt, off = in.off, ida = in.ida, idb = in.idb;
+  const double2 qa = in.qa, qb = in.qb, s = in.s;
   const bool ina = u < cnt, inb = R == 2 && u + 16 < cnt;
-  int ida = 0, idb = 0;
-  double2 qa = make_double2(0.0, 0.0), qb = qa;
-  if (ina) {
-    ida = P.nnIndx[off + u];
-    qa = P.xy[ida];
-  }
-  if (inb) {
-    idb = P.nnIndx[off + u + 16];
-    qb = P.xy[idb];
-  }
-  const double2 s = valid ? P.xy[i] : make_double2(0.0, 0.0);
+  grp_in_lu(A, P, i2, valid2, in2);  // next site, stage 1
   px[u] = qa.x;
   py[u] = qa.y;
   if constexpr (R == 2) {
@@ -1031,22 +1063,24 @@ __device__ inline bool grp_site(const BfArgs &A, const BfProblem &P,
     constexpr int l = decltype(lc)::value;
     S.ra[l] = l == u ? (ina ? c.diag : 1.0) : 0.0;
     if (l < u && ina)
-      S.ra[l] = grp_corr<MAT>(sqdist(qa.x, qa.y, px[l], py[l]), c, bt, mt);
+      S.ra[l] = grp_corr<MAT>(sqdist(qa.x, qa.y, px[l], py[l]), c, al, bt, mt);
   });
   if constexpr (R == 2) {  // row u + 16
     static_for<0, 32>([&](auto lc) {
       constexpr int l = decltype(lc)::value;
       S.rb[l] = l == u + 16 ? (inb ? c.diag : 1.0) : 0.0;
       if (l < u + 16 && inb)
-        S.rb[l] = grp_corr<MAT>(sqdist(qb.x, qb.y, px[l], py[l]), c, bt, mt);
+        S.rb[l] = grp_corr<MAT>(sqdist(qb.x, qb.y, px[l], py[l]), c, al, bt, mt);
     });
   }
   const double ca =
-      ina ? grp_corr<MAT>(sqdist(s.x, s.y, qa.x, qa.y), c, bt, mt) : 0.0;
+      ina ? grp_corr<MAT>(sqdist(s.x, s.y, qa.x, qa.y), c, al, bt, mt) : 0.0;
   const double cb =
-      inb ? grp_corr<MAT>(sqdist(s.x, s.y, qb.x, qb.y), c, bt, mt) : 0.0;
+      inb ? grp_corr<MAT>(sqdist(s.x, s.y, qb.x, qb.y), c, al, bt, mt) : 0.0;
   wave_sync();  // the row's coordinates are read; the next site rewrites them
+  grp_in_ids<R>(P, u, in2);  // next site, stage 2
   bool ok = grp_factor<MM>(S, u);
+  grp_in_xy<R>(P, u, i2, valid2, in2);  // next site, stage 3
   double ya, yb, ba, bb;
   grp_forward<MM>(S, u, ca, cb, ya, yb);
   F = c.diag - row_sum16(ya * ya + yb * yb);
@@ -1086,6 +1120,7 @@ __global__ __launch_bounds__(64 * kCoopMaxWaves) void k_bf_grp(BfArgs A) {
   extern __shared__ double smem[];
   __shared__ double red[kGrpSlots][kStatsMax];
   __shared__ CoopTabs<false> tabs;
+  __shared__ CovArgs cl;  // for the out-of-line matern series
   const int nw = (int)blockDim.x >> 6;
   const int w = (int)threadIdx.x >> 6;
   const int t = (int)threadIdx.x & 63;
@@ -1096,6 +1131,7 @@ __global__ __launch_bounds__(64 * kCoopMaxWaves) void k_bf_grp(BfArgs A) {
   const BfProblem P = bf_problem<false>(A);
   const CovArgs c = A.c;
   if (MAT) {  // the block's Bessel-K divisors and the pass's table
+    if (threadIdx.x == 0) cl = c;
     bessel_tab_fill(tabs.bt, c);
     __syncthreads();
     matern_tab_fill(tabs.mt, c, tabs.bt, smem);
@@ -1108,13 +1144,25 @@ __global__ __launch_bounds__(64 * kCoopMaxWaves) void k_bf_grp(BfArgs A) {
   for (int k = u; k < nv; k += 16) red[slot][k] = 0.0;
   wave_sync();
   const SiteRun run = xcd_site_run(A.coop_blocks, b, (A.n + 3) >> 2, nw, w);
+  GrpIn in;
+  {  // the first site's inputs; later sites' arrive during their
+     // predecessor's work
+    const int i = 4 * run.first + g;
+    const bool valid = run.first < run.end && i < A.n;
+    grp_in_lu(A, P, i, valid, in);
+    grp_in_ids<GrpRows<MM>::R>(P, u, in);
+    grp_in_xy<GrpRows<MM>::R>(P, u, i, valid, in);
+  }
   for (int q = run.first; q < run.end; q += run.stride) {
     const int i = 4 * q + g;
     const bool valid = i < A.n;
+    const int q2 = q + run.stride, i2 = 4 * q2 + g;
+    const bool valid2 = q2 < run.end && i2 < A.n;
     double F, e[kMaxNZ];
-    const bool ok =
-        grp_site<MM, MAT>(A, P, c, tabs.bt, tabs.mt, px, py, i, valid, u, F,
-                          e);
+    GrpIn in2;
+    const bool ok = grp_site<MM, MAT>(A, P, c, cl, tabs.bt, tabs.mt, px, py,
+                                      i, valid, in, i2, valid2, in2, u, F, e);
+    in = in2;
     if (valid && u == 0) {
       if (A.F) A.F[i] = F;
       if (!ok) atomicMin(P.err, i);
@@ -1239,13 +1287,19 @@ bool bf_config_ok(const BfConfig &cfg, int m, int variant) {
 
 // Untuned passes: per lane at 64 threads with the covariate rows staged
 // in LDS where that applies (the fastest per-lane shape at n = 1e6, m = 15:
-// 0.21 ms a pass against 0.35 ms at 256 threads without staging).
-BfConfig bf_default(int m, int variant) {
+// 0.21 ms a pass against 0.35 ms at 256 threads without staging); else,
+// for single FP64 problems with m <= 32, the row kernel at 256 threads
+// (n = 4e6, m = 32, Bessel-K matern: 14.8 ms against 41 ms on the
+// wavefront kernel); else the wavefront kernel.
+BfConfig bf_default(int m, int variant, bool single_fp64) {
   BfConfig c;
   if (bf_lane_ok(m, variant)) {
     c.family = 0;
     c.threads = 64;
     c.mode = 2;
+  } else if (single_fp64 && m <= kGrpMaxM) {
+    c.family = 2;
+    c.threads = 256;
   } else {
     c.family = 1;
     int w = kCoopMaxWaves;
@@ -1532,7 +1586,7 @@ int bf_autotune(const BfArgs &a, int nprob, hipStream_t st, BfConfig *best,
   }
   const int reps = 3;
   double best_ms = 1e300;
-  BfConfig pick = bf_default(a.m, variant);
+  BfConfig pick = bf_default(a.m, variant, !a.mixed && a.ng == 0);
   int nc = 0;
   int rc = 0;
   for (const BfConfig &cfg : cand) {

@@ -339,8 +339,11 @@ constexpr int kMatN = kMatDeg + 1;        // nodes and coefficients a piece
 constexpr int kMatPieces = 2 * kMatOct;
 constexpr double kSqrt2 = 1.4142135623730951;
 
+// Rows padded by one double: lanes evaluating neighboring pieces then read
+// their k-th coefficients from different LDS banks (at 16 doubles a row,
+// every piece's k-th coefficient sat in the same bank).
 struct MaternTab {
-  double a[kMatPieces][kMatN];  // monomial coefficients in u, s2 folded in
+  double a[kMatPieces][kMatN + 1];  // monomial coefficients in u, s2 folded
 };
 
 // Piece of mantissa m in [1, 2): lo [1, sqrt 2), hi [sqrt 2, 2); u = m / h

@@ -325,7 +325,7 @@ int Model::bf(const CovArgs &c, int slot, bool storeB) {
   a.variant = cov_variant(c.cov, c.nu);
   a.mixed = precision == NNGP_MIXED ? 1 : 0;
   a.lu_closed = lu_closed ? 1 : 0;
-  BfConfig cfg = bf_default(m, a.variant);
+  BfConfig cfg = bf_default(m, a.variant, !a.mixed);
   if (a.mixed) {  // untuned mixed passes keep the covariance copy in LDS
     const BfConfig copy{0, 64, 1};
     if (bf_config_ok(copy, m, a.variant)) cfg = copy;

@@ -99,8 +99,10 @@ int q_partial_rows(int n);
 int rows_capacity(int n);
 bool bf_lane_ok(int m, int variant);
 bool bf_config_ok(const BfConfig &cfg, int m, int variant);
-// Per-lane kernel at 256 threads when it applies, else the widest LDS tile.
-BfConfig bf_default(int m, int variant);
+// Untuned launch shape: the per-lane kernel where it applies, else the row
+// kernel for single FP64 problems (single_fp64) with m <= 32, else the
+// wavefront kernel's widest LDS tile.
+BfConfig bf_default(int m, int variant, bool single_fp64);
 // Partial rows one problem of a pass writes under cfg.
 int bf_rows(const BfArgs &a, const BfConfig &cfg);
 // nprob problems (1 unless a.ng > 0).  -1: cfg does not apply to (m,

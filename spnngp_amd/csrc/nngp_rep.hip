@@ -240,7 +240,7 @@ extern "C" int nngp_rNNGPReplicated(
       a.c = c;
       a.variant = cov_variant(c.cov, c.nu);
       a.lu_closed = md.lu_closed ? 1 : 0;
-      BfConfig cfg = bf_default(m, a.variant);
+      BfConfig cfg = bf_default(m, a.variant, true);
       if (md.have_tuned && bf_config_ok(md.tuned_cfg, m, a.variant))
         cfg = md.tuned_cfg;
       const int rc = bf_launch(a, cfg, 1, st);
