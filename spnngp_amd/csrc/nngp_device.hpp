@@ -100,6 +100,21 @@ __device__ __forceinline__ double s2_exp2(double t, const CovArgs &a) {
   return ldexp(p, (int)k);
 }
 
+// s2_exp2 for finite t <= 0, without the clamp: for t far below -1075 the
+// conversion of k to int saturates at INT_MIN and ldexp returns 0, and f =
+// t - k stays exact (0 once |t| >= 2^52), so the result is the same.  Only
+// an infinite t (which the per-lane kernels never form: their padding sites
+// sit at finite coordinates) would give NaN here.
+__device__ __forceinline__ double s2_exp2_finite(double t,
+                                                 const CovArgs &a) {
+  const double k = __builtin_rint(t);
+  const double f = t - k;
+  double p = a.ec2[kExp2Deg];
+#pragma unroll
+  for (int j = kExp2Deg - 1; j >= 0; j--) p = fma(p, f, a.ec2[j]);
+  return ldexp(p, (int)k);
+}
+
 // K_nu(x) for x > 0 by Temme's series (x <= 2) or Steed's continued
 // fraction (x > 2) for K_mu, K_{mu+1}, |mu| <= 1/2, then upward recurrence.
 __device__ inline double bessel_k(double x, const CovArgs &a) {
@@ -494,16 +509,16 @@ template <int COV>
 __device__ __forceinline__ double lane_corr(double d2, const CovArgs &c) {
   constexpr double kLn2 = 0.6931471805599453;  // phi d = d' ln 2
   if constexpr (COV == 0) {
-    return s2_exp2(-nn_sqrt(d2), c);
+    return s2_exp2_finite(-nn_sqrt(d2), c);
   } else if constexpr (COV == 3) {
-    return s2_exp2(-d2, c);
+    return s2_exp2_finite(-d2, c);
   } else if constexpr (COV == kCovMatern15) {
     const double dd = nn_sqrt(d2);
-    return s2_exp2(-dd, c) * (1.0 + dd * kLn2);
+    return s2_exp2_finite(-dd, c) * (1.0 + dd * kLn2);
   } else if constexpr (COV == kCovMatern25) {
     const double dd = nn_sqrt(d2);
     const double x = dd * kLn2;
-    return s2_exp2(-dd, c) * (1.0 + x + x * x * (1.0 / 3.0));
+    return s2_exp2_finite(-dd, c) * (1.0 + x + x * x * (1.0 / 3.0));
   } else {
     return s2_corr<COV>(d2, c);
   }
