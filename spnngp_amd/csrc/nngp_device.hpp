@@ -104,7 +104,11 @@ __device__ __forceinline__ double s2_exp2(double t, const CovArgs &a) {
 // conversion of k to int saturates at INT_MIN and ldexp returns 0, and f =
 // t - k stays exact (0 once |t| >= 2^52), so the result is the same.  Only
 // an infinite t (which the per-lane kernels never form: their padding sites
-// sit at finite coordinates) would give NaN here.
+// sit at finite coordinates) would give NaN here.  t must arrive rounded:
+// the per-lane kernels pass -sqrt(d2) or -d2 from a fused multiply-add.  A
+// product passed as t could be contracted into t - k, which would leave f
+// as the product's rounding error (about 1e84 for a distance of 1e100)
+// instead of 0; the clamp of s2_exp2 prevents that contraction.
 __device__ __forceinline__ double s2_exp2_finite(double t,
                                                  const CovArgs &a) {
   const double k = __builtin_rint(t);
