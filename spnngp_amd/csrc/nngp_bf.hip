@@ -1200,7 +1200,7 @@ __global__ __launch_bounds__(64 * kCoopMaxWaves) void k_bf_grp(BfArgs A) {
 // err_out[j] = err[j], err[j] = INT_MAX.
 __global__ __launch_bounds__(kRowsThreads) void k_rows_sum(
     const double *rows, int cap, int nrows, long long ps_rows, double *out,
-    int out_stride, int *err, double *err_out) {
+    int out_stride, int *err, double *err_out, SumsSignal sig) {
   __shared__ double red[kRowsThreads / 64];
   const int k = (int)blockIdx.x, j = (int)blockIdx.y;
   const int w = (int)threadIdx.x >> 6, t = (int)threadIdx.x & 63;
@@ -1235,15 +1235,25 @@ __global__ __launch_bounds__(kRowsThreads) void k_rows_sum(
       err_out[j] = (double)err[j];
       err[j] = INT_MAX;
     }
+    if (sig.flag) {  // the completion word, from the last block to finish
+      __threadfence_system();  // this block's outputs, visible to the host
+      const unsigned nb = gridDim.x * gridDim.y;
+      if ((unsigned)atomicAdd(sig.done, 1) == nb - 1) {
+        *sig.done = 0;
+        __hip_atomic_store(sig.flag, sig.seq, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+      }
+    }
   }
 }
 
 int rows_sum_launch(const double *rows, int cap, int nrows, long long ps_rows,
                     int nv, int nprob, double *out, int out_stride, int *err,
-                    double *err_out, hipStream_t st) {
+                    double *err_out, hipStream_t st, const SumsSignal *sig) {
+  const SumsSignal none;
   hipLaunchKernelGGL(k_rows_sum, dim3(nv, nprob), dim3(kRowsThreads), 0, st,
                      rows, cap, nrows, ps_rows, out, out_stride, err,
-                     err_out);
+                     err_out, sig ? *sig : none);
   return (int)hipGetLastError();
 }
 
@@ -1530,10 +1540,10 @@ int bf_launch(const BfArgs &a0, const BfConfig &cfg, int nprob,
 
 int bf_sums_launch(const BfArgs &a, const BfConfig &cfg, int nprob,
                    double *out, int out_stride, double *err_out,
-                   hipStream_t st) {
+                   hipStream_t st, const SumsSignal *sig) {
   const int nv = 2 + a.nz * (a.nz + 1) / 2;
   return rows_sum_launch(a.rows, a.rows_cap, bf_rows(a, cfg), a.ps_rows, nv,
-                         nprob, out, out_stride, a.err, err_out, st);
+                         nprob, out, out_stride, a.err, err_out, st, sig);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,6 +1,8 @@
 // nngp_host.hip — host helpers and the device-resident model of
 // libnngp_hip.so (see nngp_host.hpp).
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <climits>
 #include <cstring>
 
@@ -289,13 +291,19 @@ int Model::create(int n_, int m_, int p_, const double *coords,
   NNGP_TRY(err.alloc(sizeof(int)));
   if (fill_int_launch(err.as<int>(), INT_MAX, st.s))
     return set_err(NNGP_ERR_HIP, "flag reset kernel failed");
-  NNGP_TRY(hsums.alloc(sizeof(double) * kSumsLen));
+  NNGP_TRY(hsums.alloc(sizeof(double) * (kSumsLen + 1)));
   {
     void *dp = nullptr;
     if (hipHostGetDevicePointer(&dp, hsums.p, 0) == hipSuccess && dp)
       sums_mapped = static_cast<double *>(dp);
     else
       (void)hipGetLastError();  // fall back to a copy per pass
+  }
+  if (sums_mapped) {
+    NNGP_TRY(sig_done.alloc(sizeof(int)));
+    if (fill_int_launch(sig_done.as<int>(), 0, st.s))
+      return set_err(NNGP_ERR_HIP, "counter reset kernel failed");
+    *reinterpret_cast<volatile unsigned *>(hsums.as<double>() + kSumsLen) = 0u;
   }
   NNGP_HIP(hipStreamSynchronize(st.s));
   if (opts && opts->shape[1] > 0)
@@ -397,8 +405,14 @@ int Model::sums_launch(int slot, int off) {
   double *err_out = base + 2 * kStatsMax + (off ? 1 : 0);
   const int ev = prof ? ev_begin(1) : -1;
   (void)slot;
-  const int rc =
-      bf_sums_launch(last_bf, last_cfg, 1, out, kStatsMax, err_out, st.s);
+  SumsSignal sig;
+  if (sums_mapped) {
+    sig.done = sig_done.as<int>();
+    sig.flag = reinterpret_cast<unsigned *>(sums_mapped + kSumsLen);
+    sig.seq = ++sig_seq;
+  }
+  const int rc = bf_sums_launch(last_bf, last_cfg, 1, out, kStatsMax, err_out,
+                                st.s, sums_mapped ? &sig : nullptr);
   if (ev >= 0) ev_end(ev);
   if (rc) return set_err(NNGP_ERR_HIP, "sum kernel launch failed");
   return NNGP_OK;
@@ -421,11 +435,35 @@ double quad_resid(const PassOut &po, const double *beta, int p) {
 }
 
 int Model::fetch_sums() {
-  if (!sums_mapped)
+  if (!sums_mapped) {
     NNGP_HIP(hipMemcpyAsync(hsums.p, sums.p, sizeof(double) * kSumsLen,
                             hipMemcpyDeviceToHost, st.s));
-  NNGP_HIP(hipStreamSynchronize(st.s));
-  if (prof) NNGP_TRY(collect());
+    NNGP_HIP(hipStreamSynchronize(st.s));
+  } else {
+    // Poll the last sum launch's completion word: about 5 us sooner than
+    // the stream's own completion signal.  The spin is bounded (0.5 s);
+    // past that, or when the sums never come (a failed launch), the stream
+    // synchronization below waits and reports as before.
+    const volatile unsigned *flag =
+        reinterpret_cast<const volatile unsigned *>(hsums.as<double>() +
+                                                    kSumsLen);
+    const auto t0 = std::chrono::steady_clock::now();
+    bool seen = false;
+    for (long spin = 0;; spin++) {
+      if (*flag == sig_seq) {
+        seen = true;
+        break;
+      }
+      if ((spin & 1023) == 1023 &&
+          std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(500))
+        break;
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    if (!seen) NNGP_HIP(hipStreamSynchronize(st.s));
+  }
+  // kernel timing: collected every 64 launches (and by nngp_model_timing),
+  // so that waiting for the events does not undo the polling above
+  if (prof && (!sums_mapped || nev >= 64)) NNGP_TRY(collect());
   return NNGP_OK;
 }
 

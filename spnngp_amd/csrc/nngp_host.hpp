@@ -285,9 +285,13 @@ struct Model {
   // sums: [0, kStatsMax) pass at off 0, [kStatsMax, 2 kStatsMax) pass at
   // off kStatsMax, then the updateBF failure flag of each (as doubles)
   DevBuf rows, sums, err;
-  PinnedBuf hsums;  // host mirror of sums
+  PinnedBuf hsums;  // host mirror of sums, then the completion word
   double *sums_mapped = nullptr;  // device address of hsums: the sum kernel
                                   // writes there directly (no copy)
+  // With sums_mapped, each sum launch ends by writing its sequence number
+  // to the word after the sums (SumsSignal), which fetch_sums polls.
+  DevBuf sig_done;        // the sum kernels' block counter
+  unsigned sig_seq = 0;   // sequence number of the last sum launch
   BfArgs last_bf;      // arguments of the last bf() launch
   BfConfig last_cfg;   // and its launch shape
   BfConfig tuned_cfg;  // launch shape chosen by autotune(), when have_tuned

@@ -109,15 +109,26 @@ int bf_rows(const BfArgs &a, const BfConfig &cfg);
 // variant); -2: bad batch or rows_cap too small; else a hipError_t.
 int bf_launch(const BfArgs &a, const BfConfig &cfg, int nprob,
               hipStream_t st);
+// Completion word of a sum launch in host memory: the launch's last block
+// to finish writes seq to *flag (a device address of host memory) once the
+// outputs of every block are visible to the host, so the host can poll it
+// instead of synchronizing the stream.  done: a device counter, zero
+// between launches (the last block resets it).
+struct SumsSignal {
+  int *done = nullptr;
+  unsigned *flag = nullptr;
+  unsigned seq = 0;
+};
 // Sums of the bf_launch with the same arguments: problem j's column k to
 // out[j*out_stride + k]; its failure flag to err_out[j] (and reset).
 int bf_sums_launch(const BfArgs &a, const BfConfig &cfg, int nprob,
                    double *out, int out_stride, double *err_out,
-                   hipStream_t st);
+                   hipStream_t st, const SumsSignal *sig = nullptr);
 // Column sums of nprob blocks of partial rows (see k_rows_sum).
 int rows_sum_launch(const double *rows, int cap, int nrows, long long ps_rows,
                     int nv, int nprob, double *out, int out_stride, int *err,
-                    double *err_out, hipStream_t st);
+                    double *err_out, hipStream_t st,
+                    const SumsSignal *sig = nullptr);
 // Time every applicable launch shape of this pass and keep the fastest
 // (cached per (m, variant, nz, size class) unless rep asks for timings).
 int bf_autotune(const BfArgs &a, int nprob, hipStream_t st, BfConfig *best,
