@@ -101,22 +101,14 @@ def make_data(n, m, seed):
     return c, X, y, idx, knn_s
 
 
-def mcmc_bench(c, X, y, idx, m, steps, warmup, start, precision,
-               gpu_warm_s=0.3):
+def mcmc_bench(c, X, y, idx, m, steps, warmup, start, precision):
     """Warm up, autotune the updateBF launch shape at the chain's current
-    theta, then time `steps` MH iterations with the data resident.  Before
-    tuning, the chain's updateBF pass runs for gpu_warm_s seconds (untimed)
-    so that the GPU's clocks have left their idle state, which the host-side
-    data generation before it leaves them in: the tuner and the timed steps
-    then both see the steady clock."""
+    theta, then time `steps` MH iterations with the data resident."""
     import spnngp_amd as nn
     md = nn.Model(c, idx["nnIndx"], idx["nnIndxLU"], m, "exponential", X, y,
                   precision=precision)
     w = md.rNNGP(nSamples=warmup, seed=11, **start)
     th_last = w["p.theta.samples"][:, -1]
-    t_warm = time.perf_counter() + gpu_warm_s
-    while time.perf_counter() < t_warm:
-        md.update(th_last, slot=0, want_xprod=False)
     start_t = dict(start, sigmaSqStarting=th_last[0], tauSqStarting=th_last[1],
                    phiStarting=th_last[2],
                    betaStarting=w["p.beta.samples"][:, -1])
