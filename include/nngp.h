@@ -389,6 +389,15 @@ int nngp_model_rNNGP(nngp_model *model, const double *sigmaSqIG,
 int nngp_model_profile(nngp_model *model, int enable);
 int nngp_model_timing(nngp_model *model, double *t);
 
+/* Host wall time of the MH samplers' iteration loops since timing was
+ * enabled: t[0] = ms, t[1] = iterations.  Each loop is measured from its
+ * first proposal to its last accept/reject decision, with the model's
+ * stream drained at both ends; the call's set-up (the initial pass at the
+ * starting theta) and the copy of the samples out are outside.  This is the
+ * per-iteration rate of a running chain, which the reference reports only
+ * as the whole call's run.time (R/spNNGP.R:395,437). */
+int nngp_model_loop_timing(nngp_model *model, double *t);
+
 /* Launch shapes of the updateBF pass timed by the autotuner (at most
  * NNGP_TUNE_MAX reported). */
 #define NNGP_TUNE_MAX 12

@@ -39,6 +39,7 @@ EXPORTS = (
     "nngp_model_create", "nngp_model_destroy", "nngp_model_update",
     "nngp_model_qresid", "nngp_model_get_bf", "nngp_model_rNNGP",
     "nngp_prior_draw", "nngp_model_profile", "nngp_model_timing",
+    "nngp_model_loop_timing",
     "nngp_server_create", "nngp_server_destroy", "nngp_server_submit",
     "nngp_server_wait", "nngp_server_release", "nngp_server_predict",
     "nngp_server_get_stats",
@@ -199,6 +200,7 @@ def load():
            _dp, C.POINTER(_i)])
     L.nngp_model_profile.argtypes = [_vp, _i]
     L.nngp_model_timing.argtypes = [_vp, _dp]
+    L.nngp_model_loop_timing.argtypes = [_vp, _dp]
     L.nngp_server_create.argtypes = [C.POINTER(_vp), _dp, _dp, _dp, _i, _i,
                                      _i, _dp, _dp, _i, _i,
                                      C.POINTER(NngpServerOpts)]

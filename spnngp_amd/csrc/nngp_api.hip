@@ -129,6 +129,11 @@ static int mcmc_loop(Model &md, const McmcSpec &S, Rng &R,
   if (S.nRep > 0) NNGP_TRY(rep.init(n, S.nnIndx, S.nnIndxLU));
   bool thetaUpdate = true;
   int accept = 0, batchAccept = 0, status = 0, repCnt = 0;
+  std::chrono::steady_clock::time_point loop_t0;
+  if (md.prof) {
+    NNGP_HIP(hipStreamSynchronize(md.st.s));
+    loop_t0 = std::chrono::steady_clock::now();
+  }
 
   for (int s = 0; s < S.nSamples; s++) {
     if (thetaUpdate) {
@@ -219,6 +224,13 @@ static int mcmc_loop(Model &md, const McmcSpec &S, Rng &R,
     if (S.nAccept) *S.nAccept = accept;
     if (opts && opts->interrupt && opts->interrupt(opts->interrupt_arg))
       return set_err(NNGP_ERR_INTERRUPT, "interrupted after %d samples", s + 1);
+  }
+  if (md.prof) {
+    NNGP_HIP(hipStreamSynchronize(md.st.s));
+    md.loop_ms += std::chrono::duration<double, std::milli>(
+                      std::chrono::steady_clock::now() - loop_t0)
+                      .count();
+    md.loop_iters += S.nSamples;
   }
   if (S.verbose) {
     printf("Sampled: %i of %i, %3.2f%%\n", S.nSamples, S.nSamples, 100.0);
@@ -1060,6 +1072,16 @@ int nngp_model_profile(nngp_model *model, int enable) {
   md.nev = 0;
   md.tms[0] = md.tms[1] = 0.0;
   md.tcount[0] = md.tcount[1] = 0;
+  md.loop_ms = 0.0;
+  md.loop_iters = 0;
+  return NNGP_OK;
+}
+
+int nngp_model_loop_timing(nngp_model *model, double *t) {
+  clear_err();
+  if (!model || !t) return set_err(NNGP_ERR_ARG, "bad arguments");
+  t[0] = model->md.loop_ms;
+  t[1] = (double)model->md.loop_iters;
   return NNGP_OK;
 }
 

@@ -331,6 +331,11 @@ struct Model {
   int nev = 0;
   double tms[2] = {0.0, 0.0};
   long long tcount[2] = {0, 0};
+  // host wall time of the samplers' iteration loops while timing is on:
+  // from the first proposal (initial pass done, stream drained) to the last
+  // decision (stream drained); the initial pass is outside
+  double loop_ms = 0.0;
+  long long loop_iters = 0;
   int ev_begin(int kind);
   void ev_end(int idx);
   int collect();

@@ -467,11 +467,17 @@ class Model:
         check(load().nngp_model_profile(self._h, int(enable)))
 
     def timing(self):
-        """{'bf_ms', 'bf_launches', 'sum_ms', 'sum_launches'} so far."""
+        """{'bf_ms', 'bf_launches', 'sum_ms', 'sum_launches', 'loop_ms',
+        'loop_iters'} so far: kernel times by HIP events, and the host wall
+        time of the samplers' iteration loops (first proposal to last
+        decision, stream drained at both ends; the initial pass outside)."""
         t = np.zeros(4)
         check(load().nngp_model_timing(self._h, t))
+        lt = np.zeros(2)
+        check(load().nngp_model_loop_timing(self._h, lt))
         return {"bf_ms": t[0], "bf_launches": int(t[1]), "sum_ms": t[2],
-                "sum_launches": int(t[3])}
+                "sum_launches": int(t[3]), "loop_ms": lt[0],
+                "loop_iters": int(lt[1])}
 
     def autotune(self, theta):
         """Time the updateBF launch shapes at theta and keep the fastest.

@@ -6,6 +6,8 @@ and conjugate outputs within 1e-10 relative (FP64 path); MCMC chains with
 the same host random numbers accept and reject identically, samples within
 1e-9 relative.
 """
+import time
+
 import numpy as np
 import pytest
 
@@ -750,6 +752,39 @@ def test_mixed_chain_matches_fp64_chain(oracle, nngp):
         a, b = res[0][key], res[1][key]
         se = a.std(1) / np.sqrt(S)
         assert np.all(np.abs(a.mean(1) - b.mean(1)) < 0.01 * se + 1e-9)
+
+
+@pytest.mark.gpu
+def test_loop_timing_covers_the_iterations_only(oracle, nngp):
+    """The loop clock counts each sampler call's iterations, not its initial
+    pass; the event counters count every launch; enabling resets both."""
+    print("loop timing", flush=True)
+    n, m, p, S = 2000, 10, 2, 40
+    c, X, y = make_response_data(n, p, 31)
+    nn = oracle.mkNNIndxCB(c, m)
+    args = dict(sigmaSqIG=[2.0, 5.0], tauSqIG=[2.0, 1.0], phiUnif=[3.0, 30.0],
+                nuUnif=None, betaStarting=np.zeros(p), sigmaSqStarting=5.0,
+                tauSqStarting=1.0, phiStarting=6.0, nuStarting=0.0,
+                sigmaSqTuning=0.3, tauSqTuning=0.3, phiTuning=0.3,
+                nuTuning=0.0, nSamples=S, seed=4)
+    with nngp.Model(c, nn["nnIndx"], nn["nnIndxLU"], m, 0, X, y) as md:
+        md.rNNGP(**args)  # untimed
+        md.profile(True)
+        t0 = time.perf_counter()
+        md.rNNGP(**args)
+        md.rNNGP(**args)
+        wall_ms = (time.perf_counter() - t0) * 1e3
+        tm = md.timing()
+        assert tm["loop_iters"] == 2 * S
+        assert tm["bf_launches"] == 2 * (S + 1)  # the initial pass each call
+        assert tm["sum_launches"] == 2 * (S + 1)
+        assert 0 < tm["loop_ms"] < wall_ms
+        # every iteration holds its updateBF and sum kernels
+        assert tm["loop_ms"] > 2 * S * tm["bf_ms"] / tm["bf_launches"]
+        md.profile(True)
+        tm = md.timing()
+        assert tm["loop_iters"] == 0 and tm["loop_ms"] == 0.0
+        assert tm["bf_launches"] == 0
 
 
 # ---------------------------------------------------------------------------
