@@ -27,6 +27,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MCMC iter/s @ n=1e6,m=15 (FP64 vs mixed) + kriging p50/p99 latency"
 PEAK_FP64_TFLOPS = 78.6
+# The chains time every 4th updateBF launch (and its sum launch) with HIP
+# events: timing each one adds two event packets per kernel to the stream and
+# slows the chain by about 5%; one in four costs about 1%.
+PROF_EVERY = 4
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of the dominant kernels, from two rocprofv3 --pmc
 # passes (FETCH_SIZE, WRITE_SIZE) of this script, each scaled by what the
@@ -126,7 +130,7 @@ def mcmc_bench(c, X, y, idx, m, steps, warmup, start, precision):
                    phiStarting=th_last[2],
                    betaStarting=w["p.beta.samples"][:, -1])
     cand, chosen = md.autotune(th_last)
-    md.profile(True)
+    md.profile(True, every=PROF_EVERY)
     md.timing()  # drains the stream: nothing in flight when the clock starts
     t0 = time.perf_counter()
     out = md.rNNGP(nSamples=steps, seed=12, **start_t)

@@ -382,10 +382,13 @@ int nngp_model_rNNGP(nngp_model *model, const double *sigmaSqIG,
 
 /* Kernel timing on the model's stream (HIP events around each launch), for
  * the run-time breakdown the reference reports as run.time
- * (R/spNNGP.R:395,437).  enable != 0 starts (and zeroes) the counters.
+ * (R/spNNGP.R:395,437).  enable != 0 starts (and zeroes) the counters;
+ * enable = k > 1 times only every k-th updateBF launch and the sum launch
+ * after it (each event is a packet on the stream: timing every launch slows
+ * a chain of short passes by about 5%), 0 stops them.
  * nngp_model_timing fills t[0] = total updateBF kernel ms, t[1] = number of
- * updateBF launches, t[2] = total ms of the sum kernels, t[3] = their
- * launches. */
+ * timed updateBF launches, t[2] = total ms of the timed sum kernels, t[3] =
+ * their launches. */
 int nngp_model_profile(nngp_model *model, int enable);
 int nngp_model_timing(nngp_model *model, double *t);
 

@@ -1069,6 +1069,9 @@ int nngp_model_profile(nngp_model *model, int enable) {
   if (!model) return set_err(NNGP_ERR_ARG, "model is NULL");
   Model &md = model->md;
   md.prof = enable != 0;
+  md.prof_every = enable > 1 ? enable : 1;
+  md.prof_seq = 0;
+  md.prof_sample = false;
   md.nev = 0;
   md.tms[0] = md.tms[1] = 0.0;
   md.tcount[0] = md.tcount[1] = 0;

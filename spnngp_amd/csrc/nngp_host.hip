@@ -341,7 +341,8 @@ int Model::bf(const CovArgs &c, int slot, bool storeB) {
   if (have_tuned && bf_config_ok(tuned_cfg, m, a.variant)) cfg = tuned_cfg;
   last_bf = a;
   last_cfg = cfg;
-  const int ev = prof ? ev_begin(0) : -1;
+  prof_sample = prof && prof_seq++ % prof_every == prof_every - 1;
+  const int ev = prof_sample ? ev_begin(0) : -1;
   const int rc = bf_launch(a, cfg, 1, st.s);
   if (ev >= 0) ev_end(ev);
   if (rc == -1)
@@ -403,7 +404,7 @@ int Model::sums_launch(int slot, int off) {
   double *base = sums_mapped ? sums_mapped : sums.as<double>();
   double *out = base + off;
   double *err_out = base + 2 * kStatsMax + (off ? 1 : 0);
-  const int ev = prof ? ev_begin(1) : -1;
+  const int ev = prof_sample ? ev_begin(1) : -1;
   (void)slot;
   SumsSignal sig;
   if (sums_mapped) {

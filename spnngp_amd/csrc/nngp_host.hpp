@@ -321,12 +321,18 @@ struct Model {
   void unpack(const double *raw, PassOut *out) const;
 
   // optional kernel timing with HIP events on st (kind 0: updateBF,
-  // kind 1: sum kernels); collect() after each stream synchronization
+  // kind 1: sum kernels); collect() after each stream synchronization.
+  // Every prof_every-th updateBF launch and the sum launch after it are
+  // timed: each event is a packet on the stream, and timing every launch
+  // slows a chain of short passes by about 5%.
   struct EvPair {
     hipEvent_t a = nullptr, b = nullptr;
     int kind = 0;
   };
   bool prof = false;
+  int prof_every = 1;
+  long long prof_seq = 0;
+  bool prof_sample = false;  // the last updateBF launch was timed
   std::vector<EvPair> evs;
   int nev = 0;
   double tms[2] = {0.0, 0.0};

@@ -462,9 +462,12 @@ class Model:
             res["ytViy"] = out.ytViy
         return res
 
-    def profile(self, enable=True):
-        """Start (and zero) or stop per-kernel event timing."""
-        check(load().nngp_model_profile(self._h, int(enable)))
+    def profile(self, enable=True, every=1):
+        """Start (and zero) or stop per-kernel event timing and the loop
+        clock.  every=k > 1 times every k-th updateBF launch (and the sum
+        launch after it) instead of each one."""
+        check(load().nngp_model_profile(self._h,
+                                        max(int(every), 1) if enable else 0))
 
     def timing(self):
         """{'bf_ms', 'bf_launches', 'sum_ms', 'sum_launches', 'loop_ms',

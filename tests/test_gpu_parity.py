@@ -785,6 +785,15 @@ def test_loop_timing_covers_the_iterations_only(oracle, nngp):
         tm = md.timing()
         assert tm["loop_iters"] == 0 and tm["loop_ms"] == 0.0
         assert tm["bf_launches"] == 0
+        # sampled: every 5th updateBF launch and the sum launch after it
+        md.profile(True, every=5)
+        md.rNNGP(**args)
+        md.rNNGP(**args)
+        tm = md.timing()
+        assert tm["loop_iters"] == 2 * S
+        assert tm["bf_launches"] == 2 * (S + 1) // 5
+        assert tm["sum_launches"] == tm["bf_launches"]
+        assert tm["bf_ms"] > 0 and tm["sum_ms"] > 0
 
 
 # ---------------------------------------------------------------------------
