@@ -40,20 +40,23 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_per_launch.md")
 
 
 def traffic(key, kernel):
-    """(bytes, note) of the summary row `key` when it measured `kernel`
-    (the launch this run used), else (None, why)."""
+    """(bytes, note) of the summary row `key` that measured `kernel` (the
+    launch this run used), else (None, why)."""
+    seen = []
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
                 cells = [x.strip() for x in line.strip().strip("|").split("|")]
                 if len(cells) >= 4 and cells[0] == key:
-                    if cells[1].strip("`") != kernel:
-                        return None, (f"summary row {key} measured "
-                                      f"{cells[1]}, this run used {kernel}")
-                    return int(cells[3].replace(",", "")), \
-                        f"profiles/hbm_traffic_per_launch.md row {key}"
+                    if cells[1].strip("`") == kernel:
+                        return int(cells[3].replace(",", "")), \
+                            f"profiles/hbm_traffic_per_launch.md row {key}"
+                    seen.append(cells[1])
     except OSError:
         return None, "no traffic summary"
+    if seen:
+        return None, (f"summary rows {key} measured {', '.join(seen)}; "
+                      f"this run used {kernel}")
     return None, f"no summary row {key}"
 
 
