@@ -323,12 +323,13 @@ __device__ __forceinline__ void lane_site(const BfArgs &A, const BfProblem &P,
   double v[M];
 #pragma unroll
   for (int k = 0; k < M; k++) {
-    v[k] = lane_corr<COV>(kBase2 ? fma(px[k], px[k], py[k] * py[k])
+    v[k] = lane_corr<COV>(kBase2 ? lane_d2<COV>(px[k], py[k])
                                  : sqdist(s.x, s.y, px[k], py[k]),
                           c);
 #pragma unroll
     for (int l = 0; l < k; l++)
-      a[tri(k, l)] = lane_corr<COV>(sqdist(px[k], py[k], px[l], py[l]), c);
+      a[tri(k, l)] = lane_corr<COV>(
+          lane_d2<COV>(px[k] - px[l], py[k] - py[l]), c);
     a[tri(k, k)] = c.diag;
   }
   double vv;

@@ -507,20 +507,49 @@ constexpr bool lane_base2() {
   return COV == 0 || COV == 3 || COV == kCovMatern15 || COV == kCovMatern25;
 }
 
-// Covariance entry of the per-lane kernels from a squared distance in those
-// units (scaled by tsc where lane_base2<COV>(), as given otherwise).
+// Covariances of the per-lane kernels that take the square root of the
+// squared distance: the exponential and the closed-form materns.
+template <int COV>
+constexpr bool lane_sqrt() {
+  return COV == 0 || COV == kCovMatern15 || COV == kCovMatern25;
+}
+
+// Squared distance of an offset (dx, dy) for lane_corr.  Where lane_corr
+// takes the square root, the floor 1e-300 that keeps rsq finite at zero
+// distance (duplicate sites) enters as the addend of the first multiply-add
+// instead of as a max before rsq: one instruction fewer on each entry's
+// dependency chain.  The floor changes no squared distance above 1e-284,
+// and a zero distance still gives the factor 2^-1e-150, which rounds to the
+// factor at 0.
+template <int COV>
+__device__ __forceinline__ double lane_d2(double dx, double dy) {
+  if constexpr (lane_sqrt<COV>()) return fma(dx, dx, fma(dy, dy, 1e-300));
+  else return fma(dx, dx, dy * dy);
+}
+
+// nn_sqrt for s >= 1e-300, as lane_d2 gives it: no floor of its own.
+__device__ __forceinline__ double nn_sqrt_floored(double s) {
+  double y = __builtin_amdgcn_rsq(s);
+  double d = s * y;
+  double r = fma(-d, d, s);
+  return fma(r * y, 0.5, d);
+}
+
+// Covariance entry of the per-lane kernels from a squared distance from
+// lane_d2 (of offsets scaled by tsc where lane_base2<COV>(), as given
+// otherwise).
 template <int COV>
 __device__ __forceinline__ double lane_corr(double d2, const CovArgs &c) {
   constexpr double kLn2 = 0.6931471805599453;  // phi d = d' ln 2
   if constexpr (COV == 0) {
-    return s2_exp2_finite(-nn_sqrt(d2), c);
+    return s2_exp2_finite(-nn_sqrt_floored(d2), c);
   } else if constexpr (COV == 3) {
     return s2_exp2_finite(-d2, c);
   } else if constexpr (COV == kCovMatern15) {
-    const double dd = nn_sqrt(d2);
+    const double dd = nn_sqrt_floored(d2);
     return s2_exp2_finite(-dd, c) * (1.0 + dd * kLn2);
   } else if constexpr (COV == kCovMatern25) {
-    const double dd = nn_sqrt(d2);
+    const double dd = nn_sqrt_floored(d2);
     const double x = dd * kLn2;
     return s2_exp2_finite(-dd, c) * (1.0 + x + x * x * (1.0 / 3.0));
   } else {

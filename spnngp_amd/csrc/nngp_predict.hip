@@ -348,13 +348,14 @@ __global__ __launch_bounds__(kPredBlock) void k_conj_pred_lane(ConjPredArgs A) {
   double a[T], v[M], c0[M];
 #pragma unroll
   for (int k = 0; k < M; k++) {
-    c0[k] = lane_corr<COV>(kBase2 ? fma(px[k], px[k], py[k] * py[k])
+    c0[k] = lane_corr<COV>(kBase2 ? lane_d2<COV>(px[k], py[k])
                                   : sqdist(s0.x, s0.y, px[k], py[k]),
                            c);
     v[k] = c0[k];
 #pragma unroll
     for (int l = 0; l < k; l++)
-      a[tri(k, l)] = lane_corr<COV>(sqdist(px[k], py[k], px[l], py[l]), c);
+      a[tri(k, l)] = lane_corr<COV>(
+          lane_d2<COV>(px[k] - px[l], py[k] - py[l]), c);
     a[tri(k, k)] = c.diag;
   }
   double vv;
