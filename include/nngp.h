@@ -412,7 +412,9 @@ typedef struct nngp_tune_report {
   int threads[NNGP_TUNE_MAX];     /* threads per block                    */
   int mode[NNGP_TUNE_MAX];        /* family 0, NNGP_MIXED: 1 = the
                                      covariance copy kept in LDS for the
-                                     refinement residual; family 0, FP64:
+                                     refinement residual, 3 = kept in
+                                     registers (one wave per SIMD), 0 =
+                                     re-evaluated; family 0, FP64:
                                      2 = the covariate rows of each site's
                                      neighbors staged in LDS during the
                                      factorization (same results bit for

@@ -641,6 +641,176 @@ __global__ __launch_bounds__(kLaneMaxThreads, 2) void k_bf_mixed(BfArgs A) {
   if (A.rows) block_site_terms(A.nz, A.rows, A.rows_cap, valid, Fi, e, b);
 }
 
+// k_bf_mixed's scheme at one wave per SIMD, where the FP32 factor and a
+// register copy of C both fit: no entry is evaluated twice and C does not
+// go through LDS.  The arithmetic is k_bf_mixed's step for step, so B and F
+// equal its results bit for bit.  As in lane_site, ZH > 0 stages the
+// covariate rows of the M neighbor slots and of the site in LDS with the
+// coordinate gathers (chunk h of slot k at stg[(k * ZH + h) * 64 + lane]).
+template <int M, int COV, int ZH>
+__device__ __forceinline__ void mixed_reg_site(const BfArgs &A, int i,
+                                               int cnt, int off,
+                                               int (&nb)[M], double &Fi,
+                                               double (&e)[kMaxNZ],
+                                               double2 *stg) {
+  const CovArgs &c = A.c;
+  const CovArgs32 c32 = cov32(c);
+  if (cnt == 0) {  // site 0: F = s2 + t2, residual = the data
+    Fi = c.diag;
+#pragma unroll
+    for (int j = 0; j < kMaxNZ; j++)
+      if (j < A.nz) e[j] = A.Z[(size_t)i * A.zs + j];
+    return;
+  }
+  if (cnt < M) {
+#pragma unroll
+    for (int k = 0; k < M; k++)
+      if (k >= cnt) nb[k] = 0;
+  }
+  if constexpr (ZH > 0) {
+#pragma unroll
+    for (int k = 0; k <= M; k++) {
+      const double *zr = A.Z + (size_t)(k < M ? nb[k] : i) * (2 * ZH);
+#pragma unroll
+      for (int h = 0; h < ZH; h++)
+        lds_load16(zr + 2 * h, stg + (k * ZH + h) * 64);
+    }
+  }
+  const double2 s = A.xy[i];
+  float dx[M], dy[M];
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const double2 q = A.xy[nb[k]];
+    dx[k] = (float)(q.x - s.x);
+    dy[k] = (float)(q.y - s.y);
+  }
+  if (cnt < M) {  // far away and far apart: correlations exactly 0
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+      if (k >= cnt) {
+        dx[k] = 1e30f * (k + 1);
+        dy[k] = -1e30f * (k + 1);
+      }
+    }
+  }
+  constexpr int T = M * (M + 1) / 2;
+  float a[T], ac[T], v[M], cv[M];
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    v[k] = s2_corr32<COV>(fmaf(dx[k], dx[k], dy[k] * dy[k]), c32);
+    cv[k] = v[k];
+#pragma unroll
+    for (int l = 0; l < k; l++) {
+      a[tri(k, l)] = pair32<COV>(dx[k], dy[k], dx[l], dy[l], c32);
+      ac[tri(k, l)] = a[tri(k, l)];
+    }
+    a[tri(k, k)] = c32.diag;
+  }
+  const bool ok = chol_factor32<M>(a);
+  chol_apply32<M>(a, v);  // v <- B0
+  // residual r = c - C B0 in FP64
+  double r[M];
+  const double dg = (double)c32.diag;
+#pragma unroll
+  for (int k = 0; k < M; k++) r[k] = fma(-dg, (double)v[k], (double)cv[k]);
+#pragma unroll
+  for (int k = 1; k < M; k++) {
+#pragma unroll
+    for (int l = 0; l < k; l++) {
+      const double ckl = (double)ac[tri(k, l)];
+      r[k] = fma(-ckl, (double)v[l], r[k]);
+      r[l] = fma(-ckl, (double)v[k], r[l]);
+    }
+  }
+  float d[M];
+#pragma unroll
+  for (int k = 0; k < M; k++) d[k] = (float)r[k];
+  chol_apply32<M>(a, d);  // d <- C^-1 r
+  double bd[M];
+  double cb = 0.0;
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    bd[k] = (double)v[k] + (double)d[k];
+    cb = fma((double)cv[k], bd[k], cb);
+  }
+  if (A.B) {
+#pragma unroll
+    for (int k = 0; k < M; k++)
+      if (k < cnt) A.B[off + k] = bd[k];
+  }
+  Fi = c.diag - cb;
+  // residual vectors; padded coefficients are 0
+  if constexpr (ZH > 0) {
+    const int t = (int)threadIdx.x & 63;
+#pragma unroll
+    for (int h = 0; h < ZH; h++) {
+      const double2 zi = stg[(M * ZH + h) * 64 + t];
+      double e0 = zi.x, e1 = zi.y;
+#pragma unroll
+      for (int k = 0; k < M; k++) {
+        const double2 z = stg[(k * ZH + h) * 64 + t];
+        e0 = fma(-bd[k], z.x, e0);
+        e1 = fma(-bd[k], z.y, e1);
+      }
+      e[2 * h] = e0;
+      e[2 * h + 1] = e1;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < kMaxNZ; j++) {
+      if (j < A.nz) {
+        double ej = A.Z[(size_t)i * A.zs + j];
+#pragma unroll
+        for (int k = 0; k < M; k++)
+          ej = fma(-bd[k], A.Z[(size_t)nb[k] * A.zs + j], ej);
+        e[j] = ej;
+      }
+    }
+  }
+  if (!ok || !(Fi > 0.0)) atomicMin(A.err, i);
+}
+
+// Dynamic LDS of k_bf_mixed_reg a wavefront, in double2: the staged rows,
+// which the block sums then reuse, or the sums' area alone.
+__host__ __device__ inline int mixed_reg_wave_chunks(int M, int ZH, int nz) {
+  const int sums = (site_terms(nz) * 64 + 1) / 2;
+  const int stage = (M + 1) * ZH * 64;
+  return stage > sums ? stage : sums;
+}
+
+template <int M, int COV, int ZH>
+__global__ __launch_bounds__(kLaneMaxThreads) void k_bf_mixed_reg(BfArgs A) {
+  extern __shared__ double2 mstage[];
+  const int nt = (int)blockDim.x;
+  const int b = xcd_block((A.n + nt - 1) / nt);
+  if (b < 0) return;  // whole padding block
+  const int i = b * nt + (int)threadIdx.x;
+  const bool valid = i < A.n;
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  double2 *stg = mstage + (size_t)w * mixed_reg_wave_chunks(M, ZH, A.nz);
+  double Fi = 1.0;
+  double e[kMaxNZ];
+#pragma unroll
+  for (int j = 0; j < kMaxNZ; j++) e[j] = 0.0;
+  if (valid) {
+    int cnt, off;
+    if (A.lu_closed) {
+      nn_closed(i, A.m, off, cnt);
+    } else {
+      cnt = A.lu[A.n + i];
+      off = A.lu[i];
+    }
+    int nb[M];
+#pragma unroll
+    for (int k = 0; k < M; k++) nb[k] = A.nnIndx[off + k];
+    mixed_reg_site<M, COV, ZH>(A, i, cnt, off, nb, Fi, e, stg);
+    if (A.F) A.F[i] = Fi;
+  }
+  if (A.rows)
+    block_site_terms_lds(A.nz, A.rows, A.rows_cap, valid, Fi, e, b,
+                         reinterpret_cast<double *>(stg));
+}
+
 // ---------------------------------------------------------------------------
 // one site per wavefront (generic m, generic covariance)
 // ---------------------------------------------------------------------------
@@ -1285,7 +1455,7 @@ bool bf_config_ok(const BfConfig &cfg, int m, int variant) {
   if (cfg.family == 0)
     return bf_lane_ok(m, variant) &&
            (cfg.threads == 64 || cfg.threads == 128 || cfg.threads == 256) &&
-           (cfg.mode == 0 || cfg.mode == 2 ||
+           (cfg.mode == 0 || cfg.mode == 2 || cfg.mode == 3 ||
             (cfg.mode == 1 && mixed_copy_shm(m, cfg.threads) <= 65536));
   if (cfg.family == 2)
     return m <= kGrpMaxM && cfg.mode == 0 &&
@@ -1422,26 +1592,71 @@ static void launch_mixed_mc(const BfArgs &a, int variant, const dim3 &grid,
   }
 }
 
+// Rows of Z that k_bf_mixed_reg stages in LDS: ZH chunks of 16 bytes a
+// row (zs = 2 ZH, Z 16-byte aligned, m > 8 as for the FP64 staging, and the
+// block's stage within 64 KiB), else 0.
+static int mixed_reg_zh(const BfArgs &a, int threads) {
+  if (!a.Z || a.nz <= 0 || a.m <= 8 || ((uintptr_t)a.Z & 15) != 0) return 0;
+  const int zh = a.zs == 2 ? 1 : a.zs == 4 ? 2 : 0;
+  const size_t stage = (size_t)(threads / 64) *
+                       mixed_reg_wave_chunks(lane_M(a.m), zh, a.nz) *
+                       sizeof(double2);
+  return stage <= 65536 ? zh : 0;
+}
+
+static size_t mixed_reg_shm(const BfArgs &a, int threads) {
+  return (size_t)(threads / 64) *
+         mixed_reg_wave_chunks(lane_M(a.m), mixed_reg_zh(a, threads), a.nz) *
+         sizeof(double2);
+}
+
+template <int M, int ZH>
+static void launch_mixed_reg_mh(const BfArgs &a, int variant, const dim3 &grid,
+                                const dim3 &blk, size_t shm, hipStream_t st) {
+  switch (variant) {
+    case 0: hipLaunchKernelGGL((k_bf_mixed_reg<M, 0, ZH>), grid, blk, shm, st, a); break;
+    case 1: hipLaunchKernelGGL((k_bf_mixed_reg<M, 1, ZH>), grid, blk, shm, st, a); break;
+    case kCovMatern15: hipLaunchKernelGGL((k_bf_mixed_reg<M, kCovMatern15, ZH>), grid, blk, shm, st, a); break;
+    case kCovMatern25: hipLaunchKernelGGL((k_bf_mixed_reg<M, kCovMatern25, ZH>), grid, blk, shm, st, a); break;
+    default: hipLaunchKernelGGL((k_bf_mixed_reg<M, 3, ZH>), grid, blk, shm, st, a); break;
+  }
+}
+
 template <int M>
-static void launch_mixed_m(const BfArgs &a, int variant, bool copy,
+static void launch_mixed_reg(const BfArgs &a, int variant, const dim3 &grid,
+                             const dim3 &blk, hipStream_t st) {
+  const size_t shm = mixed_reg_shm(a, (int)blk.x);
+  switch (mixed_reg_zh(a, (int)blk.x)) {
+    case 1: launch_mixed_reg_mh<M, 1>(a, variant, grid, blk, shm, st); break;
+    case 2: launch_mixed_reg_mh<M, 2>(a, variant, grid, blk, shm, st); break;
+    default: launch_mixed_reg_mh<M, 0>(a, variant, grid, blk, shm, st); break;
+  }
+}
+
+// mode 1: the covariance copy in LDS; 3: in registers (one wave per SIMD,
+// covariate rows staged in LDS where they can be); else re-evaluated
+template <int M>
+static void launch_mixed_m(const BfArgs &a, int variant, int mode,
                            const dim3 &grid, const dim3 &blk, hipStream_t st) {
-  if (copy)
+  if (mode == 1)
     launch_mixed_mc<M, true>(a, variant, grid, blk,
                              mixed_copy_shm(a.m, (int)blk.x), st);
+  else if (mode == 3)
+    launch_mixed_reg<M>(a, variant, grid, blk, st);
   else
     launch_mixed_mc<M, false>(a, variant, grid, blk,
                               mixed_offsets_shm(a.m, (int)blk.x), st);
 }
 
-static void launch_mixed(const BfArgs &a, int variant, bool copy,
+static void launch_mixed(const BfArgs &a, int variant, int mode,
                          const dim3 &grid, const dim3 &blk, hipStream_t st) {
   switch (lane_M(a.m)) {
-    case 4: launch_mixed_m<4>(a, variant, copy, grid, blk, st); break;
-    case 8: launch_mixed_m<8>(a, variant, copy, grid, blk, st); break;
-    case 10: launch_mixed_m<10>(a, variant, copy, grid, blk, st); break;
-    case 12: launch_mixed_m<12>(a, variant, copy, grid, blk, st); break;
-    case 15: launch_mixed_m<15>(a, variant, copy, grid, blk, st); break;
-    default: launch_mixed_m<16>(a, variant, copy, grid, blk, st); break;
+    case 4: launch_mixed_m<4>(a, variant, mode, grid, blk, st); break;
+    case 8: launch_mixed_m<8>(a, variant, mode, grid, blk, st); break;
+    case 10: launch_mixed_m<10>(a, variant, mode, grid, blk, st); break;
+    case 12: launch_mixed_m<12>(a, variant, mode, grid, blk, st); break;
+    case 15: launch_mixed_m<15>(a, variant, mode, grid, blk, st); break;
+    default: launch_mixed_m<16>(a, variant, mode, grid, blk, st); break;
   }
 }
 
@@ -1516,7 +1731,7 @@ int bf_launch(const BfArgs &a0, const BfConfig &cfg, int nprob,
       else
         launch_lane<true>(a, variant, grid, blk, st);
     }
-    else if (a.mixed) launch_mixed(a, variant, cfg.mode == 1, grid, blk, st);
+    else if (a.mixed) launch_mixed(a, variant, cfg.mode, grid, blk, st);
     else if (cfg.mode == 2 && lane_lds_applies(a, cfg.threads))
       launch_lane_lds(a, variant, grid, blk, st);
     else launch_lane<false>(a, variant, grid, blk, st);
@@ -1577,9 +1792,9 @@ int bf_autotune(const BfArgs &a, int nprob, hipStream_t st, BfConfig *best,
   }
   std::vector<BfConfig> cand;
   if (a.mixed) {  // the per-lane mixed kernel: block size x covariance
-                  // copy in LDS; then the wavefront one
+                  // copy (none, LDS, registers); then the wavefront one
     for (int th : {64, 128, 256})
-      for (int mode : {0, 1}) cand.push_back(BfConfig{0, th, mode});
+      for (int mode : {0, 1, 3}) cand.push_back(BfConfig{0, th, mode});
   } else {  // the per-lane kernel: block size x covariate rows in LDS
     for (int th : {64, 128, 256}) {
       cand.push_back(BfConfig{0, th, 0});

@@ -359,7 +359,7 @@ int Model::pin_shape(int family, int threads, int mode) {
   const int variant = cov == 2 ? kCovMatern15 : cov;
   const bool mixed = precision == NNGP_MIXED;
   if (!bf_config_ok(cfg, m, variant) ||
-      (family == 0 && (mixed ? mode == 2 : mode == 1)) ||
+      (family == 0 && (mixed ? mode == 2 : (mode == 1 || mode == 3))) ||
       (family == 2 && mixed))  // the row kernel is FP64 only
     return set_err(NNGP_ERR_ARG,
                    "launch shape (%d, %d, %d) does not apply to this model",

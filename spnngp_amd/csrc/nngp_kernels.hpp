@@ -69,8 +69,8 @@ struct BfConfig {
   int family = 0;
   int threads = 256;
   int mode = 0;          // mixed-precision passes: 1 = covariance copy in
-                         // LDS; FP64 per-lane passes: 2 = covariate rows
-                         // staged in LDS
+                         // LDS, 3 = in registers; FP64 per-lane passes:
+                         // 2 = covariate rows staged in LDS
 };
 
 struct BfTuneReport {

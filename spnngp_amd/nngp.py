@@ -105,12 +105,12 @@ def nn2(coords, coords0, k: int):
 # ---------------------------------------------------------------------------
 def _shape_name(family, mode):
     """Name of an updateBF launch shape of the autotuner: 'lane' (one site
-    per lane; '+lds-copy' for the mixed form's covariance copy in LDS,
-    '+lds-z' for the FP64 form's covariate rows staged in LDS), 'wave'
-    (one site per wavefront) or 'row' (one site per 16-lane row, four per
-    wavefront)."""
+    per lane; '+lds-copy' and '+reg-copy' for the mixed form's covariance
+    copy in LDS or in registers, '+lds-z' for the FP64 form's covariate rows
+    staged in LDS), 'wave' (one site per wavefront) or 'row' (one site per
+    16-lane row, four per wavefront)."""
     if family == 0:
-        return ("lane", "lane+lds-copy", "lane+lds-z")[mode]
+        return ("lane", "lane+lds-copy", "lane+lds-z", "lane+reg-copy")[mode]
     return "row" if family == 2 else "wave"
 
 
@@ -497,8 +497,9 @@ class Model:
         """Pin the updateBF launch shape: family 0 (one site per lane), 1
         (per wavefront) or 2 (per 16-lane row: four sites a wavefront, m <=
         32, FP64), threads per block, and mode as in the autotuner's
-        report (1: the mixed per-lane kernel's covariance copy in LDS; 2: the
-        FP64 per-lane kernel's covariate rows staged in LDS)."""
+        report (1 and 3: the mixed per-lane kernel's covariance copy in LDS
+        or in registers; 2: the FP64 per-lane kernel's covariate rows staged
+        in LDS)."""
         check(load().nngp_model_set_shape(self._h, int(family), int(threads),
                                           int(mode)))
 

@@ -673,6 +673,55 @@ def test_mixed_updateBF_close_to_oracle(oracle, nngp, cov, m):
     assert abs(out2["logDet"] - out["logDet"]) < 1e-9 * abs(out["logDet"])
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("cov,m,p", [(0, 15, 3), (0, 15, 2), (3, 10, 1),
+                                     (1, 12, 3), (0, 8, 3), (0, 15, 0)])
+def test_mixed_lane_modes_agree(oracle, nngp, cov, m, p):
+    """The mixed per-lane kernel's three ways to keep C for the refinement
+    residual (re-evaluated, LDS copy, register copy at one wave per SIMD
+    with the covariate rows staged in LDS where they can be) run the same
+    arithmetic: B and F equal bit for bit across modes and block sizes, and
+    the sums, added in another order by the register form, agree to
+    rounding.  n = 4000 + 13 leaves a ragged last block; p = 2 has rows
+    that stage in one 16-byte chunk, p = 1 and 0 none; at 256 threads the
+    register form's stage of p = 3 rows exceeds 64 KiB and it reads them
+    from memory."""
+    print(f"mixed modes cov={cov} m={m} p={p}", flush=True)
+    n = 4000 + 13
+    c = uniform_coords(n, 60 + m + p)
+    rng = np.random.default_rng(m + p)
+    X = np.column_stack([np.ones(n), rng.normal(size=(n, p - 1))]) \
+        if p else None
+    beta = np.linspace(1.0, -1.0, p) if p else np.zeros(1)
+    y = (X @ beta if p else 0.0) + rng.normal(size=n)
+    nn = oracle.mkNNIndxCB(c, m)
+    theta = [2.0, 0.2, 12.0 if cov != 1 else 3.0]
+    B0, F0, ld0 = oracle.updateBF(c.T.reshape(-1).copy(), nn["nnIndx"],
+                                  nn["nnIndxLU"], n, m, theta, cov)
+    ref = None
+    with nngp.Model(c, nn["nnIndx"], nn["nnIndxLU"], m, cov, X, y,
+                    precision=1) as md:
+        # the LDS copy (mode 1) fits a 64-thread block at m <= 16
+        for th, mode in ((64, 0), (64, 1), (64, 3), (128, 0), (128, 3),
+                         (256, 0), (256, 3)):
+            md.set_shape(0, th, mode)
+            out = md.update(theta, slot=0, beta=beta)
+            B, F = md.get_bf(0)
+            if ref is None:
+                ref = (out, B, F)
+                continue
+            assert np.array_equal(B, ref[1]), (th, mode)
+            assert np.array_equal(F, ref[2]), (th, mode)
+            assert abs(out["logDet"] - ref[0]["logDet"]) \
+                < 1e-12 * abs(ref[0]["logDet"])
+            if p:
+                assert abs(out["qResid"] - ref[0]["qResid"]) \
+                    < 1e-12 * abs(ref[0]["qResid"])
+    assert np.max(np.abs(ref[2] - F0) / F0) < 1e-5
+    assert np.max(np.abs(ref[1] - B0)) / np.max(np.abs(B0)) < 1e-4
+    assert abs(ref[0]["logDet"] - ld0) < 1e-6 * abs(ld0)
+
+
 # The wavefront form (m > 16, or the Bessel-K matern at any m): FP32
 # covariance build, including an FP32 Bessel K (Temme / Steed to 6e-8, a
 # few ulp per entry), FP32 factor with one row per lane, one FP64 residual.
