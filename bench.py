@@ -688,10 +688,21 @@ def roofline_obj(kernel, achieved_tflops, traffic_bytes, traffic_note,
             "traffic": traffic_bytes, "traffic_note": traffic_note, **extra}
 
 
-def chain_kernel(chosen, m):
-    """Name of the chain's FP64 updateBF kernel for an autotuned shape."""
+def chain_kernel(chosen, m, mixed=False, zs=4):
+    """Name of the chain's updateBF kernel (exponential) for an autotuned
+    shape of the FP64 or the mixed form."""
     fam, threads = chosen
     M = 15 if m == 15 else m
+    if mixed:
+        if fam == "lane+reg-copy":
+            # rows staged (zs / 2 chunks) when the block's stage fits 64 KiB
+            stage = threads // 64 * (M + 1) * (zs // 2) * 64 * 16
+            zh = zs // 2 if m > 8 and stage <= 65536 else 0
+            return f"nngp::k_bf_mixed_reg<{M}, 0, {zh}> ({threads} threads)"
+        if fam in ("lane", "lane+lds-copy"):
+            copy = "true" if fam == "lane+lds-copy" else "false"
+            return f"nngp::k_bf_mixed<{M}, 0, {copy}> ({threads} threads)"
+        return f"{fam} ({threads} threads)"
     if fam == "lane+lds-z":
         return f"nngp::k_bf_lane_lds<{M}, 0, 2, false> ({threads} threads)"
     if fam == "lane":
@@ -753,8 +764,25 @@ def main():
     flops = sum(f_alg(min(i, m)) for i in range(1, min(n, m))) \
         + (n - m) * f_alg(m)
     achieved = flops / (bf_ms * 1e-3) / 1e12
+    mix_ms = mix["tm"]["bf_ms"] / max(mix["tm"]["bf_launches"], 1)
+    mix_achieved = flops / (mix_ms * 1e-3) / 1e12
     mixed = {"iters_s": mix["iters_s"], "call_iters_s": mix["call_iters_s"],
              "speedup_vs_fp64": mix["iters_s"] / iters_s,
+             "roofline": dict(roofline_obj(
+                 "updateBF (mixed per-lane kernel, m=15, exponential, %s %d "
+                 "threads)" % tuple(mix["tune"]["chosen"]), mix_achieved,
+                 *(traffic("updateBF_chain_mixed_n1e6_m15",
+                           chain_kernel(mix["tune"]["chosen"], m, True,
+                                        p + 1))
+                   if (n, m) == (1_000_000, 15) else
+                   (None, "other workload")),
+                 algorithmic_bytes=(24 + 4 * m) * n, flops_per_launch=flops,
+                 launch_ms=mix_ms, launches=mix["tm"]["bf_launches"]),
+                 bound="valu",
+                 bound_note="FP32 build and factor, FP64 refinement and "
+                            "sums; non-packed FP32 and FP64 vector FMAs "
+                            "issue at the same rate, 78.6 TFLOP/s of the "
+                            "algorithm's flops"),
              "updateBF_ms": mix["tm"]["bf_ms"] / max(mix["tm"]["bf_launches"],
                                                       1),
              "acceptance": mix["out"]["accept"] / args.steps,
