@@ -675,7 +675,8 @@ def test_mixed_updateBF_close_to_oracle(oracle, nngp, cov, m):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("cov,m,p", [(0, 15, 3), (0, 15, 2), (3, 10, 1),
-                                     (1, 12, 3), (0, 8, 3), (0, 15, 0)])
+                                     (1, 12, 3), (0, 8, 3), (0, 15, 0),
+                                     (2, 10, 3), (2, 16, 2)])
 def test_mixed_lane_modes_agree(oracle, nngp, cov, m, p):
     """The mixed per-lane kernel's three ways to keep C for the refinement
     residual (re-evaluated, LDS copy, register copy at one wave per SIMD
@@ -685,7 +686,8 @@ def test_mixed_lane_modes_agree(oracle, nngp, cov, m, p):
     rounding.  n = 4000 + 13 leaves a ragged last block; p = 2 has rows
     that stage in one 16-byte chunk, p = 1 and 0 none; at 256 threads the
     register form's stage of p = 3 rows exceeds 64 KiB and it reads them
-    from memory."""
+    from memory.  The matern cases run the closed forms at nu = 1.5 and
+    2.5."""
     print(f"mixed modes cov={cov} m={m} p={p}", flush=True)
     n = 4000 + 13
     c = uniform_coords(n, 60 + m + p)
@@ -696,6 +698,8 @@ def test_mixed_lane_modes_agree(oracle, nngp, cov, m, p):
     y = (X @ beta if p else 0.0) + rng.normal(size=n)
     nn = oracle.mkNNIndxCB(c, m)
     theta = [2.0, 0.2, 12.0 if cov != 1 else 3.0]
+    if cov == 2:
+        theta.append(1.5 if m == 10 else 2.5)
     B0, F0, ld0 = oracle.updateBF(c.T.reshape(-1).copy(), nn["nnIndx"],
                                   nn["nnIndxLU"], n, m, theta, cov)
     ref = None
