@@ -334,8 +334,9 @@ int Model::bf(const CovArgs &c, int slot, bool storeB) {
   a.mixed = precision == NNGP_MIXED ? 1 : 0;
   a.lu_closed = lu_closed ? 1 : 0;
   BfConfig cfg = bf_default(m, a.variant, !a.mixed);
-  if (a.mixed) {  // untuned mixed passes keep the covariance copy in LDS
-    const BfConfig copy{0, 64, 1};
+  if (a.mixed) {  // untuned mixed passes keep the covariance copy in
+                  // registers (the fastest per-lane mixed shape at m = 15)
+    const BfConfig copy{0, 64, 3};
     if (bf_config_ok(copy, m, a.variant)) cfg = copy;
   }
   if (have_tuned && bf_config_ok(tuned_cfg, m, a.variant)) cfg = tuned_cfg;
