@@ -185,10 +185,17 @@ extern "C" int nngp_rNNGPReplicated(
   NNGP_TRY(md.create(n, m, 0, coords, nnIndx, nnIndxLU, nullptr, nullptr,
                      covModel, opts));
   hipStream_t st = md.st.s;
-  // samples per group: per-sample device buffers within 16 GB
+  // samples per group: B, F, v, y and z of each sample
   const long long bstride = (md.nIndx + 1) & ~1LL;
   const double per = 8.0 * ((double)bstride + 4.0 * n);
-  int G = (int)std::min<double>(16e9 / per, 256.0);
+  // The draw takes about as long for one sample as for 256 (one block per
+  // CU, each walking the levels), so a group is as large as half the free
+  // device memory allows, up to 256 samples; at least 16 GB is assumed.
+  double budget = 16e9;
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+    budget = std::max(budget, 0.5 * (double)free_b);
+  int G = (int)std::min<double>(budget / per, 256.0);
   G = std::max(1, std::min(G, nSamples));
   DevBuf dB, dF, dz, dv, dy, dX, dbeta, dorder, dloff;
   NNGP_TRY(dB.alloc(sizeof(double) * (size_t)bstride * G));
