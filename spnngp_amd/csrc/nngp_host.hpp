@@ -266,8 +266,8 @@ double quad_resid(const PassOut &po, const double *beta, int p);
 // draw() uploads them, walks the neighbor DAG level by level (the level
 // schedule built once by init from the host index) and leaves v in v().
 struct RepDrawer {
-  int n = 0, nlev = 0;
-  DevBuf dorder, dloff, dz, dv, dy;
+  int n = 0, m = 0, nlev = 0;  // m: the largest neighbor count
+  DevBuf dorder, dloff, dsites, dz, dv, dy;
   PinnedBuf hz, hv;
   int init(int n, const int *nnIndx, const int *nnIndxLU);
   double *z() { return hz.as<double>(); }

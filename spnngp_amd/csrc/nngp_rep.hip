@@ -35,6 +35,7 @@ constexpr int kRepThreads = 1024;
 struct RepArgs {
   const double *B, *F, *z, *X, *beta;
   const int *nnIndx, *lu, *order, *loff;
+  const int4 *sites;  // per position of order: {site, offset, count, 0}
   double *v, *y;
   long long bstride;  // doubles between samples' B
   int n, nlev, p, nS, s0;
@@ -73,6 +74,123 @@ __global__ __launch_bounds__(kRepThreads) void k_rep_draw(RepArgs A) {
   }
 }
 
+// k_rep_draw for m <= M with one site a thread per level.  Slot t of
+// level L is sites[loff[L] + t] = {site, its neighbor offset, count}.  The
+// slot of level L + 2 and the rows of level L + 1 (neighbor ids, B row, F
+// and the normal) are requested while level L waits for its neighbors' v,
+// and nothing waits for them before that wait: each level costs one memory
+// round trip and the barrier.  A level with more sites than threads runs
+// its other sites as k_rep_draw does.  Every site's arithmetic is
+// k_rep_draw's, in the same order.
+constexpr int kRepPfThreads = 256;
+
+template <int M>
+struct RepSlot {
+  int i;  // the site, or -1 when the level has no site for this thread
+  int cnt;
+  int nb[M];
+  double b[M];
+  double f, z;  // F_i and the normal
+};
+
+__device__ __forceinline__ int4 rep_site(const RepArgs &A, int L, int t) {
+  int4 q = make_int4(-1, 0, 0, 0);
+  if (L < A.nlev) {
+    const int a = A.loff[L], b = A.loff[L + 1];
+    if (a + t < b) q = A.sites[a + t];
+  }
+  return q;
+}
+
+// Straight-line code (no branch per slot, which would make the compiler
+// wait for these loads where the branches join): slots past the count read
+// slot 0 of the site (or of the index for an empty level slot, q = {-1, 0,
+// 0, 0}) and are then set to 0.
+template <int M>
+__device__ __forceinline__ void rep_rows(const RepArgs &A, const double *B,
+                                         const double *F, const double *z,
+                                         int s, int4 q, RepSlot<M> &r) {
+  r.i = q.x;
+  r.cnt = q.z;
+#pragma unroll
+  for (int j = 0; j < M; j++) {
+    const int jj = j < q.z ? j : 0;
+    const int nb = A.nnIndx[q.y + jj];
+    const double b = B[q.y + jj];
+    r.nb[j] = j < q.z ? nb : 0;
+    r.b[j] = j < q.z ? b : 0.0;
+  }
+  const int ii = q.x < 0 ? 0 : q.x;
+  const double f = F[ii];
+  const double zz =
+      z ? z[ii] : philox_normal(A.seed, (uint32_t)ii, (uint32_t)s);
+  r.f = q.x < 0 ? 0.0 : f;
+  r.z = q.x < 0 ? 0.0 : zz;
+}
+
+template <int M>
+__global__ __launch_bounds__(kRepPfThreads) void k_rep_draw_pf(RepArgs A) {
+  const int g = (int)blockIdx.x;
+  const int n = A.n;
+  const int t = (int)threadIdx.x;
+  const double *B = A.B + (size_t)g * A.bstride;
+  const double *F = A.F + (size_t)g * n;
+  const double *z = A.z ? A.z + (size_t)g * n : nullptr;
+  double *v = A.v + (size_t)g * n;
+  const int s = A.s0 + g;
+  RepSlot<M> cur, nxt;
+  int4 q2 = rep_site(A, 1, t);
+  rep_rows<M>(A, B, F, z, s, rep_site(A, 0, t), cur);
+  for (int L = 0; L < A.nlev; L++) {
+    // this level's neighbor values first, every slot at once (padded slots
+    // read v[0], which no product uses); the loads for the levels ahead go
+    // out behind them, so that the wait for these leaves those in flight
+    double vn[M];
+#pragma unroll
+    for (int j = 0; j < M; j++) vn[j] = v[cur.nb[j]];
+    const int4 q3 = rep_site(A, L + 2, t);
+    rep_rows<M>(A, B, F, z, s, q2, nxt);
+    // past the count b = 0 and the value is replaced by 0, so those terms
+    // add +0 to a sum that is never -0: the sum is k_rep_draw's bit for bit
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < M; j++)
+      acc += cur.b[j] * (j < cur.cnt ? vn[j] : 0.0);
+    const double vi = acc + sqrt(cur.f) * cur.z;
+    if (cur.i >= 0) v[cur.i] = vi;
+    const int a = A.loff[L], b = A.loff[L + 1];
+    for (int k = a + t + (int)blockDim.x; k < b; k += (int)blockDim.x) {
+      const int i = A.order[k];
+      const int off = A.lu[i], cnt = A.lu[n + i];
+      double acc = 0.0;
+      for (int j = 0; j < cnt; j++) acc += B[off + j] * v[A.nnIndx[off + j]];
+      const double zi =
+          z ? z[i] : philox_normal(A.seed, (uint32_t)i, (uint32_t)s);
+      v[i] = acc + sqrt(F[i]) * zi;
+    }
+    __syncthreads();
+    cur = nxt;
+    q2 = q3;
+  }
+  for (int i = t; i < n; i += (int)blockDim.x) {
+    double yi = v[i];
+    for (int j = 0; j < A.p; j++)
+      yi += A.X[(size_t)j * n + i] * A.beta[s + (size_t)j * A.nS];
+    A.y[(size_t)g * n + i] = yi;
+  }
+}
+
+// One block per sample of the group: the prefetching form for m <= 16.
+static void rep_draw_launch(const RepArgs &A, int groups, int m,
+                            hipStream_t st) {
+  if (m <= 16)
+    hipLaunchKernelGGL(k_rep_draw_pf<16>, dim3(groups), dim3(kRepPfThreads),
+                       0, st, A);
+  else
+    hipLaunchKernelGGL(k_rep_draw, dim3(groups), dim3(kRepThreads), 0, st,
+                       A);
+}
+
 // Level schedule of the neighbor DAG: order[] lists the sites level by
 // level (ascending site index within a level), loff[L] .. loff[L+1] is
 // level L.  Fails if a neighbor does not precede its site.
@@ -103,8 +221,25 @@ static int build_levels(int n, const int *nnIndx, const int *lu,
   return NNGP_OK;
 }
 
+// sites[k] = {order[k], its neighbor offset, its count, 0}, on the device.
+static int upload_sites(int n, const int *lu, const std::vector<int> &order,
+                        DevBuf &dsites, hipStream_t st) {
+  std::vector<int4> h((size_t)n);
+  for (int k = 0; k < n; k++) {
+    const int i = order[(size_t)k];
+    h[(size_t)k] = make_int4(i, lu[i], lu[(size_t)n + i], 0);
+  }
+  NNGP_TRY(dsites.alloc(sizeof(int4) * (size_t)n));
+  NNGP_HIP(hipMemcpyAsync(dsites.p, h.data(), dsites.bytes,
+                          hipMemcpyHostToDevice, st));
+  NNGP_HIP(hipStreamSynchronize(st));
+  return NNGP_OK;
+}
+
 int RepDrawer::init(int n_, const int *nnIndx, const int *lu) {
   n = n_;
+  m = 0;
+  for (int i = 0; i < n; i++) m = std::max(m, lu[(size_t)n + i]);
   std::vector<int> order, loff;
   NNGP_TRY(build_levels(n, nnIndx, lu, order, loff));
   nlev = (int)loff.size() - 1;
@@ -114,6 +249,7 @@ int RepDrawer::init(int n_, const int *nnIndx, const int *lu) {
                      hipMemcpyHostToDevice));
   NNGP_HIP(hipMemcpy(dloff.p, loff.data(), sizeof(int) * loff.size(),
                      hipMemcpyHostToDevice));
+  NNGP_TRY(upload_sites(n, lu, order, dsites, nullptr));
   NNGP_TRY(dz.alloc(sizeof(double) * (size_t)n));
   NNGP_TRY(dv.alloc(sizeof(double) * (size_t)n));
   NNGP_TRY(dy.alloc(sizeof(double) * (size_t)n));
@@ -136,6 +272,7 @@ int RepDrawer::draw(const double *B, const double *F, const int *nnIndx,
   A.lu = lu;
   A.order = dorder.as<int>();
   A.loff = dloff.as<int>();
+  A.sites = dsites.as<int4>();
   A.v = dv.as<double>();
   A.y = dy.as<double>();
   A.bstride = 0;
@@ -145,7 +282,7 @@ int RepDrawer::draw(const double *B, const double *F, const int *nnIndx,
   A.nS = 1;
   A.s0 = 0;
   A.seed = 0;
-  hipLaunchKernelGGL(k_rep_draw, dim3(1), dim3(kRepThreads), 0, st, A);
+  rep_draw_launch(A, 1, m, st);
   NNGP_HIP(hipGetLastError());
   NNGP_HIP(hipMemcpyAsync(hv.p, dy.p, sizeof(double) * (size_t)n,
                           hipMemcpyDeviceToHost, st));
@@ -197,7 +334,7 @@ extern "C" int nngp_rNNGPReplicated(
     budget = std::max(budget, 0.5 * (double)free_b);
   int G = (int)std::min<double>(budget / per, 256.0);
   G = std::max(1, std::min(G, nSamples));
-  DevBuf dB, dF, dz, dv, dy, dX, dbeta, dorder, dloff;
+  DevBuf dB, dF, dz, dv, dy, dX, dbeta, dorder, dloff, dsites;
   NNGP_TRY(dB.alloc(sizeof(double) * (size_t)bstride * G));
   NNGP_TRY(dF.alloc(sizeof(double) * (size_t)n * G));
   NNGP_TRY(dv.alloc(sizeof(double) * (size_t)n * G));
@@ -216,6 +353,7 @@ extern "C" int nngp_rNNGPReplicated(
                           hipMemcpyHostToDevice, st));
   NNGP_HIP(hipMemcpyAsync(dloff.p, loff.data(), sizeof(int) * loff.size(),
                           hipMemcpyHostToDevice, st));
+  NNGP_TRY(upload_sites(n, nnIndxLU, order, dsites, st));
   PinnedBuf hz;
   if (rng) NNGP_TRY(hz.alloc(sizeof(double) * (size_t)n * G));
   Rng R(rng, opts ? opts->seed : 1);
@@ -271,6 +409,7 @@ extern "C" int nngp_rNNGPReplicated(
     A.lu = md.lu.as<int>();
     A.order = dorder.as<int>();
     A.loff = dloff.as<int>();
+    A.sites = dsites.as<int4>();
     A.v = dv.as<double>();
     A.y = dy.as<double>();
     A.bstride = bstride;
@@ -280,7 +419,7 @@ extern "C" int nngp_rNNGPReplicated(
     A.nS = nSamples;
     A.s0 = s0;
     A.seed = opts ? opts->seed : 1;
-    hipLaunchKernelGGL(k_rep_draw, dim3(g), dim3(kRepThreads), 0, st, A);
+    rep_draw_launch(A, g, m, st);
     NNGP_HIP(hipGetLastError());
     int herr = INT_MAX;
     NNGP_HIP(hipMemcpyAsync(&herr, md.err.p, sizeof(int),
