@@ -94,6 +94,9 @@ struct SweepArgs {
   // (the per-site lookups of the sweep, laid out in the order it reads them)
   const int4 *meta;
   const int2 *uoc;
+  // per transpose entry t: B of the site uIndx[t] at its neighbor slot that
+  // names the entry's site, gathered from B once per B (k_gather_bt)
+  const double *Bt;
   const int *lu;  // nnIndxLU
   // per site {F_i, r_i}: F of the sweep's B, F slot and the residual
   // w_i - B_i . w_N(i) at the current w, side by side so that a child's two
@@ -189,7 +192,7 @@ __device__ void sweep_site(const SweepArgs &A, int4 mt, int lane) {
       if (j < uc) {
         const int t = uo + j;
         const int jj = A.uIndx[t];
-        const double bij = A.B[A.uoc[t].x + A.uiIndx[t]];
+        const double bij = A.Bt[t];
         const double2 frj = A.FR[jj];
         const double fj = frj.x, rj = frj.y;
         ta[c] = bij * (rj + bij * wi) / fj;
@@ -235,12 +238,24 @@ __device__ void sweep_site(const SweepArgs &A, int4 mt, int lane) {
       } else {
         const int t = uo + j;
         jj = A.uIndx[t];
-        bij = A.B[A.uoc[t].x + A.uiIndx[t]];
+        bij = A.Bt[t];
         rj = A.FR[jj].y;
       }
       A.FR[jj].y = rj - bij * d;
     }
   }
+}
+
+// Bt[t] = B[bpos[t]]: the sweep's B entries in the order it reads them, so
+// that a child's coefficient comes with its index instead of after two
+// index loads.  Once per B (the sampler's B changes when a phi proposal is
+// accepted).
+__global__ __launch_bounds__(kSweepGrid) void k_gather_bt(double *Bt,
+                                                         const double *B,
+                                                         const int *bpos,
+                                                         long long nt) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nt) Bt[t] = B[bpos[t]];
 }
 
 // {F_i, r_i = w_i - B_i . w_N(i)} for every site, before a sweep
@@ -753,7 +768,8 @@ extern "C" int nngp_sNNGP(
                        covModel, opts));
   }
   hipStream_t st = md.st.s;
-  DevBuf dy, dX, du, dui, duoc, dmeta, dR, dphase, dz, dpart, dsum, dwout;
+  DevBuf dy, dX, du, dui, duoc, dmeta, dR, dphase, dz, dpart, dsum, dwout,
+      dbpos, dBt;
   NNGP_TRY(dy.alloc(sizeof(double) * (size_t)n));
   NNGP_TRY(dX.alloc(sizeof(double) * (size_t)n * p));
   NNGP_TRY(du.alloc(sizeof(int) * (size_t)std::max(nIndx, 1LL)));
@@ -779,6 +795,14 @@ extern "C" int nngp_sNNGP(
     huoc[(size_t)t] = make_int2(ok ? nnIndxLU[jj] : 0,
                                 ok ? nnIndxLU[(size_t)n + jj] : 0);
   }
+  // position in B of each transpose entry's coefficient
+  std::vector<int> hbpos((size_t)std::max(ut, 1LL), 0);
+  for (long long t = 0; t < ut; t++)
+    hbpos[(size_t)t] = huoc[(size_t)t].x + uiIndx[t];
+  NNGP_TRY(dbpos.alloc(sizeof(int) * hbpos.size()));
+  NNGP_TRY(dBt.alloc(sizeof(double) * hbpos.size()));
+  NNGP_HIP(hipMemcpyAsync(dbpos.p, hbpos.data(), dbpos.bytes,
+                          hipMemcpyHostToDevice, st));
   std::vector<int4> hmeta((size_t)n);
   for (int k = 0; k < n; k++) {
     const int i = order[(size_t)k];
@@ -904,6 +928,8 @@ extern "C" int nngp_sNNGP(
   PassOut p0;
   NNGP_TRY(md.pass(cov_of(th[0], th[2], th[3]), cur, true, &p0));
   double sigF = th[0], ldF = p0.logDet;
+  int bt_slot = -1;  // the B slot that dBt holds (slot cur is written only
+                     // before the loop; proposals go to 1 - cur)
   for (int s = 0; s < nSamples; s++) {
     // w | rest: n normals in site order (the reference draws them in its
     // sweep, which visits the sites in that order)
@@ -926,6 +952,7 @@ extern "C" int nngp_sNNGP(
     A.phase = dphase.as<int>();
     A.meta = dmeta.as<int4>();
     A.uoc = duoc.as<int2>();
+    A.Bt = dBt.as<double>();
     A.lu = md.lu.as<int>();
     A.FR = dR.as<double2>();
     for (int k = 0; k < kMaxP; k++) A.beta[k] = k < p ? beta[k] : 0.0;
@@ -937,6 +964,14 @@ extern "C" int nngp_sNNGP(
     A.ws = md.zs;
     A.nph = nph;
     A.fscale = th[0] == sigF ? 1.0 : th[0] / sigF;
+    if (bt_slot != cur && ut > 0) {
+      hipLaunchKernelGGL(k_gather_bt,
+                         dim3((unsigned)((ut + kSweepGrid - 1) / kSweepGrid)),
+                         dim3(kSweepGrid), 0, st, dBt.as<double>(), A.B,
+                         dbpos.as<int>(), ut);
+      NNGP_HIP(hipGetLastError());
+      bt_slot = cur;
+    }
     hipLaunchKernelGGL(k_resid_init, dim3((n + kSweepGrid - 1) / kSweepGrid),
                        dim3(kSweepGrid), 0, st, A);
     NNGP_HIP(hipGetLastError());
