@@ -361,9 +361,68 @@ extern "C" int nngp_rNNGPReplicated(
     std::printf("Computing %d replicates of the NNGP response model: %d "
                 "observations, %d neighbors, %d dependency levels.\n",
                 nSamples, n, m, nlev);
+  // With device normals, each group's replicates leave on a copy stream
+  // while the next group computes: two output buffers, used in turn.  A
+  // group's buffer is free again once its copy has returned, which is
+  // before the group after next is launched.  With a caller's generator
+  // the pinned normals of a group must have reached the device before the
+  // next group's are written, so each group is copied (and its stream
+  // drained) before the next one starts.
+  const bool twobuf = nSamples > G && !rng;
+  DevBuf dy2;
+  Stream cs;
+  if (twobuf) {
+    NNGP_TRY(dy2.alloc(sizeof(double) * (size_t)n * G));
+    NNGP_TRY(cs.create());
+  }
+  hipEvent_t done[2] = {nullptr, nullptr};
+  for (hipEvent_t &e : done)
+    NNGP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  struct EventPair {
+    hipEvent_t *e;
+    ~EventPair() {
+      for (int k = 0; k < 2; k++)
+        if (e[k]) (void)hipEventDestroy(e[k]);
+    }
+  } done_guard{done};
+  struct Pending {
+    int s0 = -1, g = 0, buf = 0;
+  } pend;
   int status = 0;
-  for (int s0 = 0; s0 < nSamples; s0 += G) {
+  // copy group `pend` out (after its kernels), then check its failure flag
+  auto copy_out = [&]() -> int {
+    if (pend.s0 < 0) return NNGP_OK;
+    hipStream_t q = twobuf ? cs.s : st;
+    const double *src = pend.buf ? dy2.as<double>() : dy.as<double>();
+    int herr = INT_MAX;
+    NNGP_HIP(hipStreamWaitEvent(q, done[pend.buf], 0));
+    NNGP_HIP(hipMemcpyAsync(&herr, md.err.p, sizeof(int),
+                            hipMemcpyDeviceToHost, q));
+    NNGP_HIP(hipMemcpyAsync(yRep + (size_t)pend.s0 * n, src,
+                            sizeof(double) * (size_t)n * pend.g,
+                            hipMemcpyDeviceToHost, q));
+    NNGP_HIP(hipStreamSynchronize(q));
+    if (herr != INT_MAX) {
+      (void)hipStreamSynchronize(st);  // nothing may run past the return
+      return set_err(NNGP_ERR_NOT_PD,
+                     "c++ Rf_error: dpotrf failed\n (site %d of a sample in "
+                     "%d..%d)",
+                     herr, pend.s0, pend.s0 + pend.g - 1);
+    }
+    status += pend.g;
+    if (verbose && nReport > 0 && status >= nReport) {
+      std::printf("Sampled: %d of %d, %3.2f%%\n", pend.s0 + pend.g,
+                  nSamples, 100.0 * (pend.s0 + pend.g) / nSamples);
+      std::fflush(stdout);
+      status = 0;
+    }
+    pend.s0 = -1;
+    return NNGP_OK;
+  };
+  int gi = 0;
+  for (int s0 = 0; s0 < nSamples; s0 += G, gi++) {
     const int g = std::min(G, nSamples - s0);
+    const int buf = twobuf ? gi % 2 : 0;
     for (int k = 0; k < g; k++) {
       const double *th = theta + (size_t)(s0 + k) * nTheta;
       const CovArgs c = make_cov(covModel, th[0], th[1], th[2],
@@ -411,7 +470,7 @@ extern "C" int nngp_rNNGPReplicated(
     A.loff = dloff.as<int>();
     A.sites = dsites.as<int4>();
     A.v = dv.as<double>();
-    A.y = dy.as<double>();
+    A.y = buf ? dy2.as<double>() : dy.as<double>();
     A.bstride = bstride;
     A.n = n;
     A.nlev = nlev;
@@ -421,26 +480,15 @@ extern "C" int nngp_rNNGPReplicated(
     A.seed = opts ? opts->seed : 1;
     rep_draw_launch(A, g, m, st);
     NNGP_HIP(hipGetLastError());
-    int herr = INT_MAX;
-    NNGP_HIP(hipMemcpyAsync(&herr, md.err.p, sizeof(int),
-                            hipMemcpyDeviceToHost, st));
-    NNGP_HIP(hipMemcpyAsync(yRep + (size_t)s0 * n, dy.p,
-                            sizeof(double) * (size_t)n * g,
-                            hipMemcpyDeviceToHost, st));
-    NNGP_HIP(hipStreamSynchronize(st));
-    if (herr != INT_MAX)
-      return set_err(NNGP_ERR_NOT_PD,
-                     "c++ Rf_error: dpotrf failed\n (site %d of a sample in "
-                     "%d..%d)",
-                     herr, s0, s0 + g - 1);
-    status += g;
-    if (verbose && nReport > 0 && status >= nReport) {
-      std::printf("Sampled: %d of %d, %3.2f%%\n", s0 + g, nSamples,
-                  100.0 * (s0 + g) / nSamples);
-      std::fflush(stdout);
-      status = 0;
-    }
+    if (twobuf) NNGP_TRY(copy_out());  // the previous group, while this one
+                                       // computes
+    NNGP_HIP(hipEventRecord(done[buf], st));
+    pend.s0 = s0;
+    pend.g = g;
+    pend.buf = buf;
+    if (!twobuf) NNGP_TRY(copy_out());  // this group, before the next
   }
+  NNGP_TRY(copy_out());
   if (verbose) {
     std::printf("Sampled: %d of %d, %3.2f%%\n", nSamples, nSamples, 100.0);
     std::fflush(stdout);

@@ -102,6 +102,27 @@ def test_replicates_in_several_groups(oracle, nngp):
         assert rel(got[:, s], want[:, 0]) < RTOL
 
 
+@pytest.mark.gpu
+def test_replicates_in_three_groups_reuse_the_output_buffers(oracle, nngp):
+    """600 samples in groups of 256, 256 and 88: each group's replicates
+    leave while the next one computes, from two device buffers used in
+    turn, so the third group writes the first group's buffer after its
+    copy.  Samples of every group, at both ends, against the oracle."""
+    n, m, p, S, seed = 300, 5, 2, 600, 13
+    print("replicates three groups", flush=True)
+    c, X, beta, th = problem(n, m, p, S, 0, 14)
+    nn = oracle.mkNNIndxCB(c, m)
+    got = nngp.rNNGPReplicated(X, p, n, m, c, 0, nn["nnIndx"],
+                               nn["nnIndxLU"], beta, th, S,
+                               seed=seed)["y.rep.samples"]
+    for s in (0, 255, 256, 511, 512, 599):
+        z = np.array([oracle.lib().or_philox_normal(seed, i, s)
+                      for i in range(n)])
+        want = oracle_rep(oracle, c, X, beta[s:s + 1], th[:, s:s + 1], nn, m,
+                          0, z)
+        assert rel(got[:, s], want[:, 0]) < RTOL
+
+
 def test_replicates_reject_a_neighbor_that_is_not_a_predecessor(oracle,
                                                                  nngp):
     n, m, p, S = 200, 4, 1, 2
