@@ -776,7 +776,9 @@ def main():
                                         p + 1))
                    if (n, m) == (1_000_000, 15) else
                    (None, "other workload")),
-                 algorithmic_bytes=(24 + 4 * m) * n, flops_per_launch=flops,
+                 algorithmic_bytes=(24 + 4 * m) * n,
+                 algorithmic_bytes_fused=(24 + 4 * m + 8 * (p + 1)) * n,
+                 flops_per_launch=flops,
                  launch_ms=mix_ms, launches=mix["tm"]["bf_launches"]),
                  bound="valu",
                  bound_note="FP32 build and factor, FP64 refinement and "
@@ -839,7 +841,11 @@ def main():
             *(traffic("updateBF_chain_n1e6_m15",
                       chain_kernel(f64["tune"]["chosen"], m))
               if (n, m) == (1_000_000, 15) else (None, "other workload")),
-            algorithmic_bytes=(24 + 4 * m) * n, flops_per_launch=flops,
+            algorithmic_bytes=(24 + 4 * m) * n,
+            # the fused pass also reads each site's own [X | y] row for the
+            # cross-products (neighbors' rows are gathers of the same data)
+            algorithmic_bytes_fused=(24 + 4 * m + 8 * (p + 1)) * n,
+            flops_per_launch=flops,
             launch_ms=bf_ms, launches=tm["bf_launches"]),
         # one updateBF and one sum launch per step; the call's initial pass
         # adds one of each outside the loop, so per-launch means
