@@ -56,6 +56,24 @@ def test_replicates_follow_the_callers_stream(oracle, nngp, cov, m):
     assert rel(got, want) < RTOL
 
 
+def test_replicates_follow_the_callers_stream_in_two_groups(oracle, nngp):
+    """300 samples with the caller's generator: two launch groups through
+    one output buffer, each copied out before the next starts, with the
+    normals drawn sample after sample across the group boundary."""
+    n, m, p, S = 300, 6, 2, 300
+    print("rep caller's stream, two groups", flush=True)
+    c, X, beta, th = problem(n, m, p, S, 0, 41)
+    nn = oracle.mkNNIndxCB(c, m)
+    z = np.random.default_rng(2).normal(size=n * S)
+    want = oracle_rep(oracle, c, X, beta, th, nn, m, 0, z)
+    got = nngp.rNNGPReplicated(X, p, n, m, c, 0, nn["nnIndx"],
+                               nn["nnIndxLU"], beta, th, S,
+                               rng=oracle.TapeRng(z, []))["y.rep.samples"]
+    assert got.shape == (n, S)
+    for s in (0, 255, 256, 299):
+        assert rel(got[:, s], want[:, s]) < RTOL
+
+
 def test_replicates_with_the_counter_based_stream(oracle, nngp):
     n, m, p, S, seed = 2500, 8, 3, 4, 77
     c, X, beta, th = problem(n, m, p, S, 0, 5)
