@@ -231,7 +231,10 @@ int nngp_sNNGPPredict(const double *X, const double *y, const double *coords,
  * rng in the reference's order (all n of sample s, then sample s + 1);
  * with rng == NULL they are Philox normals keyed by (opts->seed, site,
  * sample), as nngp_server_submit draws them.  Runs on the device: the
- * draw follows the neighbor DAG level by level, one sample per block. */
+ * draw follows the neighbor DAG level by level, one sample per block, for
+ * groups of up to 256 samples whose B, F and draw buffers (about
+ * 8 (nnIndx entries + 4n) bytes a sample) take at most half the free device
+ * memory (at least 16 GB). */
 int nngp_rNNGPReplicated(const double *X, int p, int n, int m,
                          const double *coords, int covModel,
                          const int *nnIndx, const int *nnIndxLU,
