@@ -339,6 +339,22 @@ def test_set_shape_rejects_shapes_that_do_not_apply(nngp):
         with pytest.raises(nngp.NngpError):
             md.set_shape(1, 96, 0)
         md.set_shape(1, 128, 0)
+    # modes of the other precision: 1 and 3 are the mixed kernel's
+    # covariance copies, 2 the FP64 kernel's staged rows
+    nn15 = nngp.mkNNIndxCB(c, 15)
+    with nngp.Model(c, nn15["nnIndx"], nn15["nnIndxLU"], 15, 0) as md:
+        for mode in (1, 3):
+            with pytest.raises(nngp.NngpError):
+                md.set_shape(0, 64, mode)
+        md.set_shape(0, 64, 2)
+    with nngp.Model(c, nn15["nnIndx"], nn15["nnIndxLU"], 15, 0,
+                    precision=1) as md:
+        with pytest.raises(nngp.NngpError):
+            md.set_shape(0, 64, 2)
+        with pytest.raises(nngp.NngpError):
+            md.set_shape(2, 256, 0)  # the row kernel is FP64 only
+        for mode in (0, 1, 3):
+            md.set_shape(0, 64, mode)
 
 
 @pytest.mark.parametrize("cov,nu", [(0, 0.0), (2, 1.5), (3, 0.0)])
