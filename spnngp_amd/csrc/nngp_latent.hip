@@ -258,6 +258,16 @@ __global__ __launch_bounds__(kSweepGrid) void k_gather_bt(double *Bt,
   if (t < nt) Bt[t] = B[bpos[t]];
 }
 
+// The sweep's counter-based normals for every site at once: the site-order
+// sweep then reads z_i with its other loads instead of computing the
+// Philox draw (ten rounds, a log, a sqrt and a cos) on each level's
+// critical path.  The same values as philox_normal in the sweep.
+__global__ __launch_bounds__(kSweepGrid) void k_philox_fill(
+    double *z, int n, unsigned long long seed, int iter) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n) z[i] = philox_normal(seed, (uint32_t)i, (uint32_t)iter);
+}
+
 // {F_i, r_i = w_i - B_i . w_N(i)} for every site, before a sweep
 __global__ __launch_bounds__(kSweepGrid) void k_resid_init(SweepArgs A) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -781,7 +791,7 @@ extern "C" int nngp_sNNGP(
   NNGP_TRY(dpart.alloc(sizeof(double) * kRedMax * kLatStats));
   NNGP_TRY(dsum.alloc(sizeof(double) * kLatStats));
   NNGP_TRY(dwout.alloc(sizeof(double) * (size_t)n));
-  if (rng) NNGP_TRY(dz.alloc(sizeof(double) * (size_t)n));
+  NNGP_TRY(dz.alloc(sizeof(double) * (size_t)n));  // caller's or Philox
   NNGP_HIP(hipMemcpyAsync(dy.p, y, dy.bytes, hipMemcpyHostToDevice, st));
   NNGP_HIP(hipMemcpyAsync(dX.p, X, dX.bytes, hipMemcpyHostToDevice, st));
   long long ut = 0;
@@ -944,7 +954,7 @@ extern "C" int nngp_sNNGP(
     A.F = md.F[cur].as<double>();
     A.y = dy.as<double>();
     A.X = dX.as<double>();
-    A.z = rng ? dz.as<double>() : nullptr;
+    A.z = dz.as<double>();
     A.w = md.Z.as<double>();
     A.nnIndx = md.nnIndx.as<int>();
     A.uIndx = du.as<int>();
@@ -971,6 +981,12 @@ extern "C" int nngp_sNNGP(
                          dbpos.as<int>(), ut);
       NNGP_HIP(hipGetLastError());
       bt_slot = cur;
+    }
+    if (!rng) {
+      hipLaunchKernelGGL(k_philox_fill, dim3((n + kSweepGrid - 1) / kSweepGrid),
+                         dim3(kSweepGrid), 0, st, dz.as<double>(), n, A.seed,
+                         A.iter);
+      NNGP_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_resid_init, dim3((n + kSweepGrid - 1) / kSweepGrid),
                        dim3(kSweepGrid), 0, st, A);
